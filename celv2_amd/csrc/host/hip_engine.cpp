@@ -1,0 +1,157 @@
+// celv2::HipEngine (include/celv2/hip_engine.h) over the C ABI of include/celv2hip.h.
+#include "celv2/hip_engine.h"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/celv2hip.h"
+#include "celv2/lambda.h"
+
+namespace celv2 {
+
+namespace {
+
+[[noreturn]] void raise(int rc, const char* what) {
+  const std::string msg = std::string(what) + ": " + celv2hip_last_error();
+  if (rc == CELV2_ERR_ARG) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+
+celv2_luma luma_of(const Frame420& f, bool full_depth) {
+  celv2_luma l{};
+  if (full_depth && f.bit_depth > 8 && !f.hy.empty()) {
+    l.y16 = f.hy.data(), l.bit_depth = f.bit_depth;
+  } else {
+    l.y8 = f.y.data(), l.bit_depth = 8;
+  }
+  return l;
+}
+
+BlockDecision block_of(const celv2_block& b) {
+  BlockDecision d;
+  d.w4 = b.w4, d.h4 = b.h4, d.inter = b.inter != 0, d.intra = static_cast<IntraMode>(b.intra);
+  d.mv_row = b.mv_row, d.mv_col = b.mv_col, d.ref = b.ref, d.ref2 = b.ref2;
+  d.mv2_row = b.mv2_row, d.mv2_col = b.mv2_col;
+  return d;
+}
+
+celv2_block record_of(const BlockDecision& d) {
+  celv2_block b{};
+  b.w4 = d.w4, b.h4 = d.h4, b.inter = d.inter ? 1 : 0, b.intra = static_cast<uint8_t>(d.intra);
+  b.mv_row = d.mv_row, b.mv_col = d.mv_col, b.ref = d.ref, b.ref2 = d.ref2;
+  b.mv2_row = d.mv2_row, b.mv2_col = d.mv2_col;
+  return b;
+}
+
+}  // namespace
+
+void sb_record_to_decision(const celv2_sb& rec, SbDecision& d) {
+  d.tree.resize(rec.ntree);
+  for (int k = 0; k < rec.ntree; k++) d.tree[k] = static_cast<Part>(rec.tree[k]);
+  d.blocks.resize(rec.nblocks);
+  for (int k = 0; k < rec.nblocks; k++) d.blocks[k] = block_of(rec.blocks[k]);
+  d.cands.resize(rec.ncands);
+  for (int k = 0; k < rec.ncands; k++) {
+    const celv2_cands& c = rec.cands[k];
+    BlockCandidates& o = d.cands[k];
+    o.inter_cost = c.inter_cost, o.intra_cost = c.intra_cost;
+    o.inter = block_of(c.inter);
+    o.intra = static_cast<IntraMode>(c.intra);
+    for (int r = 0; r < BlockCandidates::kRefs; r++)
+      o.ref_cost[r] = c.ref_cost[r], o.ref_mv_col[r] = c.ref_mv_col[r], o.ref_mv_row[r] = c.ref_mv_row[r];
+  }
+}
+
+void decision_to_sb_record(const SbDecision& d, celv2_sb& rec) {
+  std::memset(&rec, 0, sizeof(rec));
+  if (d.tree.size() > CELV2_SB_CAP || d.blocks.size() > CELV2_SB_CAP || d.cands.size() > CELV2_SB_BLOCKS)
+    throw std::runtime_error("superblock decision exceeds the record");
+  rec.ntree = static_cast<int32_t>(d.tree.size());
+  rec.nblocks = static_cast<int32_t>(d.blocks.size());
+  rec.ncands = static_cast<int32_t>(d.cands.size());
+  for (size_t k = 0; k < d.tree.size(); k++) rec.tree[k] = static_cast<uint8_t>(d.tree[k]);
+  for (size_t k = 0; k < d.blocks.size(); k++) rec.blocks[k] = record_of(d.blocks[k]);
+  for (size_t k = 0; k < d.cands.size(); k++) {
+    const BlockCandidates& c = d.cands[k];
+    celv2_cands& o = rec.cands[k];
+    o.inter_cost = c.inter_cost, o.intra_cost = c.intra_cost;
+    o.inter = record_of(c.inter);
+    o.intra = static_cast<uint8_t>(c.intra);
+    for (int r = 0; r < BlockCandidates::kRefs; r++)
+      o.ref_cost[r] = c.ref_cost[r], o.ref_mv_col[r] = c.ref_mv_col[r], o.ref_mv_row[r] = c.ref_mv_row[r];
+  }
+}
+
+HipEngine::HipEngine(int width, int height, int sb_size, int device)
+    : width_(width), height_(height), sb_cols_((width + 63) / 64), sb_rows_((height + 63) / 64) {
+  if (width <= 0 || height <= 0) throw std::runtime_error("HipEngine: frame size must be positive");
+  if (sb_size != 64) throw std::runtime_error("HipEngine: only 64x64 superblocks are supported");
+  const int rc = celv2hip_create(width, height, sb_size, device, &h_);
+  if (rc != CELV2_OK) throw std::runtime_error(std::string("HipEngine: ") + celv2hip_last_error());
+  char name[256] = {};
+  celv2hip_device_name(h_, name, sizeof(name));
+  name_ = name;
+}
+
+HipEngine::~HipEngine() { celv2hip_destroy(h_); }
+
+void HipEngine::set_row_streams(int n) { celv2hip_set_row_streams(h_, n); }
+
+void HipEngine::prepare(const Frame420& next, long long key) {
+  if (key < 0 || next.width != width_ || next.height != height_) return;
+  const celv2_luma l = luma_of(next, full_depth_);
+  const int rc = celv2hip_prepare(h_, &l, key);
+  if (rc != CELV2_OK) raise(rc, "HipEngine::prepare");
+}
+
+void HipEngine::launch(const Frame420& src, const std::vector<const Frame420*>& refs, int qindex, DecisionMap& dm,
+                       const std::vector<long long>& ref_keys, long long src_key) {
+  if (src.width != width_ || src.height != height_) throw std::runtime_error("HipEngine: source size does not match the engine");
+  for (const Frame420* r : refs)
+    if (r->width != width_ || r->height != height_) throw std::runtime_error("HipEngine: reference size does not match the engine");
+  if (refs.size() > 7) throw std::runtime_error("HipEngine: too many references");
+  if (!ref_keys.empty() && ref_keys.size() != refs.size()) throw std::invalid_argument("HipEngine: one key per reference");
+  const celv2_luma s = luma_of(src, full_depth_);
+  std::vector<celv2_luma> r;
+  for (const Frame420* f : refs) r.push_back(luma_of(*f, full_depth_));
+  const int64_t lam = lambda_ >= 0 ? lambda_ : lambda_q8(qindex);
+  const int rc = celv2hip_launch(h_, &s, src_key, r.data(), ref_keys.empty() ? nullptr : ref_keys.data(),
+                                 static_cast<int>(refs.size()), lam);
+  if (rc != CELV2_OK) raise(rc, "HipEngine::launch");
+  dm.sb_size = 64, dm.sb_cols = sb_cols_, dm.sb_rows = sb_rows_;
+  dm.sbs.assign(static_cast<size_t>(sb_cols_) * sb_rows_, SbDecision{});
+}
+
+void HipEngine::collect(DecisionMap& dm, RowGate& gate) {
+  try {
+    std::vector<int> order(sb_rows_);
+    const int n = celv2hip_row_order(h_, order.data(), sb_rows_);
+    if (n != sb_rows_) throw std::runtime_error("HipEngine::collect: no launch to collect");
+    rows_.resize(sb_cols_);
+    for (int r : order) {
+      const int rc = celv2hip_wait_row(h_, r, rows_.data());
+      if (rc != CELV2_OK) raise(rc, "HipEngine::collect");
+      for (int c = 0; c < sb_cols_; c++) sb_record_to_decision(rows_[c], dm.sbs[static_cast<size_t>(r) * sb_cols_ + c]);
+      gate.publish_one(r);
+    }
+  } catch (...) {
+    gate.fail(std::current_exception());
+    celv2hip_sync(h_);  // returns with no work pending
+    throw;
+  }
+}
+
+}  // namespace celv2
+
+#ifndef CELV2_IN_TREE
+// The mirror build has no AVM: lambda from the stated step model (include/celv2/lambda.h).
+int64_t lambda_q8(int qindex) { return celv2_lambda_q8(qindex); }
+#endif
+
+extern "C" int64_t celv2_lambda_q8(int qindex) {
+  auto step = [](int q) { return (4.0 + q + static_cast<double>(q) * q * q / 9000.0) / 8.0; };
+  const double qstep = step(qindex), pivot = step(120);
+  return std::llround(0.002 * qstep * qstep * std::pow(pivot / qstep, 0.8) * 256.0);
+}

@@ -1,0 +1,134 @@
+// Device-side definitions shared by the decision kernels (gfx950, wave64).
+//
+// Planes live in HBM with a replicated border of kBorder samples on every side, so a window of up
+// to kBorder x kBorder samples at any origin reads exactly what the reference's clamped accessor
+// (Plane::at, src/gpu/intra_common.h:17-21) reads once its origin is clamped into the bordered range
+// (clamp_origin). Quarter-pel phase planes (src/gpu/intra_common.h:22-25) are bordered the same way:
+// their border holds the phase value at the clamped position, as at_q() reads it.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace celv2hip {
+
+constexpr int kBorder = 64;         // >= the widest window read at one origin (a 64x64 block)
+constexpr int kSlack = 256;         // bytes before / after each plane allocation for aligned over-reads
+constexpr int kLevels = 5;          // src/gpu/inter_common.h:209
+constexpr int kPhases = 16;         // src/gpu/inter_common.h:176
+constexpr int kModes = 13;          // src/gpu/intra_common.h:42
+constexpr int kMaxRefs = 7;         // src/gpu/inter_common.h:211
+constexpr int kBlocks = 425;        // src/gpu/sb_blocks.h:10
+constexpr int kSquares = 85;        // src/gpu/sb_blocks.h:9
+constexpr int kRange = 8;           // src/gpu/inter_common.h:200, 205
+constexpr int kPelPenalty = 4;      // src/gpu/inter_common.h:220
+constexpr int kQpelPenalty = 16;    // src/gpu/inter_common.h:305
+constexpr int kSplitBits = 2;       // src/gpu/intra_common.h:33
+constexpr int kIntraOnInter = 32;   // src/gpu/inter_common.h:215
+constexpr int kNewmvBits = 3;       // src/gpu/inter_common.h:210
+constexpr int kMaxCands = 72;       // candidates of the 64x64 block: 64 cells + 8 neighbours
+constexpr int kRegion = 10;         // 10x10 8x8-cells: a superblock's cells and the ring around it
+constexpr int kTasks = 4296;        // (block, candidate) pairs of a superblock
+
+// A bordered plane: p points at sample (0, 0); rows are `stride` bytes apart; samples at
+// y in [-kBorder, h + kBorder), x in [-kBorder, w + kBorder) are readable.
+struct DPlane {
+  uint8_t* p;
+  int stride, w, h;
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Origin of a size-wide window moved into the bordered range; reads through it equal clamped reads
+// (every sample beyond the border repeats the edge it would clamp to).
+__device__ __forceinline__ int clamp_origin(int o, int extent, int size) {
+  return clampi(o, -kBorder, extent + kBorder - size);
+}
+
+// 4 bytes starting at an arbitrary address, from two aligned dword loads (never past the bytes the
+// two dwords cover, which the plane's slack keeps inside the allocation).
+__device__ __forceinline__ uint32_t load4u(const uint8_t* a) {
+  const uintptr_t u = reinterpret_cast<uintptr_t>(a);
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(u & ~uintptr_t(3));
+  const uint32_t lo = q[0], hi = q[1];
+  return __builtin_amdgcn_alignbyte(hi, lo, static_cast<uint32_t>(u & 3));
+}
+// 8 bytes starting at an arbitrary address (three aligned dwords).
+__device__ __forceinline__ void load8u(const uint8_t* a, uint32_t& x0, uint32_t& x1) {
+  const uintptr_t u = reinterpret_cast<uintptr_t>(a);
+  const uint32_t* q = reinterpret_cast<const uint32_t*>(u & ~uintptr_t(3));
+  const uint32_t d0 = q[0], d1 = q[1], d2 = q[2];
+  const uint32_t s = static_cast<uint32_t>(u & 3);
+  x0 = __builtin_amdgcn_alignbyte(d1, d0, s);
+  x1 = __builtin_amdgcn_alignbyte(d2, d1, s);
+}
+
+__device__ __forceinline__ uint32_t sad4(uint32_t a, uint32_t b, uint32_t acc) { return __builtin_amdgcn_sad_u8(a, b, acc); }
+
+// Sum of |4x4 Walsh-Hadamard coefficients| of d (row-major), unnormalised
+// (hadamard4x4, src/gpu/intra_common.h:131-146).
+__device__ __forceinline__ int wht4_abs_sum(const int d[16]) {
+  int r[16];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int s01 = d[4 * i] + d[4 * i + 1], d01 = d[4 * i] - d[4 * i + 1];
+    const int s23 = d[4 * i + 2] + d[4 * i + 3], d23 = d[4 * i + 2] - d[4 * i + 3];
+    r[4 * i] = s01 + s23, r[4 * i + 1] = d01 + d23, r[4 * i + 2] = s01 - s23, r[4 * i + 3] = d01 - d23;
+  }
+  int t = 0;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int s01 = r[c] + r[4 + c], d01 = r[c] - r[4 + c], s23 = r[8 + c] + r[12 + c], d23 = r[8 + c] - r[12 + c];
+    t += abs(s01 + s23) + abs(d01 + d23) + abs(s01 - s23) + abs(d01 - d23);
+  }
+  return t;
+}
+
+// SATD of one 4x4 tile from four rows of packed bytes: cur c[i] against prediction p[i].
+__device__ __forceinline__ int satd4x4_rows(const uint32_t c[4], const uint32_t p[4]) {
+  int d[16];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) d[4 * i + j] = static_cast<int>((c[i] >> (8 * j)) & 255) - static_cast<int>((p[i] >> (8 * j)) & 255);
+  return wht4_abs_sum(d);
+}
+
+// Quarter-pel MVs are packed as (int16 mvx) | (int16 mvy) << 16.
+__device__ __forceinline__ uint32_t pack_mv(int x, int y) { return (static_cast<uint32_t>(x) & 0xffffu) | (static_cast<uint32_t>(y) << 16); }
+__device__ __forceinline__ int mv_x(uint32_t m) { return static_cast<int16_t>(m & 0xffffu); }
+__device__ __forceinline__ int mv_y(uint32_t m) { return static_cast<int16_t>(m >> 16); }
+
+// Candidate-block geometry (src/gpu/sb_blocks.h:172-187).
+__device__ __forceinline__ int level_first(int level) { return ((1 << (2 * level)) - 1) / 3; }
+__device__ __forceinline__ void block_geom(int b, int& x, int& y, int& w, int& h) {
+  const int sq = b / 5, kind = b - 5 * (b / 5);
+  const int level = sq >= 21 ? 3 : sq >= 5 ? 2 : sq >= 1 ? 1 : 0;
+  const int i = sq - level_first(level), n = 1 << level, size = 64 >> level;
+  const int sx = (i & (n - 1)) * size, sy = (i >> level) * size;
+  x = sx, y = sy, w = size, h = size;
+  if (kind == 1 || kind == 2) h = size / 2, y = sy + (kind == 2 ? size / 2 : 0);
+  if (kind == 3 || kind == 4) w = size / 2, x = sx + (kind == 4 ? size / 2 : 0);
+}
+
+__device__ __forceinline__ int mode_bits(int m) { return m == 0 ? 1 : (m < 4 ? 3 : 9); }  // intra_common.h:32
+
+// Wave-wide minimum of a 32-bit key (all 64 lanes take part).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const uint32_t w = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 64));
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+// XCD-aware workgroup order: consecutive logical workgroups (neighbouring blocks, which read
+// overlapping reference windows) land on the same XCD and share its L2. The hardware deals
+// workgroups round-robin over the 8 XCDs; the grid is padded to a multiple of 8 and a logical index
+// >= n means "no work".
+__device__ __forceinline__ int xcd_logical(int bid, int n) {
+  const int per = (n + 7) >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+
+}  // namespace celv2hip

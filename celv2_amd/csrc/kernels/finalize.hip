@@ -1,0 +1,252 @@
+// Per superblock: each candidate block's reference choice and compound (best_both,
+// src/gpu/kernels_ref.cpp:104-131), its intra choice, its record (BlockCandidates), the bottom-up
+// partition decision (decide_square, src/gpu/kernels_ref.cpp:178-251) and the exported superblock
+// record (celv2_sb, include/celv2hip.h).
+//
+// One workgroup per superblock. Compound SATDs are tiled: lanes take 4x4 tiles from a static list of
+// all 425 blocks' tiles and add into per-block LDS sums (integer atomics, order-free); the partition
+// DP runs level by level with one thread per square, and one thread walks the chosen tree.
+#include "../../../include/celv2hip.h"
+#include "common.h"
+#include "launch.h"
+
+namespace celv2hip {
+
+namespace {
+
+constexpr int kAllTiles = 5120;  // 4x4 tiles of all 425 blocks (256 per kind and level)
+constexpr int64_t kMax = INT64_MAX;
+
+__device__ uint32_t g_all_tiles[kAllTiles];  // b | tx << 16 | ty << 24
+
+__device__ __forceinline__ bool pair_allowed(int a, int b, int n) {  // compound_pair_ok, inter_common.h:364-367
+  const int lo = a < b ? a : b, lim = n - 2 < 2 ? n - 2 : 2;
+  return a != b && lo <= lim;
+}
+
+// The compound prediction's 4x4 SATD: predictions averaged (p0 + p1 + 1) >> 1
+// (block_compound_satd, src/gpu/inter_common.h:370-385).
+__device__ __forceinline__ int satd4x4_avg(const uint32_t c[4], const uint32_t p[4], const uint32_t q[4]) {
+  int d[16];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int a = static_cast<int>((p[i] >> (8 * j)) & 255), b = static_cast<int>((q[i] >> (8 * j)) & 255);
+      d[4 * i + j] = static_cast<int>((c[i] >> (8 * j)) & 255) - ((a + b + 1) >> 1);
+    }
+  return wht4_abs_sum(d);
+}
+
+__device__ __forceinline__ void tile_rows(const DPlane& pl, const uint8_t* plane, int x, int y, uint32_t out[4]) {
+  const int X = clamp_origin(x, pl.w, 4), Y = clamp_origin(y, pl.h, 4);
+#pragma unroll
+  for (int i = 0; i < 4; i++) out[i] = load4u(plane + static_cast<ptrdiff_t>(Y + i) * pl.stride + X);
+}
+
+__global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
+  __shared__ int64_t rc[kMaxRefs][kBlocks];    // per-reference inter cost
+  __shared__ uint32_t rmv[kMaxRefs][kBlocks];  // per-reference MV (quarter pel, packed)
+  __shared__ int8_t ra[kBlocks], rs[kBlocks];  // best and second reference (-1: none)
+  __shared__ uint32_t comp[kBlocks];           // compound SATD (blocks with an allowed pair)
+  __shared__ int64_t bcost[kBlocks];           // the block's choice's cost
+  __shared__ uint8_t bmode[kBlocks];           // its intra mode, kModes = inter
+  __shared__ uint8_t bref[kBlocks], bref2[kBlocks];
+  __shared__ uint32_t bmv[kBlocks], bmv2[kBlocks];  // the inter choice's MVs (quarter pel, packed)
+  __shared__ int64_t sq_cost[kSquares];
+  __shared__ uint8_t sq_choice[kSquares];
+  __shared__ uint8_t tree[CELV2_SB_CAP];
+  __shared__ uint16_t leaf[CELV2_SB_CAP];
+  __shared__ int counts[2];
+
+  const int tid = threadIdx.x;
+  const int sbx = blockIdx.x, sby = a.row0 + blockIdx.y;
+  const size_t sb = static_cast<size_t>(sby) * a.sb_cols + sbx;
+  const int ox = 64 * sbx, oy = 64 * sby, n = a.nrefs;
+  const int64_t lam = a.lam;
+  celv2_sb* const out = static_cast<celv2_sb*>(a.out) + sb;
+
+  // Per block: each reference's best (cost and MV), the best and the runner-up reference.
+  for (int b = tid; b < kBlocks; b += 256) {
+    int best = -1, second = -1;
+    for (int r = 0; r < n; r++) {
+      const uint2 e = a.best[(sb * kMaxRefs + r) * kBlocks + b];
+      const int64_t c = static_cast<int64_t>(e.x) * 256 + lam * (kNewmvBits + r);
+      rc[r][b] = c, rmv[r][b] = e.y;
+      if (best < 0 || c < rc[best][b]) best = r;
+    }
+    for (int r = 0; r < n; r++)
+      if (r != best && (second < 0 || rc[r][b] < rc[second][b])) second = r;
+    ra[b] = static_cast<int8_t>(best), rs[b] = static_cast<int8_t>(second);
+    comp[b] = 0;
+  }
+  __syncthreads();
+
+  // Compound SATDs of the blocks whose two best references form an allowed pair.
+  if (n >= 2) {
+    for (int slot = tid; slot < kAllTiles; slot += 256) {
+      const uint32_t e = g_all_tiles[slot];
+      const int b = static_cast<int>(e & 0xffff), tx = static_cast<int>((e >> 16) & 255), ty = static_cast<int>(e >> 24);
+      const int p = ra[b], q = rs[b];
+      if (q < 0 || !pair_allowed(p, q, n)) continue;
+      int x, y, w, h;
+      block_geom(b, x, y, w, h);
+      const int px = ox + x + 4 * tx, py = oy + y + 4 * ty;
+      uint32_t c[4], pp[4], qq[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) c[i] = *reinterpret_cast<const uint32_t*>(a.src.p + static_cast<ptrdiff_t>(py + i) * a.src.stride + px);
+      const uint32_t mp = rmv[p][b], mq = rmv[q][b];
+      const DPlane& rp = a.ref[p];
+      const DPlane& rq = a.ref[q];
+      tile_rows(rp, rp.p + a.phase_bytes * ((mv_y(mp) & 3) * 4 + (mv_x(mp) & 3)), px + (mv_x(mp) >> 2), py + (mv_y(mp) >> 2), pp);
+      tile_rows(rq, rq.p + a.phase_bytes * ((mv_y(mq) & 3) * 4 + (mv_x(mq) & 3)), px + (mv_x(mq) >> 2), py + (mv_y(mq) >> 2), qq);
+      atomicAdd(&comp[b], static_cast<uint32_t>(satd4x4_avg(c, pp, qq)));
+    }
+  }
+  __syncthreads();
+
+  // Each block's choices, its candidate record, and its cost for the DP.
+  for (int b = tid; b < kBlocks; b += 256) {
+    int x, y, w, h;
+    block_geom(b, x, y, w, h);
+    const int p = ra[b], q = rs[b];
+    int64_t ci = kMax;
+    uint32_t m1 = 0, m2 = 0;
+    int r1 = 0, r2 = CELV2_SINGLE_REF;
+    if (p >= 0) ci = rc[p][b], m1 = rmv[p][b], r1 = p;
+    if (q >= 0 && pair_allowed(p, q, n)) {
+      const int64_t cc = static_cast<int64_t>(comp[b]) * 256 + lam * (20 + p + q);
+      if (cc < ci) ci = cc, r2 = q, m2 = rmv[q][b];
+    }
+    int64_t c = kMax;
+    int m = 0;
+    if (b >= 5) {  // blocks above 32x32 are inter only (intra_common.h:39)
+      const uint32_t* is = a.intra + (sb * kBlocks + b) * kModes;
+      for (int k = 0; k < kModes; k++) {
+        const int64_t ck = static_cast<int64_t>(is[k]) * 256 + lam * (mode_bits(k) + (n > 0 ? kIntraOnInter : 0));
+        if (ck < c) c = ck, m = k;
+      }
+    }
+    celv2_cands rec;
+    rec.inter_cost = ci, rec.intra_cost = c;
+    rec.inter.w4 = static_cast<uint8_t>(w / 4), rec.inter.h4 = static_cast<uint8_t>(h / 4), rec.inter.inter = 1, rec.inter.intra = 0;
+    rec.inter.mv_col = static_cast<int16_t>(2 * mv_x(m1)), rec.inter.mv_row = static_cast<int16_t>(2 * mv_y(m1));
+    rec.inter.ref = static_cast<uint8_t>(r1), rec.inter.ref2 = static_cast<uint8_t>(r2);
+    rec.inter.mv2_col = static_cast<int16_t>(2 * mv_x(m2)), rec.inter.mv2_row = static_cast<int16_t>(2 * mv_y(m2));
+    rec.intra = static_cast<uint8_t>(m), rec.pad_ = 0;
+    for (int r = 0; r < CELV2_MAX_REFS; r++) {
+      const bool has = r < n;
+      const int64_t v = has ? (rc[r][b] >> 8) : 0;
+      rec.ref_cost[r] = has ? static_cast<uint32_t>(v < 0xfffffffeLL ? v : 0xfffffffeLL) : 0xffffffffu;
+      rec.ref_mv_col[r] = static_cast<int16_t>(has ? 2 * mv_x(rmv[r][b]) : 0);
+      rec.ref_mv_row[r] = static_cast<int16_t>(has ? 2 * mv_y(rmv[r][b]) : 0);
+    }
+    out->cands[b] = rec;
+    const bool inter = ci < c;  // inter only when strictly cheaper (kernels_ref.cpp:145)
+    bcost[b] = inter ? ci : c;
+    bmode[b] = static_cast<uint8_t>(inter ? kModes : m);
+    bref[b] = static_cast<uint8_t>(r1), bref2[b] = static_cast<uint8_t>(r2);
+    bmv[b] = m1, bmv2[b] = m2;
+  }
+  __syncthreads();
+
+  // Partition DP, 8x8 squares up to the 64x64: NONE, HORZ, VERT, then SPLIT, first strict minimum.
+  const int64_t part = lam * kSplitBits;
+  for (int level = 3; level >= 0; level--) {
+    const int cnt = 1 << (2 * level);
+    if (tid < cnt) {
+      const int sq = level_first(level) + tid, b = 5 * sq;
+      int64_t best = kMax;
+      int choice = 3;
+      if ((64 >> level) <= 32 || n > 0) {
+        const int64_t ch = part + bcost[b + 1] + bcost[b + 2], cv = part + bcost[b + 3] + bcost[b + 4];
+        best = bcost[b], choice = 0;
+        if (ch < best) best = ch, choice = 1;
+        if (cv < best) best = cv, choice = 2;
+      }
+      if (level < 3) {
+        const int nn = 1 << level, qx = tid % nn, qy = tid / nn;
+        const int child = level_first(level + 1) + (2 * qy) * (2 * nn) + 2 * qx;
+        const int64_t cs = part + sq_cost[child] + sq_cost[child + 1] + sq_cost[child + 2 * nn] + sq_cost[child + 2 * nn + 1];
+        if (cs < best) best = cs, choice = 3;
+      }
+      sq_cost[sq] = best, sq_choice[sq] = static_cast<uint8_t>(choice);
+    }
+    __syncthreads();
+  }
+
+  // Preorder walk of the chosen tree (one thread): tree entries and the leaves' blocks.
+  if (tid == 0) {
+    int st_level[16], st_qx[16], st_qy[16], sp = 1, nt = 0, nl = 0;
+    st_level[0] = 0, st_qx[0] = 0, st_qy[0] = 0;
+    while (sp > 0) {
+      sp--;
+      const int level = st_level[sp], qx = st_qx[sp], qy = st_qy[sp];
+      const int sq = level_first(level) + (qy << level) + qx, b = 5 * sq, ch = sq_choice[sq];
+      if (ch == 0) {
+        tree[nt++] = 0, leaf[nl++] = static_cast<uint16_t>(b);
+      } else if (ch == 1) {
+        tree[nt++] = 1, tree[nt++] = 0, tree[nt++] = 0;
+        leaf[nl++] = static_cast<uint16_t>(b + 1), leaf[nl++] = static_cast<uint16_t>(b + 2);
+      } else if (ch == 2) {
+        tree[nt++] = 2, tree[nt++] = 0, tree[nt++] = 0;
+        leaf[nl++] = static_cast<uint16_t>(b + 3), leaf[nl++] = static_cast<uint16_t>(b + 4);
+      } else {
+        tree[nt++] = 3;
+        for (int q = 3; q >= 0; q--) st_level[sp] = level + 1, st_qx[sp] = 2 * qx + (q & 1), st_qy[sp] = 2 * qy + (q >> 1), sp++;
+      }
+    }
+    counts[0] = nt, counts[1] = nl;
+  }
+  __syncthreads();
+  const int nt = counts[0], nl = counts[1];
+  if (tid == 0) out->ntree = nt, out->nblocks = nl, out->ncands = kBlocks, out->pad_ = 0;
+  for (int i = tid; i < CELV2_SB_CAP; i += 256) out->tree[i] = i < nt ? tree[i] : 0;
+  for (int i = tid; i < CELV2_SB_CAP; i += 256) {
+    celv2_block d = {};
+    if (i < nl) {
+      const int b = leaf[i];
+      int x, y, w, h;
+      block_geom(b, x, y, w, h);
+      const bool inter = bmode[b] == kModes;
+      d.w4 = static_cast<uint8_t>(w / 4), d.h4 = static_cast<uint8_t>(h / 4);
+      d.inter = inter ? 1 : 0, d.intra = inter ? 0 : bmode[b];
+      d.ref2 = CELV2_SINGLE_REF;
+      if (inter) {
+        d.mv_col = static_cast<int16_t>(2 * mv_x(bmv[b])), d.mv_row = static_cast<int16_t>(2 * mv_y(bmv[b]));
+        d.ref = bref[b];
+        if (bref2[b] != CELV2_SINGLE_REF)
+          d.ref2 = bref2[b], d.mv2_col = static_cast<int16_t>(2 * mv_x(bmv2[b])), d.mv2_row = static_cast<int16_t>(2 * mv_y(bmv2[b]));
+      }
+    }
+    out->blocks[i] = d;
+  }
+  if (tid == 0 && a.counter) {
+    unsigned long long k = 0;
+    for (int b = 0; b < kBlocks; b++) k += (rs[b] >= 0 && pair_allowed(ra[b], rs[b], n)) ? 1 : 0;
+    atomicAdd(a.counter, k);
+  }
+}
+
+}  // namespace
+
+void launch_finalize(const FinalizeArgs& a, hipStream_t s) {
+  if (a.nrows <= 0) return;
+  k_finalize<<<dim3(a.sb_cols, a.nrows), 256, 0, s>>>(a);
+}
+
+void upload_final_tiles(hipStream_t s) {
+  static uint32_t table[kAllTiles];
+  int n = 0;
+  for (int b = 0; b < kBlocks; b++) {
+    const int sq = b / 5, kind = b % 5;
+    const int level = sq >= 21 ? 3 : sq >= 5 ? 2 : sq >= 1 ? 1 : 0;
+    const int size = 64 >> level;
+    const int w = (kind == 3 || kind == 4) ? size / 2 : size, h = (kind == 1 || kind == 2) ? size / 2 : size;
+    for (int ty = 0; ty < h / 4; ty++)
+      for (int tx = 0; tx < w / 4; tx++) table[n++] = static_cast<uint32_t>(b) | (static_cast<uint32_t>(tx) << 16) | (static_cast<uint32_t>(ty) << 24);
+  }
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_all_tiles), table, sizeof(table), 0, hipMemcpyHostToDevice, s);
+}
+
+}  // namespace celv2hip

@@ -1,0 +1,237 @@
+// Single-reference inter scores of a superblock: for each of the 425 candidate blocks, the first
+// least SATD over its MV candidates (best_both's per-reference loop, src/gpu/kernels_ref.cpp:104-119)
+// and that MV.
+//
+// The candidates of all 425 blocks are MVs of the 10 x 10 cells of the level-0 field around the
+// superblock (its 64 cells and the ring, src/gpu/inter_common.h:225-243), and every candidate block's
+// 4x4 tiles lie on the superblock's 4x4 grid. So per distinct MV the kernel scores the whole 64x64
+// area once - 256 tile SATDs - and sums tiles up the block quadtree; each block's cost for any
+// candidate is then a lookup. A workgroup takes one (superblock, reference); its 4 waves take the
+// distinct MVs in turn. Within a block, candidate k's cost is (SATD << 7 | k) and the minimum of
+// those keys (LDS atomicMin, order-free) is the first strict minimum in candidate order, exactly the
+// reference's choice (a repeated MV costs the same as its first occurrence and never wins; the
+// lambda * (3 + r) term is the same for every candidate of a reference and cannot change the order).
+#include "common.h"
+#include "launch.h"
+
+namespace celv2hip {
+
+namespace {
+
+__device__ uint16_t g_cell_first[kRegion * kRegion + 1];  // tasks of region cell c: [first[c], first[c + 1])
+__device__ uint16_t g_cell_tasks[kTasks];                 // b << 7 | k
+__device__ uint8_t g_cand_cell[kBlocks * kMaxCands];     // region cell of candidate k of block b
+
+__device__ __forceinline__ uint32_t xor_add(uint32_t v, int m) {
+  return v + static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m, 64));
+}
+
+__global__ void __launch_bounds__(256) k_inter(InterArgs a) {
+  __shared__ uint32_t mv[kRegion * kRegion];
+  __shared__ int cuid[kRegion * kRegion];
+  __shared__ int urep[kRegion * kRegion];
+  __shared__ uint32_t best[kBlocks];
+  __shared__ uint32_t sc[4][kBlocks];
+  __shared__ int nuniq;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nwg = a.nrefs * a.nrows * a.sb_cols;
+  const int lg = xcd_logical(blockIdx.x, nwg);
+  if (lg >= nwg) return;  // whole workgroup
+  const int r = lg / (a.nrows * a.sb_cols), rem = lg - r * a.nrows * a.sb_cols;
+  const int sby = a.row0 + rem / a.sb_cols, sbx = rem % a.sb_cols;
+  const int ox = 64 * sbx, oy = 64 * sby;
+  const uint32_t* field = a.field[r];
+
+  if (tid < kRegion * kRegion) {
+    const int cx = clampi(8 * sbx - 1 + tid % kRegion, 0, a.cols - 1), cy = clampi(8 * sby - 1 + tid / kRegion, 0, a.rows - 1);
+    mv[tid] = field[cy * a.cols + cx];
+  }
+  for (int b = tid; b < kBlocks; b += 256) best[b] = 0xffffffffu;
+  __syncthreads();
+  // Distinct MVs, numbered in cell order (wave 0); each cell's class is its first cell with that MV.
+  const int c0 = lane, c1 = lane + 64;
+  int rep0 = c0, rep1 = c1;
+  if (wave == 0) {
+    const uint32_t m0 = mv[c0], m1 = c1 < 100 ? mv[c1] : 0u;
+    for (int c = 0; c < c0; c++)
+      if (mv[c] == m0) {
+        rep0 = c;
+        break;
+      }
+    if (c1 < 100)
+      for (int c = 0; c < c1; c++)
+        if (mv[c] == m1) {
+          rep1 = c;
+          break;
+        }
+    const uint64_t b0 = __ballot(rep0 == c0), b1 = __ballot(c1 < 100 && rep1 == c1);
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (rep0 == c0) {
+      const int u = __popcll(b0 & below);
+      cuid[c0] = u, urep[u] = c0;
+    }
+    if (c1 < 100 && rep1 == c1) {
+      const int u = __popcll(b0) + __popcll(b1 & below);
+      cuid[c1] = u, urep[u] = c1;
+    }
+    if (lane == 0) nuniq = __popcll(b0) + __popcll(b1);
+  }
+  __syncthreads();
+  if (wave == 0) {
+    if (rep0 != c0) cuid[c0] = cuid[rep0];
+    if (c1 < 100 && rep1 != c1) cuid[c1] = cuid[rep1];
+  }
+  __syncthreads();
+
+  // This lane's 8x8 square of the superblock: rows as two words.
+  const int sqx = lane & 7, sqy = lane >> 3;
+  uint32_t cw[8][2];
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const uint32_t* q = reinterpret_cast<const uint32_t*>(a.src.p + static_cast<ptrdiff_t>(oy + 8 * sqy + i) * a.src.stride + ox + 8 * sqx);
+    cw[i][0] = q[0], cw[i][1] = q[1];
+  }
+  const DPlane ref = a.ref[r];
+  const int U = nuniq;
+  uint32_t* const mysc = sc[wave];
+  for (int u0 = 0; u0 < U; u0 += 4) {
+    const int u = u0 + wave;
+    if (u < U) {
+      const uint32_t m = mv[urep[u]];
+      const int mx = mv_x(m), my = mv_y(m);
+      const uint8_t* plane = ref.p + a.phase_bytes * ((my & 3) * 4 + (mx & 3));
+      const int X = clamp_origin(ox + (mx >> 2), ref.w, 64), Y = clamp_origin(oy + (my >> 2), ref.h, 64);
+      uint32_t pw[8][2];
+#pragma unroll
+      for (int i = 0; i < 8; i++) load8u(plane + static_cast<ptrdiff_t>(Y + 8 * sqy + i) * ref.stride + X + 8 * sqx, pw[i][0], pw[i][1]);
+      uint32_t t[2][2];
+#pragma unroll
+      for (int ty = 0; ty < 2; ty++)
+#pragma unroll
+        for (int tx = 0; tx < 2; tx++) {
+          const uint32_t c4[4] = {cw[4 * ty][tx], cw[4 * ty + 1][tx], cw[4 * ty + 2][tx], cw[4 * ty + 3][tx]};
+          const uint32_t p4[4] = {pw[4 * ty][tx], pw[4 * ty + 1][tx], pw[4 * ty + 2][tx], pw[4 * ty + 3][tx]};
+          t[ty][tx] = static_cast<uint32_t>(satd4x4_rows(c4, p4));
+        }
+      // Level 3 (8x8 and its halves): this lane's square.
+      const int b3 = (21 + 8 * sqy + sqx) * 5;
+      const uint32_t W = t[0][0] + t[0][1] + t[1][0] + t[1][1];
+      mysc[b3] = W, mysc[b3 + 1] = t[0][0] + t[0][1], mysc[b3 + 2] = t[1][0] + t[1][1];
+      mysc[b3 + 3] = t[0][0] + t[1][0], mysc[b3 + 4] = t[0][1] + t[1][1];
+      // Level 2 (16x16): lanes (2a..2a+1, 2b..2b+1).
+      const uint32_t hp = xor_add(W, 1), vp = xor_add(W, 8), quad = xor_add(hp, 8);
+      const int b2 = (5 + 4 * (sqy >> 1) + (sqx >> 1)) * 5;
+      if (!(sqx & 1) && !(sqy & 1)) mysc[b2] = quad, mysc[b2 + 1] = hp, mysc[b2 + 3] = vp;
+      if (!(sqx & 1) && (sqy & 1)) mysc[b2 + 2] = hp;
+      if ((sqx & 1) && !(sqy & 1)) mysc[b2 + 4] = vp;
+      // Level 1 (32x32).
+      const uint32_t hp2 = xor_add(quad, 2), vp2 = xor_add(quad, 16), quad2 = xor_add(hp2, 16);
+      const int b1 = (1 + 2 * (sqy >> 2) + (sqx >> 2)) * 5;
+      if (!(sqx & 3) && !(sqy & 3)) mysc[b1] = quad2, mysc[b1 + 1] = hp2, mysc[b1 + 3] = vp2;
+      if (!(sqx & 3) && (sqy & 3) == 2) mysc[b1 + 2] = hp2;
+      if ((sqx & 3) == 2 && !(sqy & 3)) mysc[b1 + 4] = vp2;
+      // Level 0 (64x64).
+      const uint32_t hp1 = xor_add(quad2, 4), vp1 = xor_add(quad2, 32), quad1 = xor_add(hp1, 32);
+      if (lane == 0) mysc[0] = quad1, mysc[1] = hp1, mysc[3] = vp1;
+      if (lane == 32) mysc[2] = hp1;
+      if (lane == 4) mysc[4] = vp1;
+    }
+    __syncthreads();
+    if (u < U) {
+      // Every (block, candidate) whose cell has this MV.
+      const uint64_t in0 = __ballot(cuid[lane] == u), in1 = __ballot(lane + 64 < 100 && cuid[min(lane + 64, 99)] == u);
+      for (int half = 0; half < 2; half++) {
+        uint64_t bits = half ? in1 : in0;
+        while (bits) {
+          const int c = __ffsll(static_cast<long long>(bits)) - 1 + 64 * half;
+          bits &= bits - 1;
+          const int f0 = g_cell_first[c], f1 = g_cell_first[c + 1];
+          for (int i = f0 + lane; i < f1; i += 64) {
+            const uint32_t e = g_cell_tasks[i];
+            const int b = static_cast<int>(e >> 7);
+            atomicMin(&best[b], (mysc[b] << 7) | (e & 127u));
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  uint2* out = a.best + ((static_cast<size_t>(sby) * a.sb_cols + sbx) * kMaxRefs + r) * kBlocks;
+  for (int b = tid; b < kBlocks; b += 256) {
+    const uint32_t key = best[b];
+    const int cell = g_cand_cell[b * kMaxCands + (key & 127u)];
+    out[b] = make_uint2(key >> 7, mv[cell]);
+  }
+  if (tid == 0 && a.counter) atomicAdd(a.counter, static_cast<unsigned long long>(U));
+}
+
+}  // namespace
+
+void launch_inter(const InterArgs& a, hipStream_t s) {
+  const int nwg = a.nrefs * a.nrows * a.sb_cols;
+  if (nwg <= 0) return;
+  k_inter<<<((nwg + 7) / 8) * 8, 256, 0, s>>>(a);
+}
+
+// Candidate k of block b as a cell of the 10 x 10 region around the superblock (region column / row
+// = superblock cell + 1): the block's cells in raster order, then left, above, right, below,
+// above-left, above-right, below-left, below-right (candidate_mv_index, src/gpu/inter_common.h:230-243,
+// before its clamping, which the region's clamped load reproduces).
+static int region_cell(int b, int k) {
+  const int sq = b / 5, kind = b % 5;
+  const int level = sq >= 21 ? 3 : sq >= 5 ? 2 : sq >= 1 ? 1 : 0;
+  const int first = ((1 << (2 * level)) - 1) / 3, i = sq - first, n = 1 << level, size = 64 >> level;
+  int x = (i % n) * size, y = (i / n) * size, w = size, h = size;
+  if (kind == 1 || kind == 2) h = size / 2, y += kind == 2 ? size / 2 : 0;
+  if (kind == 3 || kind == 4) w = size / 2, x += kind == 4 ? size / 2 : 0;
+  const int across = (x + w - 1) / 8 - x / 8 + 1, down = (y + h - 1) / 8 - y / 8 + 1, cells = across * down;
+  int cx, cy;
+  if (k < cells) {
+    cx = x / 8 + k % across, cy = y / 8 + k / across;
+  } else {
+    const int m = k - cells, left = x / 8 - 1, right = (x + w) / 8, top = y / 8 - 1, bottom = (y + h) / 8;
+    static const int hx[8] = {-1, 0, 1, 0, -1, 1, -1, 1}, vy[8] = {0, -1, 0, 1, -1, -1, 1, 1};
+    cx = hx[m] < 0 ? left : hx[m] > 0 ? right : x / 8;
+    cy = vy[m] < 0 ? top : vy[m] > 0 ? bottom : y / 8;
+  }
+  return (cy + 1) * kRegion + (cx + 1);
+}
+
+static int cand_total(int b) {
+  const int sq = b / 5, kind = b % 5;
+  const int level = sq >= 21 ? 3 : sq >= 5 ? 2 : sq >= 1 ? 1 : 0;
+  const int first = ((1 << (2 * level)) - 1) / 3, i = sq - first, n = 1 << level, size = 64 >> level;
+  int x = (i % n) * size, y = (i / n) * size, w = size, h = size;
+  if (kind == 1 || kind == 2) h = size / 2, y += kind == 2 ? size / 2 : 0;
+  if (kind == 3 || kind == 4) w = size / 2, x += kind == 4 ? size / 2 : 0;
+  return ((x + w - 1) / 8 - x / 8 + 1) * ((y + h - 1) / 8 - y / 8 + 1) + 8;
+}
+
+void upload_inter_tables(hipStream_t s) {
+  static uint16_t first[kRegion * kRegion + 1], tasks[kTasks];
+  static uint8_t cand[kBlocks * kMaxCands];
+  int count[kRegion * kRegion] = {};
+  int total = 0;
+  for (int b = 0; b < kBlocks; b++)
+    for (int k = 0; k < cand_total(b); k++) {
+      const int c = region_cell(b, k);
+      cand[b * kMaxCands + k] = static_cast<uint8_t>(c);
+      count[c]++, total++;
+    }
+  int off = 0;
+  for (int c = 0; c < kRegion * kRegion; c++) first[c] = static_cast<uint16_t>(off), off += count[c];
+  first[kRegion * kRegion] = static_cast<uint16_t>(off);
+  int fill[kRegion * kRegion] = {};
+  for (int b = 0; b < kBlocks; b++)
+    for (int k = 0; k < cand_total(b); k++) {
+      const int c = region_cell(b, k);
+      tasks[first[c] + fill[c]++] = static_cast<uint16_t>((b << 7) | k);
+    }
+  (void)total;
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cell_first), first, sizeof(first), 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cell_tasks), tasks, sizeof(tasks), 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cand_cell), cand, sizeof(cand), 0, hipMemcpyHostToDevice, s);
+}
+
+}  // namespace celv2hip

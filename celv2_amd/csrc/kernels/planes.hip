@@ -1,0 +1,135 @@
+// Plane building: import of a frame's luma into a bordered plane, the 2x2 pyramid levels and the
+// 15 quarter-pel phase planes of a reference.
+//
+// Every kernel writes the whole bordered plane, computing a border sample from the clamped position
+// (so the border equals what the reference's clamped reads return). Threads own 4 horizontally
+// adjacent samples of one row and store them as one dword (bordered rows start 64-byte aligned).
+#include "common.h"
+#include "launch.h"
+
+namespace celv2hip {
+
+namespace {
+
+// 8-bit value of a frame sample: itself, or rounded from bit_depth bits exactly as round_to_8bit
+// (src/io/y4m.cpp:88-94): min(255, (v + 2^(d-9)) >> (d-8)).
+__device__ __forceinline__ uint32_t to8(uint32_t v, int bit_depth) {
+  if (bit_depth <= 8) return v & 255;
+  const int s = bit_depth - 8;
+  const uint32_t r = (v + (1u << (s - 1))) >> s;
+  return r > 255 ? 255 : r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_import(const T* __restrict__ src, int sstride, int bit_depth, DPlane out) {
+  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);  // group of 4 samples
+  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
+  const int x0 = 4 * gx - kBorder;
+  if (y >= out.h + kBorder || x0 >= out.w + kBorder) return;
+  const int sy = clampi(y, 0, out.h - 1);
+  const T* row = src + static_cast<size_t>(sy) * sstride;
+  uint32_t word = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) word |= to8(static_cast<uint32_t>(row[clampi(x0 + k, 0, out.w - 1)]), bit_depth) << (8 * k);
+  *reinterpret_cast<uint32_t*>(out.p + static_cast<ptrdiff_t>(y) * out.stride + x0) = word;
+}
+
+// Level l+1 from level l: the rounded 2x2 mean (downsample_px, src/gpu/inter_common.h:256-259).
+__global__ void __launch_bounds__(256) k_downsample(DPlane in, DPlane out) {
+  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
+  const int x0 = 4 * gx - kBorder;
+  if (y >= out.h + kBorder || x0 >= out.w + kBorder) return;
+  const int cy = clampi(y, 0, out.h - 1);
+  const uint8_t* r0 = in.p + static_cast<ptrdiff_t>(2 * cy) * in.stride;
+  const uint8_t* r1 = r0 + in.stride;
+  uint32_t word = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int cx = clampi(x0 + k, 0, out.w - 1);
+    const uint32_t v = (r0[2 * cx] + r0[2 * cx + 1] + r1[2 * cx] + r1[2 * cx + 1] + 2) >> 2;
+    word |= v << (8 * k);
+  }
+  *reinterpret_cast<uint32_t*>(out.p + static_cast<ptrdiff_t>(y) * out.stride + x0) = word;
+}
+
+// AV2 EIGHTTAP_REGULAR at quarter-pel phase q (subpel_tap, src/gpu/inter_common.h:178-184).
+constexpr int kTaps[4][8] = {{0, 0, 0, 128, 0, 0, 0, 0},
+                                {0, 2, -14, 110, 38, -10, 2, 0},
+                                {0, 2, -14, 76, 76, -14, 2, 0},
+                                {0, 2, -10, 38, 110, -14, 2, 0}};
+
+// Phases 1..15 (phase k = (qy, qx) = (k / 4, k % 4)) of level 0: horizontal then vertical 8-tap with
+// one rounding by 2^14, clamped to 0..255 (subpel_px, src/gpu/inter_common.h:187-198), at each
+// sample's clamped position. Phase k's plane starts phase_bytes * k after level 0's. A group of 4
+// samples lies wholly left of, inside or right of the frame (x0 is 4-aligned, w a multiple of 8), so
+// its samples' source columns are c0 + k * step with step 1 inside and 0 in the border.
+__global__ void __launch_bounds__(256) k_subpel(DPlane in, long long phase_bytes) {
+  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
+  const int x0 = 4 * gx - kBorder;
+  if (y >= in.h + kBorder || x0 >= in.w + kBorder) return;
+  const int cy = clampi(y, 0, in.h - 1);
+  const int c0 = clampi(x0, 0, in.w - 1), step = (x0 >= 0 && x0 < in.w) ? 1 : 0;
+  uint8_t* const out = in.p + static_cast<ptrdiff_t>(y) * in.stride + x0;
+#pragma unroll
+  for (int qx = 0; qx < 4; qx++) {
+    int acc[4][4];
+#pragma unroll
+    for (int qy = 0; qy < 4; qy++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) acc[qy][k] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const uint8_t* row = in.p + static_cast<ptrdiff_t>(cy + i - 3) * in.stride + c0 - 3;
+      int h[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int a = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if (kTaps[qx][j]) a += kTaps[qx][j] * static_cast<int>(row[k * step + j]);
+        h[k] = a;
+      }
+#pragma unroll
+      for (int qy = 0; qy < 4; qy++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (kTaps[qy][i]) acc[qy][k] += kTaps[qy][i] * h[k];
+    }
+#pragma unroll
+    for (int qy = 0; qy < 4; qy++) {
+      if (!qy && !qx) continue;
+      uint32_t word = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        int v = (acc[qy][k] + 8192) >> 14;
+        v = v < 0 ? 0 : (v > 255 ? 255 : v);
+        word |= static_cast<uint32_t>(v) << (8 * k);
+      }
+      *reinterpret_cast<uint32_t*>(out + phase_bytes * (4 * qy + qx)) = word;
+    }
+  }
+}
+
+dim3 plane_grid(const DPlane& p) {
+  const int groups = (p.w + 2 * kBorder + 3) / 4;
+  return dim3((groups + 63) / 64, (p.h + 2 * kBorder + 3) / 4);
+}
+
+}  // namespace
+
+void launch_import8(const uint8_t* src, int sstride, const DPlane& out, hipStream_t s) {
+  k_import<uint8_t><<<plane_grid(out), 256, 0, s>>>(src, sstride, 8, out);
+}
+void launch_import16(const uint16_t* src, int sstride, int bit_depth, const DPlane& out, hipStream_t s) {
+  k_import<uint16_t><<<plane_grid(out), 256, 0, s>>>(src, sstride, bit_depth, out);
+}
+void launch_downsample(const DPlane& in, const DPlane& out, hipStream_t s) {
+  k_downsample<<<plane_grid(out), 256, 0, s>>>(in, out);
+}
+void launch_subpel(const DPlane& level0, long long phase_bytes, hipStream_t s) {
+  k_subpel<<<plane_grid(level0), 256, 0, s>>>(level0, phase_bytes);
+}
+
+}  // namespace celv2hip

@@ -1,0 +1,253 @@
+"""Python face of libcelv2hip.so (ctypes).
+
+`HipEngine` drives the C++ engine (celv2::HipEngine, CelV2's DecisionEngine interface:
+launch / prepare / collect, src/gpu/decision_engine.h:9-51); `RawEngine` drives the C ABI
+underneath it (include/celv2hip.h), including the device-resident mode the benchmark times.
+Both return decision maps as numpy arrays of `records.SB_DTYPE`. Nothing here falls back to a CPU
+path: without the built library or a GPU the constructors raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .records import SB_DTYPE
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcelv2hip.so")
+_LIB = None
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+LL = ctypes.c_longlong
+
+
+class Luma(ctypes.Structure):
+    _fields_ = [("y8", P), ("y16", P), ("bit_depth", ctypes.c_int32), ("on_device", ctypes.c_int32)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        L.celv2hip_last_error.restype = ctypes.c_char_p
+        L.celv2hip_device_count.argtypes = [ctypes.POINTER(I)]
+        L.celv2hip_create.argtypes = [I, I, I, I, ctypes.POINTER(P)]
+        L.celv2hip_destroy.argtypes = [P]
+        L.celv2hip_device_name.argtypes = [P, ctypes.c_char_p, I]
+        L.celv2hip_set_row_streams.argtypes = [P, I]
+        L.celv2hip_set_resident.argtypes = [P, I]
+        L.celv2hip_prepare.argtypes = [P, ctypes.POINTER(Luma), LL]
+        L.celv2hip_launch.argtypes = [P, ctypes.POINTER(Luma), LL, ctypes.POINTER(Luma), ctypes.POINTER(LL), I, ctypes.c_int64]
+        L.celv2hip_row_order.argtypes = [P, ctypes.POINTER(I), I]
+        L.celv2hip_wait_row.argtypes = [P, I, P]
+        L.celv2hip_fetch_row.argtypes = [P, I, P]
+        L.celv2hip_sync.argtypes = [P]
+        L.celv2hip_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_float), I]
+        L.celv2hip_counters.argtypes = [P, ctypes.POINTER(LL), I]
+        L.celv2hip_device_alloc.argtypes = [I, LL, ctypes.POINTER(P)]
+        L.celv2hip_device_free.argtypes = [I, P]
+        L.celv2hip_copy_to_device.argtypes = [I, P, P, LL]
+        L.celv2_engine_last_error.restype = ctypes.c_char_p
+        L.celv2_engine_create.argtypes = [I, I, I, I, I]
+        L.celv2_engine_create.restype = P
+        L.celv2_engine_destroy.argtypes = [P]
+        L.celv2_engine_launch.argtypes = [P, P, P, I, P, P, ctypes.POINTER(LL), I, I, LL, LL]
+        L.celv2_engine_prepare.argtypes = [P, P, P, I, LL]
+        L.celv2_engine_collect.argtypes = [P, P, ctypes.POINTER(I)]
+        L.celv2_engine_description.argtypes = [P, ctypes.c_char_p, I]
+        L.celv2_lambda_q8.argtypes = [I]
+        L.celv2_lambda_q8.restype = ctypes.c_int64
+        if L.celv2_sizeof_sb() != SB_DTYPE.itemsize:
+            raise RuntimeError("celv2_sb layout mismatch between the library and records.SB_DTYPE")
+        _LIB = L
+    return _LIB
+
+
+def last_error():
+    return lib().celv2hip_last_error().decode()
+
+
+def device_count():
+    n = I(0)
+    lib().celv2hip_device_count(ctypes.byref(n))
+    return n.value
+
+
+def lambda_q8(qindex):
+    return int(lib().celv2_lambda_q8(qindex))
+
+
+def _ptrs(arrays):
+    arr = (P * max(1, len(arrays)))()
+    for i, a in enumerate(arrays):
+        arr[i] = a.ctypes.data if a is not None else None
+    return arr
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class HipEngine:
+    """celv2::HipEngine: CelV2's DecisionEngine on the GPU."""
+
+    def __init__(self, width, height, device=0, row_streams=1, full_depth=False):
+        self.width, self.height = width, height
+        self.sb_cols, self.sb_rows = (width + 63) // 64, (height + 63) // 64
+        self.h = lib().celv2_engine_create(width, height, device, row_streams, 1 if full_depth else 0)
+        if not self.h:
+            raise EngineError(lib().celv2_engine_last_error().decode())
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().celv2_engine_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def description(self):
+        buf = ctypes.create_string_buffer(256)
+        lib().celv2_engine_description(self.h, buf, 256)
+        return buf.value.decode()
+
+    def launch(self, src8, refs8, qindex=120, lam=-1, keys=None, src_key=-1, src16=None, refs16=None, bit_depth=8):
+        refs8 = [np.ascontiguousarray(r) for r in refs8] if refs8 is not None else None
+        refs16 = [np.ascontiguousarray(r) for r in refs16] if refs16 is not None else None
+        n = len(refs8) if refs8 is not None else len(refs16 or [])
+        k = (LL * max(1, n))(*keys) if keys is not None else None
+        rc = lib().celv2_engine_launch(
+            self.h,
+            src8.ctypes.data if src8 is not None else None,
+            src16.ctypes.data if src16 is not None else None,
+            bit_depth,
+            _ptrs(refs8) if refs8 is not None else None,
+            _ptrs(refs16) if refs16 is not None else None,
+            k,
+            n,
+            qindex,
+            lam,
+            src_key,
+        )
+        if rc == 1:
+            raise ValueError(lib().celv2_engine_last_error().decode())
+        if rc != 0:
+            raise EngineError(lib().celv2_engine_last_error().decode())
+
+    def prepare(self, y8, key, y16=None, bit_depth=8):
+        rc = lib().celv2_engine_prepare(self.h, y8.ctypes.data if y8 is not None else None,
+                                        y16.ctypes.data if y16 is not None else None, bit_depth, key)
+        if rc != 0:
+            raise EngineError(lib().celv2_engine_last_error().decode())
+
+    def collect(self):
+        out = np.zeros(self.sb_cols * self.sb_rows, dtype=SB_DTYPE)
+        ready = I(0)
+        rc = lib().celv2_engine_collect(self.h, out.ctypes.data, ctypes.byref(ready))
+        if rc != 0:
+            raise EngineError(lib().celv2_engine_last_error().decode())
+        return out, ready.value
+
+    def decide(self, src8, refs8, lam, **kw):
+        self.launch(src8, refs8, lam=lam, **kw)
+        return self.collect()[0]
+
+
+class RawEngine:
+    """The C ABI engine (include/celv2hip.h) for tests of its own behaviour and for timing."""
+
+    def __init__(self, width, height, device=0):
+        self.width, self.height, self.device = width, height, device
+        self.sb_cols, self.sb_rows = (width + 63) // 64, (height + 63) // 64
+        h = P()
+        rc = lib().celv2hip_create(width, height, 64, device, ctypes.byref(h))
+        if rc != 0:
+            raise EngineError(last_error())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().celv2hip_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise EngineError(f"{what}: {last_error()} (status {rc})")
+
+    def set_resident(self, on):
+        self._check(lib().celv2hip_set_resident(self.h, 1 if on else 0), "set_resident")
+
+    def set_row_streams(self, n):
+        self._check(lib().celv2hip_set_row_streams(self.h, n), "set_row_streams")
+
+    @staticmethod
+    def luma(y8=None, y16=None, bit_depth=8, dev8=None, dev16=None):
+        if dev8 is not None or dev16 is not None:
+            return Luma(dev8, dev16, bit_depth if dev16 is not None else 8, 1)
+        return Luma(y8.ctypes.data if y8 is not None else None, y16.ctypes.data if y16 is not None else None,
+                    bit_depth if y16 is not None else 8, 0)
+
+    def prepare(self, luma, key):
+        self._check(lib().celv2hip_prepare(self.h, ctypes.byref(luma), key), "prepare")
+
+    def launch(self, src, refs, lam, keys=None, src_key=-1):
+        arr = (Luma * max(1, len(refs)))(*refs)
+        k = (LL * max(1, len(refs)))(*keys) if keys is not None else None
+        self._check(lib().celv2hip_launch(self.h, ctypes.byref(src), src_key, arr, k, len(refs), lam), "launch")
+
+    def collect(self):
+        out = np.zeros(self.sb_cols * self.sb_rows, dtype=SB_DTYPE)
+        rows = (I * self.sb_rows)()
+        n = lib().celv2hip_row_order(self.h, rows, self.sb_rows)
+        for r in list(rows)[:n]:
+            view = out[r * self.sb_cols:(r + 1) * self.sb_cols]
+            self._check(lib().celv2hip_wait_row(self.h, r, view.ctypes.data), "wait_row")
+        return out
+
+    def fetch(self):
+        out = np.zeros(self.sb_cols * self.sb_rows, dtype=SB_DTYPE)
+        for r in range(self.sb_rows):
+            view = out[r * self.sb_cols:(r + 1) * self.sb_cols]
+            self._check(lib().celv2hip_fetch_row(self.h, r, view.ctypes.data), "fetch_row")
+        return out
+
+    def sync(self):
+        self._check(lib().celv2hip_sync(self.h), "sync")
+
+    def stage_times(self):
+        buf = (ctypes.c_float * 6)()
+        n = lib().celv2hip_stage_times(self.h, buf, 6)
+        if n < 0:
+            raise EngineError(last_error())
+        return list(buf)[:n]
+
+    def counters(self):
+        buf = (LL * 3)()
+        n = lib().celv2hip_counters(self.h, buf, 3)
+        if n < 0:
+            raise EngineError(last_error())
+        return list(buf)[:n]
+
+
+class DeviceBuffer:
+    """A device allocation holding a copy of a numpy array (frames resident in HBM)."""
+
+    def __init__(self, array, device=0):
+        self.device = device
+        self.nbytes = array.nbytes
+        p = P()
+        if lib().celv2hip_device_alloc(device, array.nbytes, ctypes.byref(p)) != 0:
+            raise EngineError(last_error())
+        self.ptr = p.value
+        if lib().celv2hip_copy_to_device(device, self.ptr, np.ascontiguousarray(array).ctypes.data, array.nbytes) != 0:
+            raise EngineError(last_error())
+
+    def free(self):
+        if self.ptr:
+            lib().celv2hip_device_free(self.device, self.ptr)
+            self.ptr = None

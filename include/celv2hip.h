@@ -1,0 +1,172 @@
+/* C ABI of libcelv2hip.so: the CelV2 decision engine on one AMD Instinct MI355X (gfx950).
+ *
+ * The engine replaces the Vulkan engine behind CelV2's engine interface `DecisionEngine`
+ * (src/gpu/decision_engine.h:9-51). Its C++ face is `celv2::HipEngine` (include/celv2/hip_engine.h);
+ * this header is the thin layer underneath it: plain pointers and sizes, an opaque handle,
+ * integer status codes and a thread-local error string. Nothing here names a C++ or torch type.
+ *
+ * Units and conventions follow the reference engine:
+ *   - frames are the padded 8-bit luma plane the decisions use (Frame420::y, src/io/y4m.h:11-23):
+ *     pw x ph samples, pw = (width + 7) & ~7, ph = (height + 7) & ~7, stride pw; or the same plane
+ *     at more than 8 bits (Frame420::hy), rounded on the device by min(255, (v + 2^(d-9)) >> (d-8))
+ *     exactly as round_to_8bit (src/io/y4m.cpp:88-94);
+ *   - motion vectors in the exported records are 1/8 pel (BlockDecision, src/gpu/decision_map.h:74-88);
+ *   - costs are SATD * 256 + lambda_q8 * bits (src/gpu/kernels_ref.cpp:46-77); lambda_q8 is an input
+ *     (the reference computes it on the host, src/vk/vk_engine.cpp:253).
+ */
+#ifndef CELV2HIP_H_
+#define CELV2HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CELV2_SB_CAP 256    /* tree entries / leaves per 64x64 superblock (src/gpu/sb_out.h:10) */
+#define CELV2_SB_BLOCKS 425 /* candidate blocks per 64x64 superblock (src/gpu/sb_blocks.h:9-10) */
+#define CELV2_MAX_REFS 7    /* AV2's INTER_REFS_PER_FRAME (src/gpu/inter_common.h:211) */
+#define CELV2_NUM_MODES 13  /* intra modes decided (src/gpu/decision_map.h:71-72) */
+#define CELV2_SINGLE_REF 255
+
+/* Status codes returned by every entry point; celv2hip_last_error() describes the last failure on
+ * the calling thread. The C++ layer maps them to the reference's exceptions:
+ * SIZE -> std::runtime_error (size mismatch, too many references, not enough device memory),
+ * ARG -> std::invalid_argument (key count mismatch), src/vk/vk_engine.cpp:176-183. */
+enum celv2_status {
+  CELV2_OK = 0,
+  CELV2_ERR_ARG = 1,
+  CELV2_ERR_SIZE = 2,
+  CELV2_ERR_DEVICE = 3,
+  CELV2_ERR_MEMORY = 4,
+  CELV2_ERR_STATE = 5
+};
+
+/* ---- The decision map as flat records (the engine's whole output) ---------------------------- */
+
+/* BlockDecision (src/gpu/decision_map.h:74-88). inter, intra: 0/1 and IntraMode. */
+typedef struct celv2_block {
+  uint8_t w4, h4, inter, intra;
+  int16_t mv_row, mv_col;
+  uint8_t ref, ref2;
+  int16_t mv2_row, mv2_col;
+} celv2_block;
+
+/* BlockCandidates (src/gpu/decision_map.h:93-102). */
+typedef struct celv2_cands {
+  int64_t inter_cost, intra_cost;
+  celv2_block inter;
+  uint8_t intra;
+  uint8_t pad_;
+  uint32_t ref_cost[CELV2_MAX_REFS];
+  int16_t ref_mv_col[CELV2_MAX_REFS], ref_mv_row[CELV2_MAX_REFS];
+} celv2_cands;
+
+/* SbDecision (src/gpu/decision_map.h:106-124) of one 64x64 superblock: preorder tree (Part values
+ * None 0, Horz 1, Vert 2, Split 3), leaves in coding order, and every candidate block's choices in
+ * src/gpu/sb_blocks.h order (ncands 0 where the superblock is not 64x64). */
+typedef struct celv2_sb {
+  int32_t ntree, nblocks, ncands, pad_;
+  uint8_t tree[CELV2_SB_CAP];
+  celv2_block blocks[CELV2_SB_CAP];
+  celv2_cands cands[CELV2_SB_BLOCKS];
+} celv2_sb;
+
+/* ---- Frames -------------------------------------------------------------------------------- */
+
+/* One frame's padded luma. Exactly one of y8 / y16 is set. on_device != 0: the pointer is a device
+ * (HBM) address on the engine's GPU, read in place (frames already resident); else host memory,
+ * copied before the call returns (the caller may change it afterwards, src/gpu/decision_engine.h:12-13). */
+typedef struct celv2_luma {
+  const uint8_t* y8;
+  const uint16_t* y16;
+  int32_t bit_depth; /* 8 with y8; 9..16 with y16 */
+  int32_t on_device;
+} celv2_luma;
+
+/* ---- The engine ---------------------------------------------------------------------------- */
+
+typedef struct celv2hip_engine celv2hip_engine;
+
+/* Last failure on this thread ("" if none). */
+const char* celv2hip_last_error(void);
+
+/* Number of HIP devices visible (0 without a GPU). */
+int celv2hip_device_count(int* n);
+
+/* VulkanEngine::VulkanEngine (src/vk/vk_engine.cpp:172-180): an engine for width x height frames,
+ * 64x64 superblocks only, on HIP device `device`. Fails with SIZE for a bad size. */
+int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_engine** out);
+int celv2hip_destroy(celv2hip_engine* e);
+
+/* VulkanEngine::device_name (src/vk/vk_engine.h:29). */
+int celv2hip_device_name(const celv2hip_engine* e, char* buf, int cap);
+
+/* DecisionEngine::set_row_streams (src/gpu/decision_engine.h:30-33). */
+int celv2hip_set_row_streams(celv2hip_engine* e, int n);
+
+/* DecisionEngine::prepare (src/gpu/decision_engine.h:19-22; src/vk/vk_engine.cpp:188-205): queues the
+ * next source's import, pyramid and intra scores behind the current work. key < 0: nothing. */
+int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key);
+
+/* DecisionEngine::launch (src/gpu/decision_engine.h:17-18; src/vk/vk_engine.cpp:207-312): queues the
+ * decision of every superblock row. refs nearest first, nrefs 0..7 (0: a key frame); ref_keys NULL
+ * or one per reference (equal keys: the same picture, its pyramid and phases are reused). */
+int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key, const celv2_luma* refs,
+                    const long long* ref_keys, int nrefs, int64_t lambda_q8);
+
+/* The superblock rows in the order the last launch queued them (row streams); returns the count. */
+int celv2hip_row_order(const celv2hip_engine* e, int* rows, int cap);
+
+/* Part of DecisionEngine::collect (src/vk/vk_engine.cpp:314-328): waits for superblock row `row` of
+ * the last launch and writes its sb_cols records to out. */
+int celv2hip_wait_row(celv2hip_engine* e, int row, celv2_sb* out);
+
+/* Waits for all queued work (used before timing and in tests). */
+int celv2hip_sync(celv2hip_engine* e);
+
+/* Device time of the last launch's kernels by stage, in ms (HIP events on the engine's stream):
+ * out[0] import+pyramid+phases, [1] motion search, [2] intra, [3] inter, [4] compound+finalize,
+ * [5] total launch-to-last-row. Returns the number of values written. */
+int celv2hip_stage_times(celv2hip_engine* e, float* out, int cap);
+
+/* Counters of the last launch: out[0] unique (MV, reference, superblock) evaluations of the tile
+ * scorer, out[1] ME searches (centres searched), out[2] compound blocks scored. */
+int celv2hip_counters(celv2hip_engine* e, long long* out, int cap);
+
+/* ---- Timing mode: device-resident throughput ------------------------------------------------ */
+
+/* Keeps results on the device: wait_row is not available and no device-to-host copy is queued.
+ * on != 0: device-resident mode; 0: normal. */
+int celv2hip_set_resident(celv2hip_engine* e, int on);
+
+/* Copies the records of superblock row `row` of the last launch in resident mode (tests). */
+int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out);
+
+/* ---- Device memory helpers for callers without another allocator --------------------------- */
+int celv2hip_device_alloc(int device, long long bytes, void** out);
+int celv2hip_device_free(int device, void* p);
+int celv2hip_copy_to_device(int device, void* dst, const void* src, long long bytes);
+
+
+/* ---- The C++ engine (celv2::HipEngine, include/celv2/hip_engine.h) behind a handle ------------
+ * For callers that cannot link C++ (tests, the benchmark): DecisionEngine::launch / prepare /
+ * collect with a RowGate, the DecisionMap exported as flat records. Frames are padded luma planes
+ * (y8 and/or y16 at bit_depth); lam >= 0 overrides lambda_q8(qindex). Returns 0, 1 for
+ * std::invalid_argument, 2 for other failures (celv2_engine_last_error describes them). */
+const char* celv2_engine_last_error(void);
+void* celv2_engine_create(int width, int height, int device, int row_streams, int full_depth);
+void celv2_engine_destroy(void* engine);
+int celv2_engine_launch(void* engine, const uint8_t* src8, const uint16_t* src16, int bit_depth,
+                        const uint8_t* const* refs8, const uint16_t* const* refs16, const long long* keys, int nrefs,
+                        int qindex, long long lam, long long src_key);
+int celv2_engine_prepare(void* engine, const uint8_t* y8, const uint16_t* y16, int bit_depth, long long key);
+int celv2_engine_collect(void* engine, celv2_sb* out, int* rows_ready);
+int celv2_engine_description(void* engine, char* buf, int cap);
+int celv2_sizeof_sb(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CELV2HIP_H_ */

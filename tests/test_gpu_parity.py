@@ -1,0 +1,102 @@
+"""HIP engine == oracle, field by field, on the same inputs (tests/test_gpu_intra.cpp:144-165 and
+tests/test_gpu_me.cpp:71-194 of CelV2 compare the GPU engine with the CPU rules the same way)."""
+import numpy as np
+import pytest
+
+from celv2_amd import synth
+from celv2_amd.records import compare_maps
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def clip(workload, seed, n, width=0, height=0, **kw):
+    p = synth.params(workload, seed, width, height, **kw)
+    y16, y8 = synth.clip_frames(p, n)
+    return p, y16, y8
+
+
+def check(h, o, cols):
+    diff = compare_maps(h, o, cols)
+    assert diff is None, diff
+
+
+@pytest.mark.parametrize("w,h", [(352, 288), (202, 122), (64, 33), (32, 32)])
+def test_intra_frame_matches_oracle(gpu, w, h):
+    _, _, y8 = clip(0, 11, 1, w, h)
+    lam = gpu.lambda_q8(120)
+    e = gpu.HipEngine(w, h)
+    got = e.decide(y8[0], [], lam)
+    want = ol.decide_frame(y8[0], [], lam, w, h)
+    check(got, want, e.sb_cols)
+
+
+@pytest.mark.parametrize("w,h,q", [(352, 288, 120), (202, 122, 60), (640, 360, 200)])
+def test_one_ref_matches_oracle(gpu, w, h, q):
+    _, _, y8 = clip(0, 5, 2, w, h)
+    lam = gpu.lambda_q8(q)
+    e = gpu.HipEngine(w, h)
+    got = e.decide(y8[1], [y8[0]], lam)
+    want = ol.decide_frame(y8[1], [y8[0]], lam, w, h)
+    check(got, want, e.sb_cols)
+
+
+def test_seven_refs_with_key_reuse(gpu):
+    w, h = 320, 192
+    _, _, y8 = clip(1, 3, 10, w, h)
+    lam = gpu.lambda_q8(90)
+    e = gpu.HipEngine(w, h)
+    for first in (1, 2):  # the second pass has one new reference, six reused by key
+        refs = [y8[i] for i in range(first, first + 7)]
+        keys = list(range(first, first + 7))
+        e.launch(y8[0], refs, lam=lam, keys=keys)
+        got, ready = e.collect()
+        assert ready == e.sb_rows
+        want = ol.decide_frame(y8[0], refs, lam, w, h)
+        check(got, want, e.sb_cols)
+
+
+def test_ten_bit_rounded_on_device(gpu):
+    w, h = 352, 288
+    _, y16, y8 = clip(0, 9, 2, w, h)
+    lam = gpu.lambda_q8(130)
+    e = gpu.HipEngine(w, h, full_depth=True)
+    e.launch(y8[1], [y8[0]], lam=lam, src16=y16[1], refs16=[y16[0]], bit_depth=10)
+    got, _ = e.collect()
+    want = ol.decide_frame(y8[1], [y8[0]], lam, w, h)
+    check(got, want, e.sb_cols)
+
+
+def test_prepare_and_row_streams(gpu):
+    w, h = 640, 360
+    _, _, y8 = clip(1, 21, 5, w, h)
+    lam = gpu.lambda_q8(110)
+    e = gpu.HipEngine(w, h, row_streams=3)
+    for i in range(2, 5):
+        refs = [y8[i - 1], y8[i - 2]]
+        e.launch(y8[i], refs, lam=lam, keys=[i - 1, i - 2], src_key=i)
+        if i + 1 < 5:
+            e.prepare(y8[i + 1], i + 1)
+        got, ready = e.collect()
+        assert ready == e.sb_rows
+        want = ol.decide_frame(y8[i], refs, lam, w, h)
+        check(got, want, e.sb_cols)
+
+
+def test_resident_device_frames(gpu):
+    w, h = 352, 288
+    _, y16, y8 = clip(0, 13, 3, w, h)
+    lam = gpu.lambda_q8(120)
+    e = gpu.RawEngine(w, h)
+    e.set_resident(True)
+    bufs = [gpu.DeviceBuffer(f) for f in y16]
+    try:
+        src = gpu.RawEngine.luma(dev16=bufs[2].ptr, bit_depth=10)
+        refs = [gpu.RawEngine.luma(dev16=bufs[1].ptr, bit_depth=10), gpu.RawEngine.luma(dev16=bufs[0].ptr, bit_depth=10)]
+        e.launch(src, refs, lam, keys=[1, 0])
+        got = e.fetch()
+    finally:
+        for b in bufs:
+            b.free()
+    want = ol.decide_frame(y8[2], [y8[1], y8[0]], lam, w, h)
+    check(got, want, e.sb_cols)
