@@ -140,6 +140,7 @@ struct celv2hip_engine {
   long long me_searches = 0;
   int nrefs_last = 0;
   bool launched = false;
+  std::vector<RefSlot*> last_refs;  // the last launch's reference slots, nearest first
 
   ~celv2hip_engine() {
     if (device >= 0) (void)hipSetDevice(device);
@@ -441,6 +442,7 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
     check(hipMemcpyAsync(e->counters_host, e->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync counters");
     check(hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
     e->nrefs_last = nrefs;
+    e->last_refs = used;
     e->launched = true;
   });
 }
@@ -470,6 +472,36 @@ int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out) {
     check(hipStreamSynchronize(e->stream), "sync");
     check(hipMemcpy(out, static_cast<char*>(e->rec_dev) + static_cast<size_t>(row) * e->sb_cols * sizeof(celv2_sb),
                     e->sb_cols * sizeof(celv2_sb), hipMemcpyDeviceToHost), "hipMemcpy row");
+  });
+}
+
+int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint8_t* out, int* w, int* h) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    if (!e->launched) throw Failure(CELV2_ERR_STATE, "no launch");
+    if (level < 0 || level >= kLevels || phase < 0 || phase >= kPhases || (phase && level)) throw Failure(CELV2_ERR_ARG, "no such plane");
+    const Pyramid* p = nullptr;
+    if (ref < 0) p = &e->srcs[e->cur_set].p;
+    else if (ref < static_cast<int>(e->last_refs.size())) p = &e->last_refs[ref]->p;
+    else throw Failure(CELV2_ERR_ARG, "no such reference");
+    if (phase && !p->phase_bytes) throw Failure(CELV2_ERR_ARG, "no phases");
+    const DPlane& d = p->lv[level];
+    *w = d.w, *h = d.h;
+    check(hipStreamSynchronize(e->stream), "sync");
+    check(hipMemcpy2D(out, d.w, d.p + static_cast<ptrdiff_t>(phase) * p->phase_bytes, d.stride, d.w, d.h, hipMemcpyDeviceToHost), "hipMemcpy2D");
+  });
+}
+
+int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, int32_t* mvy, int* cols, int* rows) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    if (!e->launched || ref < 0 || ref >= e->nrefs_last || level < 0 || level >= kLevels) throw Failure(CELV2_ERR_ARG, "no such field");
+    const int n = e->fcols[level] * e->frows[level];
+    std::vector<uint32_t> packed(n);
+    check(hipStreamSynchronize(e->stream), "sync");
+    check(hipMemcpy(packed.data(), e->fields[ref][level], n * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy field");
+    for (int i = 0; i < n; i++) mvx[i] = static_cast<int16_t>(packed[i] & 0xffffu), mvy[i] = static_cast<int16_t>(packed[i] >> 16);
+    *cols = e->fcols[level], *rows = e->frows[level];
   });
 }
 

@@ -6,6 +6,8 @@
 // One workgroup per superblock. Compound SATDs are tiled: lanes take 4x4 tiles from a static list of
 // all 425 blocks' tiles and add into per-block LDS sums (integer atomics, order-free); the partition
 // DP runs level by level with one thread per square, and one thread walks the chosen tree.
+#include <stdexcept>
+
 #include "../../../include/celv2hip.h"
 #include "common.h"
 #include "launch.h"
@@ -14,7 +16,7 @@ namespace celv2hip {
 
 namespace {
 
-constexpr int kAllTiles = 5120;  // 4x4 tiles of all 425 blocks (256 per kind and level)
+constexpr int kAllTiles = 3072;  // 4x4 tiles of all 425 blocks: per level 256 (whole) + 4 x 128 (halves)
 constexpr int64_t kMax = INT64_MAX;
 
 __device__ uint32_t g_all_tiles[kAllTiles];  // b | tx << 16 | ty << 24
@@ -246,6 +248,7 @@ void upload_final_tiles(hipStream_t s) {
     for (int ty = 0; ty < h / 4; ty++)
       for (int tx = 0; tx < w / 4; tx++) table[n++] = static_cast<uint32_t>(b) | (static_cast<uint32_t>(tx) << 16) | (static_cast<uint32_t>(ty) << 24);
   }
+  if (n != kAllTiles) throw std::logic_error("compound tile table size");
   (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_all_tiles), table, sizeof(table), 0, hipMemcpyHostToDevice, s);
 }
 

@@ -11,6 +11,8 @@
 // those keys (LDS atomicMin, order-free) is the first strict minimum in candidate order, exactly the
 // reference's choice (a repeated MV costs the same as its first occurrence and never wins; the
 // lambda * (3 + r) term is the same for every candidate of a reference and cannot change the order).
+#include <stdexcept>
+
 #include "common.h"
 #include "launch.h"
 
@@ -228,7 +230,7 @@ void upload_inter_tables(hipStream_t s) {
       const int c = region_cell(b, k);
       tasks[first[c] + fill[c]++] = static_cast<uint16_t>((b << 7) | k);
     }
-  (void)total;
+  if (total != kTasks || off != kTasks) throw std::logic_error("candidate task table size");
   (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cell_first), first, sizeof(first), 0, hipMemcpyHostToDevice, s);
   (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cell_tasks), tasks, sizeof(tasks), 0, hipMemcpyHostToDevice, s);
   (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cand_cell), cand, sizeof(cand), 0, hipMemcpyHostToDevice, s);

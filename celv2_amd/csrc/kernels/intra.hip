@@ -5,6 +5,8 @@
 // the neighbours any of its blocks reads) is staged in LDS through clamped reads; the mode loop is
 // uniform over the workgroup and lanes take 4x4 tiles from a static (block, tile) list, so a wave
 // works on one block of one mode; per-(block, mode) sums are LDS integer atomics (order-free).
+#include <stdexcept>
+
 #include "common.h"
 #include "launch.h"
 
@@ -162,6 +164,7 @@ void upload_intra_tiles(hipStream_t s) {
     for (int ty = 0; ty < h / 4; ty++)
       for (int tx = 0; tx < w / 4; tx++) table[n++] = static_cast<uint32_t>(b) | (static_cast<uint32_t>(tx) << 16) | (static_cast<uint32_t>(ty) << 24);
   }
+  if (n != kIntraTiles) throw std::logic_error("intra tile table size");
   (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_intra_tiles), table, sizeof(table), 0, hipMemcpyHostToDevice, s);
 }
 

@@ -61,55 +61,81 @@ constexpr int kTaps[4][8] = {{0, 0, 0, 128, 0, 0, 0, 0},
 
 // Phases 1..15 (phase k = (qy, qx) = (k / 4, k % 4)) of level 0: horizontal then vertical 8-tap with
 // one rounding by 2^14, clamped to 0..255 (subpel_px, src/gpu/inter_common.h:187-198), at each
-// sample's clamped position. Phase k's plane starts phase_bytes * k after level 0's. A group of 4
-// samples lies wholly left of, inside or right of the frame (x0 is 4-aligned, w a multiple of 8), so
-// its samples' source columns are c0 + k * step with step 1 inside and 0 in the border.
-__global__ void __launch_bounds__(256) k_subpel(DPlane in, long long phase_bytes) {
-  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
-  const int x0 = 4 * gx - kBorder;
-  if (y >= in.h + kBorder || x0 >= in.w + kBorder) return;
-  const int cy = clampi(y, 0, in.h - 1);
-  const int c0 = clampi(x0, 0, in.w - 1), step = (x0 >= 0 && x0 < in.w) ? 1 : 0;
-  uint8_t* const out = in.p + static_cast<ptrdiff_t>(y) * in.stride + x0;
+// sample's clamped position. Phase k's plane starts phase_bytes * k after level 0's.
+//
+// A group of 4 samples lies wholly left of, inside or right of the frame (x0 is 4-aligned, w a
+// multiple of 8). Inside, the 11 source columns x0-3 .. x0+7 of each row come from three aligned dword
+// loads (columns x0-4 .. x0+7); in the border all 4 samples are the edge column's value.
+// Keeps a value in a register as computed: the byte packing below must stay plain shifts and ORs.
+// Left to itself the compiler packs two saturated shifts with v_ashr_pk_u8_i32, and on the MI355X
+// boxes measured the packed word then carried a wrong third byte (every 4th sample of a phase plane
+// differed from the CPU rule).
+__device__ __forceinline__ void opaque(int& v) { asm volatile("" : "+v"(v)); }
+
+template <bool kInside>
+__device__ __forceinline__ void subpel_group(const DPlane& in, int cy, int c0, uint32_t out[16]) {
+  int acc[16][4];
 #pragma unroll
-  for (int qx = 0; qx < 4; qx++) {
-    int acc[4][4];
+  for (int ph = 0; ph < 16; ph++)
 #pragma unroll
-    for (int qy = 0; qy < 4; qy++)
+    for (int k = 0; k < 4; k++) acc[ph][k] = 0;
 #pragma unroll
-      for (int k = 0; k < 4; k++) acc[qy][k] = 0;
+  for (int i = 0; i < 8; i++) {
+    const uint8_t* row = in.p + static_cast<ptrdiff_t>(cy + i - 3) * in.stride + c0 - 4;
+    int s[12];
+    if (kInside) {
+      const uint32_t* q = reinterpret_cast<const uint32_t*>(row);
+      const uint32_t d[3] = {q[0], q[1], q[2]};
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-      const uint8_t* row = in.p + static_cast<ptrdiff_t>(cy + i - 3) * in.stride + c0 - 3;
+      for (int o = 0; o < 12; o++) s[o] = static_cast<int>((d[o >> 2] >> (8 * (o & 3))) & 255);
+    } else {
+#pragma unroll
+      for (int o = 0; o < 12; o++) s[o] = row[o];
+    }
+#pragma unroll
+    for (int qx = 0; qx < 4; qx++) {
       int h[4];
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         int a = 0;
 #pragma unroll
         for (int j = 0; j < 8; j++)
-          if (kTaps[qx][j]) a += kTaps[qx][j] * static_cast<int>(row[k * step + j]);
+          if (kTaps[qx][j]) a += kTaps[qx][j] * s[(kInside ? k : 0) + j + 1];
         h[k] = a;
       }
 #pragma unroll
       for (int qy = 0; qy < 4; qy++)
 #pragma unroll
         for (int k = 0; k < 4; k++)
-          if (kTaps[qy][i]) acc[qy][k] += kTaps[qy][i] * h[k];
-    }
-#pragma unroll
-    for (int qy = 0; qy < 4; qy++) {
-      if (!qy && !qx) continue;
-      uint32_t word = 0;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        int v = (acc[qy][k] + 8192) >> 14;
-        v = v < 0 ? 0 : (v > 255 ? 255 : v);
-        word |= static_cast<uint32_t>(v) << (8 * k);
-      }
-      *reinterpret_cast<uint32_t*>(out + phase_bytes * (4 * qy + qx)) = word;
+          if (kTaps[qy][i]) acc[4 * qy + qx][k] += kTaps[qy][i] * h[k];
     }
   }
+#pragma unroll
+  for (int ph = 1; ph < 16; ph++) {
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      int v = (acc[ph][k] + 8192) >> 14;
+      v = v < 0 ? 0 : (v > 255 ? 255 : v);
+      opaque(v);
+      word |= static_cast<uint32_t>(v) << (8 * k);
+    }
+    out[ph] = word;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_subpel(DPlane in, long long phase_bytes) {
+  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);
+  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
+  const int x0 = 4 * gx - kBorder;
+  if (y >= in.h + kBorder || x0 >= in.w + kBorder) return;
+  const int cy = clampi(y, 0, in.h - 1);
+  uint32_t word[16];
+  if (x0 >= 0 && x0 < in.w) subpel_group<true>(in, cy, x0, word);
+  else subpel_group<false>(in, cy, x0 < 0 ? 0 : in.w - 1, word);
+  uint8_t* const out = in.p + static_cast<ptrdiff_t>(y) * in.stride + x0;
+#pragma unroll
+  for (int ph = 1; ph < 16; ph++) *reinterpret_cast<uint32_t*>(out + phase_bytes * ph) = word[ph];
 }
 
 dim3 plane_grid(const DPlane& p) {

@@ -47,6 +47,8 @@ def lib():
         L.celv2hip_sync.argtypes = [P]
         L.celv2hip_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_float), I]
         L.celv2hip_counters.argtypes = [P, ctypes.POINTER(LL), I]
+        L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_device_alloc.argtypes = [I, LL, ctypes.POINTER(P)]
         L.celv2hip_device_free.argtypes = [I, P]
         L.celv2hip_copy_to_device.argtypes = [I, P, P, LL]
@@ -218,6 +220,23 @@ class RawEngine:
 
     def sync(self):
         self._check(lib().celv2hip_sync(self.h), "sync")
+
+    def fetch_plane(self, ref, level, phase=0):
+        """Interior of a plane of the last launch (ref -1: the source pyramid)."""
+        pw, ph = (self.width + 7) & ~7, (self.height + 7) & ~7
+        out = np.zeros(pw * ph, np.uint8)
+        w, h = I(0), I(0)
+        self._check(lib().celv2hip_fetch_plane(self.h, ref, level, phase, out.ctypes.data, ctypes.byref(w), ctypes.byref(h)), "fetch_plane")
+        return out[: w.value * h.value].reshape(h.value, w.value).copy()
+
+    def fetch_field(self, ref, level):
+        pw, ph = (self.width + 7) & ~7, (self.height + 7) & ~7
+        n = ((pw + 7) // 8) * ((ph + 7) // 8)
+        mx, my = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        c, r = I(0), I(0)
+        self._check(lib().celv2hip_fetch_field(self.h, ref, level, mx.ctypes.data, my.ctypes.data, ctypes.byref(c), ctypes.byref(r)), "fetch_field")
+        k = c.value * r.value
+        return mx[:k].reshape(r.value, c.value).copy(), my[:k].reshape(r.value, c.value).copy()
 
     def stage_times(self):
         buf = (ctypes.c_float * 6)()

@@ -58,28 +58,41 @@ def sb_canonical(sb):
 
 
 def compare_maps(a, b, sb_cols=None):
-    """None when equal, else a text naming the first differing superblock and field."""
+    """None when equal, else a text naming the first differing superblock and field, with a count of
+    differing candidates per field over the whole map."""
     if a.shape != b.shape:
         return f"sizes differ: {a.shape} vs {b.shape}"
+    first = None
+    per_field = {}
     for i in range(a.shape[0]):
         x, y = a[i], b[i]
         where = f"sb {i}" + (f" (col {i % sb_cols}, row {i // sb_cols})" if sb_cols else "")
+        msg = None
         for f in ("ntree", "nblocks", "ncands"):
-            if int(x[f]) != int(y[f]):
-                return f"{where}: {f} {int(x[f])} vs {int(y[f])}"
+            if int(x[f]) != int(y[f]) and msg is None:
+                msg = f"{where}: {f} {int(x[f])} vs {int(y[f])}"
         nt, nb, nc = int(x["ntree"]), int(x["nblocks"]), int(x["ncands"])
-        if not np.array_equal(x["tree"][:nt], y["tree"][:nt]):
-            return f"{where}: tree {list(x['tree'][:nt])} vs {list(y['tree'][:nt])}"
-        for k in range(nb):
-            if x["blocks"][k].tobytes() != y["blocks"][k].tobytes():
-                return f"{where}: leaf {k} {_block_tuple(x['blocks'][k])} vs {_block_tuple(y['blocks'][k])}"
+        if msg is None and not np.array_equal(x["tree"][:nt], y["tree"][:nt]):
+            msg = f"{where}: tree {list(x['tree'][:nt])} vs {list(y['tree'][:nt])}"
+        if msg is None:
+            for k in range(nb):
+                if x["blocks"][k].tobytes() != y["blocks"][k].tobytes():
+                    msg = f"{where}: leaf {k} {_block_tuple(x['blocks'][k])} vs {_block_tuple(y['blocks'][k])}"
+                    break
+        nc = min(int(x["ncands"]), int(y["ncands"]))
         if nc and x["cands"][:nc].tobytes() != y["cands"][:nc].tobytes():
             for k in range(nc):
                 p, q = x["cands"][k], y["cands"][k]
-                for f in CANDS_DTYPE.names:
-                    if np.asarray(p[f]).tobytes() != np.asarray(q[f]).tobytes():
-                        return f"{where}: candidate block {k} field {f}: {p[f]} vs {q[f]}"
-    return None
+                bad = [f for f in CANDS_DTYPE.names if np.asarray(p[f]).tobytes() != np.asarray(q[f]).tobytes()]
+                for f in bad:
+                    per_field[f] = per_field.get(f, 0) + 1
+                if bad and (msg is None or "candidate" not in msg):
+                    msg = f"{where}: candidate block {k}: " + "; ".join(f"{f} {p[f]} vs {q[f]}" for f in bad)
+        if msg and first is None:
+            first = msg
+    if first is None:
+        return None
+    return first + (f" | differing candidates per field: {per_field}" if per_field else "")
 
 
 def leaf_rects(sb, ox=0, oy=0, size=64):

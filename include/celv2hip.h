@@ -143,6 +143,17 @@ int celv2hip_set_resident(celv2hip_engine* e, int on);
 /* Copies the records of superblock row `row` of the last launch in resident mode (tests). */
 int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out);
 
+
+/* ---- Stage outputs of the last launch, for per-kernel tests (tests/test_vk_kernels.cpp's
+ * downsample / subpel / me checks of CelV2 compare each kernel with its CPU rule the same way) ---- */
+
+/* Interior samples (w x h of that level) of the source pyramid (ref < 0) or of reference `ref`'s
+ * (its index in the last launch); phase 1..15: that quarter-pel plane of level 0. */
+int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint8_t* out, int* w, int* h);
+
+/* Motion field of reference `ref` at `level` (cols x rows, level 0 in quarter pels). */
+int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, int32_t* mvy, int* cols, int* rows);
+
 /* ---- Device memory helpers for callers without another allocator --------------------------- */
 int celv2hip_device_alloc(int device, long long bytes, void** out);
 int celv2hip_device_free(int device, void* p);

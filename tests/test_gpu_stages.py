@@ -1,0 +1,55 @@
+"""Each stage of the HIP engine against its CPU rule (CelV2 checks each Vulkan kernel the same way,
+tests/test_vk_kernels.cpp: downsample_matches_ref, subpel_matches_ref, me_matches_ref)."""
+import numpy as np
+import pytest
+
+from celv2_amd import synth
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def launched(gpu):
+    out = {}
+    for (w, h) in [(352, 288), (202, 122)]:
+        p = synth.params(0, 5, w, h)
+        y8 = synth.clip_frames(p, 3)[1]
+        e = gpu.RawEngine(w, h)
+        lam = gpu.lambda_q8(120)
+        e.launch(gpu.RawEngine.luma(y8[2]), [gpu.RawEngine.luma(y8[1]), gpu.RawEngine.luma(y8[0])], lam, keys=[1, 0])
+        e.collect()
+        out[(w, h)] = (e, y8)
+    return out
+
+
+@pytest.mark.parametrize("size", [(352, 288), (202, 122)])
+def test_pyramids_match(launched, size):
+    e, y8 = launched[size]
+    want_src = ol.pyramid(y8[2])
+    want_ref = ol.pyramid(y8[1])
+    for l in range(5):
+        assert np.array_equal(e.fetch_plane(-1, l), want_src[l]), ("src", l)
+        assert np.array_equal(e.fetch_plane(0, l), want_ref[l]), ("ref", l)
+
+
+@pytest.mark.parametrize("size", [(352, 288), (202, 122)])
+def test_phases_match(launched, size):
+    e, y8 = launched[size]
+    want = ol.phases(y8[1])
+    for k in range(16):
+        got = e.fetch_plane(0, 0, k)
+        bad = np.argwhere(got != want[k])
+        assert bad.size == 0, (k, bad[:5].tolist())
+
+
+@pytest.mark.parametrize("size", [(352, 288), (202, 122)])
+def test_fields_match(launched, size):
+    e, y8 = launched[size]
+    O = ol.oracle()
+    for r in (0, 1):
+        want = ol.fields(O, "orc_", y8[2], y8[1 - r])
+        for l in range(4, -1, -1):
+            gx, gy = e.fetch_field(r, l)
+            bad = np.argwhere((gx != want[l][0]) | (gy != want[l][1]))
+            assert bad.size == 0, (r, l, bad[:5].tolist(), [(int(gx[i, j]), int(gy[i, j]), int(want[l][0][i, j]), int(want[l][1][i, j])) for i, j in bad[:5]])
