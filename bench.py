@@ -1,0 +1,212 @@
+"""Benchmark of the HIP decision engine on one MI355X.
+
+Workload (the reference's engine_bench method, tools/engine_bench.cpp:44-57): a 1920x1080 10-bit
+4:2:0 synthetic clip of 32 frames (pan, 12 moving objects, grain, fade to black at frames 24-27);
+frame i >= 7 is decided against the 7 source frames before it, nearest first, keys = frame indices,
+qindex 130, with the next frame prepare()d behind it. A step is one frame's decision. The 10-bit
+frames are resident in HBM before timing (the engine rounds them to the 8 bits the decisions read
+on the device); decision records stay in HBM (device-resident mode).
+
+Printed: one JSON line with frames/s, the dominant kernel's roofline fraction, stage times measured
+with HIP events on the engine's stream over the timed steps, the end-to-end rate through the C++
+HipEngine with host buffers (PCIe and DecisionMap conversion included), and the CPU engine (the
+oracle) timed on this host's cores on one frame, whose decisions are also checked against the GPU's.
+
+    python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from celv2_amd import engine, synth  # noqa: E402
+from celv2_amd.records import SB_DTYPE, compare_maps  # noqa: E402
+
+METRIC = "1080p10 frames/s on 1 GPU: plain HIP vs learned-pruned at ≤0.5% BD-rate loss"
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, one int32 op per lane per clock
+SAD_OPS, SATD_OPS = 3, 7  # int ops per pixel (sub, abs, add; diff, 2 butterfly passes, abs, add)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(y8, frame, refs, lam, gpu_map, threads):
+    """The oracle's threaded CPU engine on one frame (test infrastructure, timed here only)."""
+    so = os.path.join(ROOT, "oracle", "liboracle.so")
+    O = ctypes.CDLL(so)
+    O.orc_engine_create.restype = ctypes.c_void_p
+    O.orc_engine_create.argtypes = [ctypes.c_int] * 5
+    O.orc_engine_launch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int64]
+    O.orc_engine_collect.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    O.orc_engine_destroy.argtypes = [ctypes.c_void_p]
+    h, w = y8[0].shape
+    e = O.orc_engine_create(w, h, 64, threads, -1)
+    arr = (ctypes.c_void_p * len(refs))(*[y8[i].ctypes.data for i in refs])
+    out = np.zeros(gpu_map.shape[0], dtype=SB_DTYPE)
+    t0 = time.perf_counter()
+    O.orc_engine_launch(e, w, h, y8[frame].ctypes.data, arr, len(refs), lam)
+    rc = O.orc_engine_collect(e, out.ctypes.data, None)
+    dt = time.perf_counter() - t0
+    O.orc_engine_destroy(e)
+    if rc != 0:
+        raise RuntimeError("CPU engine failed")
+    return dt, compare_maps(gpu_map, out, (w + 63) // 64)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--frames", type=int, default=32)
+    ap.add_argument("--refs", type=int, default=7)
+    ap.add_argument("--qindex", type=int, default=130)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--e2e-steps", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the decision engine runs on a single MI355X (no multi-GPU path is built)",
+              file=sys.stderr)
+        sys.exit(2)
+    if engine.device_count() < 1:
+        raise SystemExit("no HIP device")
+
+    W, H, NR = 1920, 1080, args.refs
+    p = synth.params(1, args.seed, W, H)
+    t = time.perf_counter()
+    y16, y8 = synth.clip_frames(p, args.frames)
+    log(f"clip: {args.frames} frames {W}x{H} 10-bit in {time.perf_counter() - t:.1f}s")
+    lam = engine.lambda_q8(args.qindex)
+    dev = [engine.DeviceBuffer(f) for f in y16]  # resident 10-bit luma
+    E = engine.RawEngine(W, H)
+    E.set_resident(True)
+    lum = [engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]
+    decidable = list(range(NR, args.frames))
+
+    def step(s):
+        i = decidable[s % len(decidable)]
+        nxt = decidable[(s + 1) % len(decidable)]
+        refs = [i - k for k in range(1, NR + 1)]
+        E.launch(lum[i], [lum[r] for r in refs], lam, keys=refs, src_key=i)
+        E.prepare(lum[nxt], nxt)
+        return i
+
+    E.prepare(lum[decidable[0]], decidable[0])
+    for s in range(args.warmup):
+        step(s)
+    E.stats_reset()  # synchronizes
+    t0 = time.perf_counter()
+    last = None
+    for s in range(args.warmup, args.warmup + args.steps):
+        last = step(s)
+    E.sync()
+    wall = time.perf_counter() - t0
+    ms, cnt = E.stats()
+    fps = args.steps / wall
+    n = cnt["launches"]
+    per = {k: v / max(1, n) for k, v in ms.items() if k != "intra_prepare"}
+    per["intra"] = ms["intra_prepare"] / max(1, n)
+    log(f"timed {args.steps} frames in {wall:.3f}s: {fps:.1f} frames/s; per frame ms {json.dumps({k: round(v, 3) for k, v in per.items()})}")
+    log(f"counts over the window: {cnt}")
+    gpu_map = E.fetch()
+
+    # Algorithmic integer work per frame (weights of the reference's cost rules, counted as executed).
+    sbs = E.sb_cols * E.sb_rows
+    cells0 = ((W + 7) // 8) * ((H + 7) // 8)
+    cells = 0
+    w_, h_ = (W + 7) & ~7, (H + 7) & ~7
+    levels_cells = []
+    for l in range(5):
+        levels_cells.append(((w_ + 7) // 8) * ((h_ + 7) // 8))
+        w_, h_ = w_ // 2, h_ // 2
+    centre_cells = sum(levels_cells[:4]) * NR  # levels below the top rank 6 centres
+    me_sad_px = cnt["me_centres"] / n * 289 * 64 + centre_cells * 6 * 64
+    me_satd_px = cells0 * NR * 18 * 64
+    me_ops = SAD_OPS * me_sad_px + SATD_OPS * me_satd_px
+    inter_ops = SATD_OPS * cnt["tile_evals"] / n * 4096
+    intra_ops = SATD_OPS * sbs * 479232
+    stage_ops = {"me": me_ops, "inter": inter_ops, "intra": intra_ops}
+    dom = max(("me", "inter", "intra", "finalize"), key=lambda k: per[k])
+    if dom in stage_ops:
+        achieved = stage_ops[dom] / (per[dom] / 1e3) / 1e12
+        roofline = {"bound": "valu", "kernel": {"me": "k_me_level (5 launches)", "inter": "k_inter", "intra": "k_intra"}[dom],
+                    "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s",
+                    "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                    "ops_per_launch": stage_ops[dom], "ms_per_launch": round(per[dom], 4)}
+    else:
+        roofline = {"bound": "valu", "kernel": "k_finalize", "achieved": None, "peak": round(VALU_PEAK_TOPS, 2),
+                    "unit": "Tops/s", "frac": None, "traffic": None, "ms_per_launch": round(per[dom], 4)}
+    total_ops = me_ops + inter_ops + intra_ops
+
+    # End to end through the C++ HipEngine (host frames, upload, DecisionMap); never the metric.
+    e2e = None
+    if args.e2e_steps > 0:
+        HE = engine.HipEngine(W, H, full_depth=True)
+        frames = decidable[: args.e2e_steps + 2]
+        ts = []
+        for j, i in enumerate(frames):
+            refs = [i - k for k in range(1, NR + 1)]
+            t = time.perf_counter()
+            HE.launch(y8[i], [y8[r] for r in refs], lam=lam, keys=refs, src_key=i, src16=y16[i],
+                      refs16=[y16[r] for r in refs], bit_depth=10)
+            if j + 1 < len(frames):
+                HE.prepare(y8[frames[j + 1]], frames[j + 1], y16=y16[frames[j + 1]], bit_depth=10)
+            HE.collect()
+            ts.append(time.perf_counter() - t)
+        e2e = round(1.0 / (sum(ts[2:]) / len(ts[2:])), 2)
+        HE.close()
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        refs = [last - k for k in range(1, NR + 1)]
+        dt, diff = cpu_baseline(y8, last, refs, lam, gpu_map, threads)
+        parity = diff is None
+        if diff:
+            log("PARITY FAILURE vs the CPU engine:", diff)
+        cpu = {"value": round(1.0 / dt, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"1 frame (frame {last}) of the same 1080p 10-bit 7-reference workload, oracle CpuEngine with {threads} threads: {dt:.1f} s"}
+        log(f"cpu engine: {dt:.2f}s for one frame ({threads} threads); parity {'ok' if parity else 'FAILED'}")
+    for b in dev:
+        b.free()
+
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded 10-bit generator, frames resident in HBM)",
+        "config": {"workload": "1920x1080 10-bit 4:2:0, 32-frame clip, 7 references (previous source frames), qindex 130, "
+                               "plain HIP engine, engine_bench method (frame i decided against i-1..i-7, next frame prepared)",
+                   "width": W, "height": H, "refs": NR, "qindex": args.qindex, "lambda_q8": lam, "form": "plain"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity_vs_cpu_engine": parity,
+        "stage_ms_per_frame": {k: round(v, 4) for k, v in per.items()},
+        "counts_per_frame": {k: round(v / max(1, n), 1) for k, v in cnt.items() if k != "launches"},
+        "int_ops_per_frame": total_ops,
+        "all_kernels_tops": round(total_ops * fps / 1e12, 3),
+        "engine_path_fps_host_buffers": e2e,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -134,10 +135,15 @@ struct celv2hip_engine {
   void* rec_host = nullptr;  // pinned
   std::vector<int> order;
   std::vector<hipEvent_t> row_done;
-  hipEvent_t ev[6] = {};
-  unsigned long long* counters = nullptr;  // device: [0] unique MVs scored, [1] compound blocks
+  // Per-launch stage events in a ring (kRing launches): start, references built, motion search,
+  // inter scores, finalize, end; and per prepare(): intra scores start / end.
+  static constexpr int kRing = 512;
+  std::vector<std::array<hipEvent_t, 6>> ring;
+  std::vector<std::array<hipEvent_t, 2>> pring;
+  hipEvent_t* ev = nullptr;  // the current launch's 6 events
+  long long prepares = 0, window_launch = 0, window_prepare = 0;
+  unsigned long long* counters = nullptr;  // device: [0] unique MVs scored, [1] compound blocks, [16..]: ME searches
   unsigned long long* counters_host = nullptr;
-  long long me_searches = 0;
   int nrefs_last = 0;
   bool launched = false;
   std::vector<RefSlot*> last_refs;  // the last launch's reference slots, nearest first
@@ -168,8 +174,10 @@ struct celv2hip_engine {
     if (counters) (void)hipFree(counters);
     if (counters_host) (void)hipHostFree(counters_host);
     for (hipEvent_t e : row_done) (void)hipEventDestroy(e);
-    for (hipEvent_t e : ev)
-      if (e) (void)hipEventDestroy(e);
+    for (auto& a : ring)
+      for (hipEvent_t e : a) (void)hipEventDestroy(e);
+    for (auto& a : pring)
+      for (hipEvent_t e : a) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -219,8 +227,11 @@ struct celv2hip_engine {
 
   void intra_scores(SrcSlot& s) {
     if (!s.intra) check(hipMalloc(&s.intra, nsb() * kBlocks * kModes * sizeof(uint32_t)), "hipMalloc intra");
+    std::array<hipEvent_t, 2>& pe = pring[prepares++ % kRing];
+    check(hipEventRecord(pe[0], stream), "hipEventRecord");
     launch_intra(s.p.lv[0], sb_cols, 0, sb_rows, s.intra, stream);
     check(hipGetLastError(), "intra kernel");
+    check(hipEventRecord(pe[1], stream), "hipEventRecord");
   }
 };
 
@@ -278,11 +289,18 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     check(hipMalloc(&e->best, e->nsb() * kMaxRefs * kBlocks * sizeof(uint2)), "hipMalloc best");
     check(hipMalloc(&e->rec_dev, e->nsb() * sizeof(celv2_sb)), "hipMalloc records");
     check(hipHostMalloc(&e->rec_host, e->nsb() * sizeof(celv2_sb), hipHostMallocDefault), "hipHostMalloc records");
-    check(hipMalloc(&e->counters, 4 * sizeof(unsigned long long)), "hipMalloc counters");
-    check(hipHostMalloc(reinterpret_cast<void**>(&e->counters_host), 4 * sizeof(unsigned long long), hipHostMallocDefault), "hipHostMalloc counters");
+    check(hipMalloc(&e->counters, 80 * 16 * sizeof(unsigned long long)), "hipMalloc counters");
+    check(hipHostMalloc(reinterpret_cast<void**>(&e->counters_host), 80 * 16 * sizeof(unsigned long long), hipHostMallocDefault), "hipHostMalloc counters");
     e->row_done.resize(e->sb_rows);
     for (hipEvent_t& ev : e->row_done) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
-    for (hipEvent_t& ev : e->ev) check(hipEventCreate(&ev), "hipEventCreate");
+    e->ring.resize(celv2hip_engine::kRing);
+    e->pring.resize(celv2hip_engine::kRing);
+    for (auto& a : e->ring)
+      for (hipEvent_t& ev : a) check(hipEventCreate(&ev), "hipEventCreate");
+    for (auto& a : e->pring)
+      for (hipEvent_t& ev : a) check(hipEventCreate(&ev), "hipEventCreate");
+    e->ev = e->ring[0].data();
+    check(hipMemset(e->counters, 0, 80 * 16 * sizeof(unsigned long long)), "hipMemset counters");
     upload_intra_tiles(e->stream);
     upload_inter_tables(e->stream);
     upload_final_tiles(e->stream);
@@ -334,6 +352,7 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
     if (!src) throw Failure(CELV2_ERR_ARG, "HipEngine: no source");
     if (nrefs < 0 || nrefs > kMaxRefs) throw Failure(CELV2_ERR_SIZE, "HipEngine: too many references");
     if (nrefs > 0 && !refs) throw Failure(CELV2_ERR_ARG, "HipEngine: no references");
+    e->ev = e->ring[e->launches % celv2hip_engine::kRing].data();
     check(hipEventRecord(e->ev[0], e->stream), "hipEventRecord");
     int use = -1;
     for (int i = 0; i < 2; i++)
@@ -373,13 +392,13 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
     }
     check(hipEventRecord(e->ev[1], e->stream), "hipEventRecord");
     // Motion search, top level first, every reference per launch.
-    e->me_searches = 0;
     for (int l = kLevels - 1; l >= 0 && nrefs > 0; l--) {
       MeLevelArgs a{};
       a.cur = cs.p.lv[l];
       a.nrefs = nrefs, a.cols = e->fcols[l], a.rows = e->frows[l];
       a.pcols = l + 1 < kLevels ? e->fcols[l + 1] : 0, a.prows = l + 1 < kLevels ? e->frows[l + 1] : 0;
       a.level0 = l == 0, a.phase_bytes = used[0]->p.phase_bytes;
+      a.searches = e->counters + 16;
       for (int r = 0; r < nrefs; r++) {
         if (!e->fields[r][l]) check(hipMalloc(&e->fields[r][l], static_cast<size_t>(e->fcols[l]) * e->frows[l] * sizeof(uint32_t)), "hipMalloc field");
         a.ref[r] = used[r]->p.lv[l];
@@ -388,10 +407,8 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
       }
       launch_me_level(a, e->stream);
       check(hipGetLastError(), "me kernel");
-      e->me_searches += static_cast<long long>(nrefs) * e->fcols[l] * e->frows[l];
     }
     check(hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
-    check(hipMemsetAsync(e->counters, 0, 4 * sizeof(unsigned long long), e->stream), "hipMemsetAsync");
     // Rows: row streams as VulkanEngine orders them (src/vk/vk_engine.cpp:288-294).
     const int streams = std::max(1, std::min(e->row_streams, e->sb_rows));
     const int run = (e->sb_rows + streams - 1) / streams;
@@ -439,7 +456,6 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
       check(hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
       check(hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
     }
-    check(hipMemcpyAsync(e->counters_host, e->counters, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync counters");
     check(hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
     e->nrefs_last = nrefs;
     e->last_refs = used;
@@ -510,30 +526,48 @@ int celv2hip_sync(celv2hip_engine* e) {
   return guarded([&] { check(hipStreamSynchronize(e->stream), "hipStreamSynchronize"); });
 }
 
-int celv2hip_stage_times(celv2hip_engine* e, float* out, int cap) {
-  if (int rc = validate(e)) return -rc;
-  int n = 0;
-  const int rc = guarded([&] {
-    check(hipEventSynchronize(e->ev[5]), "hipEventSynchronize");
-    const int pairs[6][2] = {{0, 1}, {1, 2}, {0, 0}, {2, 3}, {3, 4}, {0, 5}};
-    for (int i = 0; i < 6 && i < cap; i++) {
-      float ms = 0;
-      if (pairs[i][0] != pairs[i][1]) check(hipEventElapsedTime(&ms, e->ev[pairs[i][0]], e->ev[pairs[i][1]]), "hipEventElapsedTime");
-      out[n++] = ms;
-    }
+int celv2hip_stats_reset(celv2hip_engine* e) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    check(hipMemset(e->counters, 0, 80 * 16 * sizeof(unsigned long long)), "hipMemset counters");
+    e->window_launch = e->launches, e->window_prepare = e->prepares;
   });
-  return rc ? -rc : n;
 }
 
-int celv2hip_counters(celv2hip_engine* e, long long* out, int cap) {
+int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts, int cap_counts) {
   if (int rc = validate(e)) return -rc;
-  int n = 0;
+  long long n = 0;
   const int rc = guarded([&] {
-    check(hipEventSynchronize(e->ev[5]), "hipEventSynchronize");
-    const long long v[3] = {static_cast<long long>(e->counters_host[0]), e->me_searches, static_cast<long long>(e->counters_host[1])};
-    for (int i = 0; i < 3 && i < cap; i++) out[n++] = v[i];
+    check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    n = e->launches - e->window_launch;
+    const long long np = e->prepares - e->window_prepare;
+    if (n > celv2hip_engine::kRing || np > celv2hip_engine::kRing) throw Failure(CELV2_ERR_STATE, "stats window longer than the event ring");
+    double acc[7] = {};
+    for (long long i = e->window_launch; i < e->launches; i++) {
+      auto& a = e->ring[i % celv2hip_engine::kRing];
+      const int pairs[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {0, 5}};
+      for (int k = 0; k < 6; k++) {
+        float t = 0;
+        check(hipEventElapsedTime(&t, a[pairs[k][0]], a[pairs[k][1]]), "hipEventElapsedTime");
+        acc[k] += t;
+      }
+    }
+    for (long long i = e->window_prepare; i < e->prepares; i++) {
+      auto& a = e->pring[i % celv2hip_engine::kRing];
+      float t = 0;
+      check(hipEventElapsedTime(&t, a[0], a[1]), "hipEventElapsedTime");
+      acc[6] += t;
+    }
+    for (int k = 0; k < 7 && k < cap_ms; k++) ms[k] = acc[k];
+    check(hipMemcpy(e->counters_host, e->counters, 80 * 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "hipMemcpy counters");
+    unsigned long long searches = 0;
+    for (int i = 0; i < 64; i++) searches += e->counters_host[16 + 16 * i];
+    const long long v[4] = {n, static_cast<long long>(e->counters_host[0]), static_cast<long long>(searches),
+                            static_cast<long long>(e->counters_host[1])};
+    for (int k = 0; k < 4 && k < cap_counts; k++) counts[k] = v[k];
   });
-  return rc ? -rc : n;
+  return rc ? -rc : static_cast<int>(n);
 }
 
 int celv2hip_device_alloc(int device, long long bytes, void** out) {

@@ -23,6 +23,7 @@ struct MeLevelArgs {
   uint32_t* out[kMaxRefs];
   int nrefs, cols, rows, pcols, prows, level0;
   long long phase_bytes;
+  unsigned long long* searches;  // centres searched, spread over 64 slots 16 words apart (may be null)
 };
 void launch_me_level(const MeLevelArgs& a, hipStream_t s);
 

@@ -171,7 +171,10 @@ __global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
     }
     mvx = qcx, mvy = qcy;
   }
-  if (lane == 0) a.out[r][by * a.cols + bx] = pack_mv(mvx, mvy);
+  if (lane == 0) {
+    a.out[r][by * a.cols + bx] = pack_mv(mvx, mvy);
+    if (a.searches) atomicAdd(a.searches + 16 * (item & 63), two ? 2ull : 1ull);
+  }
 }
 
 }  // namespace

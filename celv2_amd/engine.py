@@ -45,8 +45,8 @@ def lib():
         L.celv2hip_wait_row.argtypes = [P, I, P]
         L.celv2hip_fetch_row.argtypes = [P, I, P]
         L.celv2hip_sync.argtypes = [P]
-        L.celv2hip_stage_times.argtypes = [P, ctypes.POINTER(ctypes.c_float), I]
-        L.celv2hip_counters.argtypes = [P, ctypes.POINTER(LL), I]
+        L.celv2hip_stats_reset.argtypes = [P]
+        L.celv2hip_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), I, ctypes.POINTER(LL), I]
         L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_device_alloc.argtypes = [I, LL, ctypes.POINTER(P)]
@@ -238,19 +238,19 @@ class RawEngine:
         k = c.value * r.value
         return mx[:k].reshape(r.value, c.value).copy(), my[:k].reshape(r.value, c.value).copy()
 
-    def stage_times(self):
-        buf = (ctypes.c_float * 6)()
-        n = lib().celv2hip_stage_times(self.h, buf, 6)
-        if n < 0:
-            raise EngineError(last_error())
-        return list(buf)[:n]
+    def stats_reset(self):
+        self._check(lib().celv2hip_stats_reset(self.h), "stats_reset")
 
-    def counters(self):
-        buf = (LL * 3)()
-        n = lib().celv2hip_counters(self.h, buf, 3)
+    def stats(self):
+        """(stage ms sums over the window, counts) - see celv2hip_stats."""
+        ms = (ctypes.c_double * 7)()
+        cnt = (LL * 4)()
+        n = lib().celv2hip_stats(self.h, ms, 7, cnt, 4)
         if n < 0:
             raise EngineError(last_error())
-        return list(buf)[:n]
+        keys = ["build", "me", "inter", "finalize", "copy", "total", "intra_prepare"]
+        ckeys = ["launches", "tile_evals", "me_centres", "compound_blocks"]
+        return dict(zip(keys, list(ms))), dict(zip(ckeys, list(cnt)))
 
 
 class DeviceBuffer:

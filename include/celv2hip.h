@@ -125,14 +125,15 @@ int celv2hip_wait_row(celv2hip_engine* e, int row, celv2_sb* out);
 /* Waits for all queued work (used before timing and in tests). */
 int celv2hip_sync(celv2hip_engine* e);
 
-/* Device time of the last launch's kernels by stage, in ms (HIP events on the engine's stream):
- * out[0] import+pyramid+phases, [1] motion search, [2] intra, [3] inter, [4] compound+finalize,
- * [5] total launch-to-last-row. Returns the number of values written. */
-int celv2hip_stage_times(celv2hip_engine* e, float* out, int cap);
-
-/* Counters of the last launch: out[0] unique (MV, reference, superblock) evaluations of the tile
- * scorer, out[1] ME searches (centres searched), out[2] compound blocks scored. */
-int celv2hip_counters(celv2hip_engine* e, long long* out, int cap);
+/* Statistics over the launches since the last reset (at most 512), measured on the engine's stream
+ * with HIP events, without host waits between launches. ms[0..6] sums, in milliseconds:
+ * [0] import + pyramids + phase planes queued by launch (a source not prepare()d includes its intra
+ * scores), [1] motion search (5 levels), [2] inter scores, [3] compound + finalize, [4] record copy,
+ * [5] launch start to end, [6] intra scores queued by prepare(). counts: [0] launches, [1] distinct
+ * (MV, reference, superblock) tile evaluations, [2] motion-search centres searched, [3] compound
+ * blocks scored. Returns the number of launches (negative status on failure). */
+int celv2hip_stats_reset(celv2hip_engine* e);
+int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts, int cap_counts);
 
 /* ---- Timing mode: device-resident throughput ------------------------------------------------ */
 
