@@ -279,7 +279,8 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     e->sb_cols = (width + 63) / 64, e->sb_rows = (height + 63) / 64;
     hipDeviceProp_t prop;
     check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-    e->name = prop.name;
+    e->name = prop.name[0] ? prop.name : prop.gcnArchName;
+    if (e->name.empty()) e->name = "HIP device " + std::to_string(device);
     check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
     int w = e->pw, h = e->ph;
     for (int l = 0; l < kLevels; l++) {

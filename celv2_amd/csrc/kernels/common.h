@@ -112,14 +112,37 @@ __device__ __forceinline__ void block_geom(int b, int& x, int& y, int& w, int& h
 
 __device__ __forceinline__ int mode_bits(int m) { return m == 0 ? 1 : (m < 4 ? 3 : 9); }  // intra_common.h:32
 
-// Wave-wide minimum of a 32-bit key (all 64 lanes take part).
+// Cross-lane moves within 16-lane rows (DPP, no LDS traffic): quad_perm [1,0,3,2] = lane ^ 1,
+// quad_perm [2,3,0,1] = lane ^ 2, row_ror:8 = lane ^ 8 (within a row), row_half_mirror maps lane i of
+// each 8 to 7 - i (the other quad of the 8, for values uniform within quads), row_ror:4 = lane + 4
+// mod 16.
+constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRor4 = 0x124, kDppRor8 = 0x128, kDppHalfMirror = 0x141;
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), kCtrl, 0xf, 0xf, false));
+}
+
+// Wave-wide minimum of a 32-bit key (all 64 lanes take part; the result is wave-uniform).
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) {
-    const uint32_t w = static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), o, 64));
-    v = w < v ? w : v;
-  }
-  return v;
+  v = min(v, dpp<kDppXor1>(v));
+  v = min(v, dpp<kDppXor2>(v));
+  v = min(v, dpp<kDppRor4>(v));
+  v = min(v, dpp<kDppRor8>(v));  // every lane: its row's minimum
+  const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
+  const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
+  const uint32_t r2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
+  const uint32_t r3 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
+  return min(min(r0, r1), min(r2, r3));
+}
+
+// Sums over aligned lane groups of 4 and of 8 (every lane of the group gets the sum).
+__device__ __forceinline__ uint32_t quad_sum(uint32_t v) {
+  v += dpp<kDppXor1>(v);
+  return v + dpp<kDppXor2>(v);
+}
+__device__ __forceinline__ uint32_t oct_sum(uint32_t v) {
+  v = quad_sum(v);
+  return v + dpp<kDppHalfMirror>(v);
 }
 
 // XCD-aware workgroup order: consecutive logical workgroups (neighbouring blocks, which read

@@ -27,6 +27,9 @@ __device__ uint8_t g_cand_cell[kBlocks * kMaxCands];     // region cell of candi
 __device__ __forceinline__ uint32_t xor_add(uint32_t v, int m) {
   return v + static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m, 64));
 }
+// Lane grid of the superblock's 8x8 squares: lane = 8 * sqy + sqx. Neighbour sums within a 16-lane
+// row by DPP: lane ^ 1 (next square across), lane ^ 8 (next square down), lane ^ 2, and lane ^ 4 for
+// values already equal over each quad.
 
 __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
   __shared__ uint32_t mv[kRegion * kRegion];
@@ -122,19 +125,19 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
       mysc[b3] = W, mysc[b3 + 1] = t[0][0] + t[0][1], mysc[b3 + 2] = t[1][0] + t[1][1];
       mysc[b3 + 3] = t[0][0] + t[1][0], mysc[b3 + 4] = t[0][1] + t[1][1];
       // Level 2 (16x16): lanes (2a..2a+1, 2b..2b+1).
-      const uint32_t hp = xor_add(W, 1), vp = xor_add(W, 8), quad = xor_add(hp, 8);
+      const uint32_t hp = W + dpp<kDppXor1>(W), vp = W + dpp<kDppRor8>(W), quad = hp + dpp<kDppRor8>(hp);
       const int b2 = (5 + 4 * (sqy >> 1) + (sqx >> 1)) * 5;
       if (!(sqx & 1) && !(sqy & 1)) mysc[b2] = quad, mysc[b2 + 1] = hp, mysc[b2 + 3] = vp;
       if (!(sqx & 1) && (sqy & 1)) mysc[b2 + 2] = hp;
       if ((sqx & 1) && !(sqy & 1)) mysc[b2 + 4] = vp;
       // Level 1 (32x32).
-      const uint32_t hp2 = xor_add(quad, 2), vp2 = xor_add(quad, 16), quad2 = xor_add(hp2, 16);
+      const uint32_t hp2 = quad + dpp<kDppXor2>(quad), vp2 = xor_add(quad, 16), quad2 = xor_add(hp2, 16);
       const int b1 = (1 + 2 * (sqy >> 2) + (sqx >> 2)) * 5;
       if (!(sqx & 3) && !(sqy & 3)) mysc[b1] = quad2, mysc[b1 + 1] = hp2, mysc[b1 + 3] = vp2;
       if (!(sqx & 3) && (sqy & 3) == 2) mysc[b1 + 2] = hp2;
       if ((sqx & 3) == 2 && !(sqy & 3)) mysc[b1 + 4] = vp2;
       // Level 0 (64x64).
-      const uint32_t hp1 = xor_add(quad2, 4), vp1 = xor_add(quad2, 32), quad1 = xor_add(hp1, 32);
+      const uint32_t hp1 = quad2 + dpp<kDppHalfMirror>(quad2), vp1 = xor_add(quad2, 32), quad1 = xor_add(hp1, 32);
       if (lane == 0) mysc[0] = quad1, mysc[1] = hp1, mysc[3] = vp1;
       if (lane == 32) mysc[2] = hp1;
       if (lane == 4) mysc[4] = vp1;

@@ -31,11 +31,7 @@ __device__ __forceinline__ uint32_t block_word(const DPlane& p, int x, int y, in
 __device__ __forceinline__ uint32_t centre_sad(const DPlane& ref, int x, int y, int row, uint32_t c0, uint32_t c1) {
   uint32_t a, b;
   load8u(ref.p + static_cast<ptrdiff_t>(y + row) * ref.stride + x, a, b);
-  uint32_t s = sad4(c1, b, sad4(c0, a, 0));
-  s += static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), 1, 64));
-  s += static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), 2, 64));
-  s += static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), 4, 64));
-  return s;
+  return oct_sum(sad4(c1, b, sad4(c0, a, 0)));
 }
 
 // Full search of +-kRange around (px, py) (me_search, src/gpu/inter_common.h:273-284): the wave's
@@ -161,9 +157,7 @@ __global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
       uint32_t pw[4];
 #pragma unroll
       for (int i = 0; i < 4; i++) pw[i] = load4u(plane + static_cast<ptrdiff_t>(Y + i) * ref.stride + X);
-      uint32_t s = static_cast<uint32_t>(satd4x4_rows(cw, pw));
-      s += static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), 1, 64));
-      s += static_cast<uint32_t>(__shfl_xor(static_cast<int>(s), 2, 64));
+      const uint32_t s = quad_sum(static_cast<uint32_t>(satd4x4_rows(cw, pw)));
       const uint32_t cost = s + (o ? kQpelPenalty : 0);
       const uint32_t key = wave_min_u32(act ? ((cost << 4) | static_cast<uint32_t>(o)) : 0xffffffffu);
       const int ob = static_cast<int>(key & 15);
