@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--e2e-steps", type=int, default=12)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the decision engine runs on a single MI355X (no multi-GPU path is built)",
@@ -87,6 +89,12 @@ def main():
     log(f"clip: {args.frames} frames {W}x{H} 10-bit in {time.perf_counter() - t:.1f}s")
     lam = engine.lambda_q8(args.qindex)
     dev = [engine.DeviceBuffer(f) for f in y16]  # resident 10-bit luma
+    if args.calibrate_copy_mib > 0:
+        blk = np.zeros(args.calibrate_copy_mib << 20, np.uint8)
+        a, b = engine.DeviceBuffer(blk), engine.DeviceBuffer(blk)
+        log(f"calibration copy: {b.copy_from(a)} bytes device to device")
+        a.free()
+        b.free()
     E = engine.RawEngine(W, H)
     E.set_resident(True)
     lum = [engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]

@@ -30,13 +30,6 @@ HIP_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
-    # Multi-byte loads only where provably aligned: merged byte loads at unaligned addresses return
-    # wrong bytes on the MI355X boxes this was measured on; unaligned data goes through explicit
-    # aligned dword loads + v_alignbyte (kernels/common.h).
-    "-Xclang",
-    "-target-feature",
-    "-Xclang",
-    "-unaligned-access-mode",
     "-Wall",
     "-Wno-unused-function",
 ]

@@ -522,6 +522,20 @@ int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, i
   });
 }
 
+long long celv2hip_fetch_intra(celv2hip_engine* e, uint32_t* out, long long cap) {
+  if (int rc = validate(e)) return rc;
+  long long n = 0;
+  const int rc = guarded([&] {
+    const SrcSlot& s = e->srcs[e->cur_set];
+    if (!e->launched || !s.intra) throw Failure(CELV2_ERR_STATE, "no launch");
+    n = static_cast<long long>(e->nsb()) * kBlocks * kModes;
+    if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "intra buffer too small");
+    check(hipStreamSynchronize(e->stream), "sync");
+    check(hipMemcpy(out, s.intra, static_cast<size_t>(n) * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy intra");
+  });
+  return rc ? rc : n;
+}
+
 int celv2hip_sync(celv2hip_engine* e) {
   if (int rc = validate(e)) return rc;
   return guarded([&] { check(hipStreamSynchronize(e->stream), "hipStreamSynchronize"); });
@@ -587,6 +601,14 @@ int celv2hip_copy_to_device(int device, void* dst, const void* src, long long by
   return guarded([&] {
     check(hipSetDevice(device), "hipSetDevice");
     check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice), "hipMemcpy");
+  });
+}
+
+int celv2hip_copy_on_device(int device, void* dst, const void* src, long long bytes) {
+  return guarded([&] {
+    check(hipSetDevice(device), "hipSetDevice");
+    check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   });
 }
 

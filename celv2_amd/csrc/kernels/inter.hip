@@ -97,6 +97,13 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
     const uint32_t* q = reinterpret_cast<const uint32_t*>(a.src.p + static_cast<ptrdiff_t>(oy + 8 * sqy + i) * a.src.stride + ox + 8 * sqx);
     cw[i][0] = q[0], cw[i][1] = q[1];
   }
+  TilePair cpair[2];  // the square's tile rows as packed pairs (tiles (ty, 0) low, (ty, 1) high)
+#pragma unroll
+  for (int ty = 0; ty < 2; ty++) {
+    const uint32_t ca[4] = {cw[4 * ty][0], cw[4 * ty + 1][0], cw[4 * ty + 2][0], cw[4 * ty + 3][0]};
+    const uint32_t cb[4] = {cw[4 * ty][1], cw[4 * ty + 1][1], cw[4 * ty + 2][1], cw[4 * ty + 3][1]};
+    cpair[ty] = tile_pair(ca, cb);
+  }
   const DPlane ref = a.ref[r];
   const int U = nuniq;
   uint32_t* const mysc = sc[wave];
@@ -110,15 +117,15 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
       uint32_t pw[8][2];
 #pragma unroll
       for (int i = 0; i < 8; i++) load8u(plane + static_cast<ptrdiff_t>(Y + 8 * sqy + i) * ref.stride + X + 8 * sqx, pw[i][0], pw[i][1]);
+      // Tiles (ty, 0) and (ty, 1) of the square in one packed pass per tile row.
       uint32_t t[2][2];
 #pragma unroll
-      for (int ty = 0; ty < 2; ty++)
-#pragma unroll
-        for (int tx = 0; tx < 2; tx++) {
-          const uint32_t c4[4] = {cw[4 * ty][tx], cw[4 * ty + 1][tx], cw[4 * ty + 2][tx], cw[4 * ty + 3][tx]};
-          const uint32_t p4[4] = {pw[4 * ty][tx], pw[4 * ty + 1][tx], pw[4 * ty + 2][tx], pw[4 * ty + 3][tx]};
-          t[ty][tx] = static_cast<uint32_t>(satd4x4_rows(c4, p4));
-        }
+      for (int ty = 0; ty < 2; ty++) {
+        const uint32_t pa[4] = {pw[4 * ty][0], pw[4 * ty + 1][0], pw[4 * ty + 2][0], pw[4 * ty + 3][0]};
+        const uint32_t pb[4] = {pw[4 * ty][1], pw[4 * ty + 1][1], pw[4 * ty + 2][1], pw[4 * ty + 3][1]};
+        const uint32_t two = satd4x4_pair(cpair[ty], pa, pb);
+        t[ty][0] = two & 0xffffu, t[ty][1] = two >> 16;
+      }
       // Level 3 (8x8 and its halves): this lane's square.
       const int b3 = (21 + 8 * sqy + sqx) * 5;
       const uint32_t W = t[0][0] + t[0][1] + t[1][0] + t[1][1];

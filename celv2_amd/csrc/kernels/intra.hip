@@ -1,11 +1,27 @@
 // Intra scores of a superblock: SATD of the 13 intra modes (block_mode_satd,
 // src/gpu/intra_common.h:149-163) for each of the 420 candidate blocks of 32x32 and below.
 //
-// One workgroup per superblock. The source around it (rows and columns -1 .. 95 from its origin, all
-// the neighbours any of its blocks reads) is staged in LDS through clamped reads; the mode loop is
-// uniform over the workgroup and lanes take 4x4 tiles from a static (block, tile) list, so a wave
-// works on one block of one mode; per-(block, mode) sums are LDS integer atomics (order-free).
+// The 4x4 Walsh-Hadamard transform is linear: for 16 tiles at once, (H (x) H) vec(cur - pred) is one
+// 16x16x16 matrix product. A sample v in 0..255 becomes the exact half 1024 + v by placing the byte
+// 0x64 above it, so v_mfma_f32_16x16x16_f16 computes every coefficient exactly in f32 (integers
+// below 2^24). Per 16 tiles: the source tiles' transform C = HH * (1024 + cur) once, then per mode
+// D = C - HH * (1024 + pred); |D| costs nothing on the f32 add.
+//
+// Work layout. The 2304 (block, 4x4 tile) pairs of a superblock are sorted by block shape (nine
+// shapes of 256 tiles each), a block's tiles contiguous, so every 16-tile "step" has one shape and
+// a block's tiles fill an aligned group of 2..16 columns of the product. Lane (j, g) = (lane & 15,
+// lane >> 4) holds row g of tile j: it predicts that row's four samples in each mode. A superblock is
+// split over 4 workgroups of 36 steps, at block boundaries; each wave takes every 4th step.
+//
+// Edges. The source around the superblock (rows and columns -1 .. 95) is staged in LDS twice. The
+// copies differ only in row -1 and column -1, so that every edge sample the reference reads through
+// edge_above / edge_left (intra_common.h:47-56) is a plain LDS byte: the above copy has row -1 = 128
+// at the frame top and column -1 = the clamped sample; the left copy has column -1 = 128 at the frame
+// left and row -1 = the clamped sample. The directional modes read two samples per predicted sample
+// at offsets from a static per-(shape, mode, row, column) table built from dir_pred's geometry.
+#include <algorithm>
 #include <stdexcept>
+#include <vector>
 
 #include "common.h"
 #include "launch.h"
@@ -14,158 +30,301 @@ namespace celv2hip {
 
 namespace {
 
-constexpr int kRegionSide = 97;  // -1 .. 95
-constexpr int kRegionStride = 100;
-constexpr int kIntraTiles = 2304;  // 4x4 tiles of the blocks of levels 1-3 (768 per level)
+constexpr int kRS = 104;               // region row stride (bytes): columns -4 .. 99
+constexpr int kRRows = 97;             // rows -1 .. 95
+constexpr int kRBytes = kRRows * kRS;  // one copy of the region
+constexpr int kShapes = 9;
+constexpr int kTilesPerShape = 256;
+constexpr int kIntraTiles = kShapes * kTilesPerShape;  // 2304
+constexpr int kSteps = kIntraTiles / 16;               // 144
+constexpr int kGroups = 4;                             // workgroups per superblock
+constexpr int kGroupSteps = kSteps / kGroups;          // 36
+constexpr int kMaxGroupBlocks = 272;
+constexpr int kDirModes = 6;                           // modes 7..12
+constexpr int kDirEntries = kDirModes * 2688;          // 2688 = samples of the nine shapes
 
-__device__ uint32_t g_intra_tiles[kIntraTiles];  // b | tx << 16 | ty << 24 (tile origin in 4-px units)
-
-struct Region {
-  const uint8_t* r;  // LDS, sample (y, x) relative to the superblock at r[(y + 1) * stride + x + 1]
-  __device__ __forceinline__ int at(int y, int x) const { return r[(y + 1) * kRegionStride + x + 1]; }
+struct ShapeInfo {
+  int w, h, lw, lh;     // size and log2 size
+  int den;              // smooth: w + h + 2
+  uint32_t magic;       // ceil(2^32 / den): floor(n / den) = mulhi(n, magic) for n < 2^16
+  int seg;              // tiles per block, at most 16
+  int dir;              // first directional table entry
 };
 
-// intra_common.h:47-56 with the superblock origin (ox, oy): the frame's top/left edges read 128.
-__device__ __forceinline__ int edge_above(const Region& s, int ox, int oy, int x, int y, int k) {
-  if (oy + y == 0) return 128;
-  if (k < 0 && ox + x == 0) return s.at(y - 1, x);
-  return s.at(y - 1, x + k);
-}
-__device__ __forceinline__ int edge_left(const Region& s, int ox, int oy, int x, int y, int k) {
-  if (ox + x == 0) return 128;
-  if (k < 0 && oy + y == 0) return s.at(y, x - 1);
-  return s.at(y + k, x - 1);
-}
+__constant__ ShapeInfo g_shape[kShapes];
+__constant__ int g_group_pos[kGroups + 1];  // block positions (shape-sorted order) of each group
+__device__ uint32_t g_itile[kIntraTiles];   // x/4 | y/4 << 4 | tx << 8 | ty << 11 | pos << 14
+__device__ uint16_t g_pos_block[420];       // block index of each position
+__device__ __align__(16) uint32_t g_dir[kDirEntries];  // uint16 offset | f << 16 | step << 21
 
-__device__ __forceinline__ int floor_div64(int v) { return v >= 0 ? v >> 6 : -((-v + 63) >> 6); }
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// dir_pred (intra_common.h:61-85) for the angles 45, 67 (z1), 113, 135, 157 (z2), 203 (z3).
-__device__ __forceinline__ int directional(const Region& s, int ox, int oy, int x, int y, int w, int h, int angle, int i,
-                                           int j) {
-  const int last = w + h - 1;
-  auto deriv = [](int a) { return a == 23 ? 151 : a == 45 ? 64 : 27; };
-  if (angle < 90) {
-    const int pos = (i + 1) * deriv(angle), base = (pos >> 6) + j, f = (pos & 63) >> 1;
-    const int a0 = edge_above(s, ox, oy, x, y, min(base, last)), a1 = edge_above(s, ox, oy, x, y, min(base + 1, last));
-    return (a0 * (32 - f) + a1 * f + 16) >> 5;
-  }
-  if (angle > 180) {
-    const int pos = (j + 1) * deriv(270 - angle), base = (pos >> 6) + i, f = (pos & 63) >> 1;
-    const int l0 = edge_left(s, ox, oy, x, y, min(base, last)), l1 = edge_left(s, ox, oy, x, y, min(base + 1, last));
-    return (l0 * (32 - f) + l1 * f + 16) >> 5;
-  }
-  const int px = 64 * j - (i + 1) * deriv(180 - angle), bx = floor_div64(px);
-  if (bx >= -1) {
-    const int f = (px - 64 * bx) >> 1;
-    return (edge_above(s, ox, oy, x, y, bx) * (32 - f) + edge_above(s, ox, oy, x, y, bx + 1) * f + 16) >> 5;
-  }
-  const int py = 64 * i - (j + 1) * deriv(angle - 90), by = floor_div64(py);
-  const int f = (py - 64 * by) >> 1;
-  return (edge_left(s, ox, oy, x, y, max(by, -1)) * (32 - f) + edge_left(s, ox, oy, x, y, max(by + 1, -1)) * f + 16) >> 5;
+__device__ __forceinline__ half4 as_half4(uint32_t lo, uint32_t hi) {
+  return __builtin_bit_cast(half4, static_cast<uint64_t>(hi) << 32 | lo);
+}
+__device__ __forceinline__ float as_f32(uint32_t v) { return __builtin_bit_cast(float, v); }
+__device__ __forceinline__ uint32_t as_u32f(float v) { return __builtin_bit_cast(uint32_t, v); }
+
+// Bytes 0,1 (lo) or 2,3 (hi) of x as halves 1024 + byte.
+constexpr uint32_t kHalfBias = 0x64646464u;
+__device__ __forceinline__ uint32_t halves01(uint32_t x) { return __builtin_amdgcn_perm(kHalfBias, x, 0x04010400u); }
+__device__ __forceinline__ uint32_t halves23(uint32_t x) { return __builtin_amdgcn_perm(kHalfBias, x, 0x04030402u); }
+// Low 16 bits of a and b as one dword (a low).
+__device__ __forceinline__ uint32_t pack_lo16(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
+
+__device__ __forceinline__ f32x4 mfma(half4 a, half4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
 }
 
-// intra_pred (intra_common.h:89-118), modes 1..12; mode 0 (DC) is the block's dc value.
-__device__ __forceinline__ int predict(const Region& s, int ox, int oy, int x, int y, int w, int h, int mode, int i,
-                                       int j, int dc) {
-  const bool up = oy + y > 0, lf = ox + x > 0;
-  switch (mode) {
-    case 0: return dc;
-    case 1: return up ? s.at(y - 1, x + j) : 128;
-    case 2: return lf ? s.at(y + i, x - 1) : 128;
-    case 3: {
-      const int a = up ? s.at(y - 1, x + j) : 128, l = lf ? s.at(y + i, x - 1) : 128;
-      const int tr = up ? s.at(y - 1, x + w - 1) : 128, bl = lf ? s.at(y + h - 1, x - 1) : 128;
-      const int den = w + h + 2;
-      return ((h - i) * a + (i + 1) * bl + (w - j) * l + (j + 1) * tr + den / 2) / den;
+// Lane (i, g)'s part of the Hadamard operand: HH[i][4g + k], k = 0..3, as halves +-1.
+__device__ __forceinline__ half4 hadamard_operand(int lane, bool negate) {
+  const int i = lane & 15, g = lane >> 4;
+  uint32_t w[2];
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    uint32_t v = 0;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int k = 2 * p + h;
+      const int par = (__builtin_popcount((i >> 2) & g) + __builtin_popcount((i & 3) & k) + (negate ? 1 : 0)) & 1;
+      v |= (par ? 0xBC00u : 0x3C00u) << (16 * h);
     }
-    case 4:
-      return ((h - 1 - i) * edge_above(s, ox, oy, x, y, j) + (i + 1) * edge_left(s, ox, oy, x, y, h - 1) + h / 2) / h;
-    case 5:
-      return ((w - 1 - j) * edge_left(s, ox, oy, x, y, i) + (j + 1) * edge_above(s, ox, oy, x, y, w - 1) + w / 2) / w;
-    case 6: {
-      const int a = edge_above(s, ox, oy, x, y, j), l = edge_left(s, ox, oy, x, y, i), tl = edge_above(s, ox, oy, x, y, -1);
-      const int base = a + l - tl, pa = abs(base - a), pl = abs(base - l), ptl = abs(base - tl);
-      return (pl <= pa && pl <= ptl) ? l : (pa <= ptl ? a : tl);
-    }
-    case 7: return directional(s, ox, oy, x, y, w, h, 45, i, j);
-    case 8: return directional(s, ox, oy, x, y, w, h, 67, i, j);
-    case 9: return directional(s, ox, oy, x, y, w, h, 113, i, j);
-    case 10: return directional(s, ox, oy, x, y, w, h, 135, i, j);
-    case 11: return directional(s, ox, oy, x, y, w, h, 157, i, j);
-    default: return directional(s, ox, oy, x, y, w, h, 203, i, j);
+    w[p] = v;
   }
+  return as_half4(w[0], w[1]);
 }
 
 __global__ void __launch_bounds__(256) k_intra(DPlane src, int sb_cols, int row0, uint32_t* __restrict__ out) {
-  __shared__ uint8_t reg[kRegionSide * kRegionStride];
-  __shared__ int dcv[kBlocks];
-  __shared__ uint32_t acc[kBlocks * kModes];
-  const int tid = threadIdx.x;
-  const int sbx = blockIdx.x, sby = row0 + blockIdx.y;
+  __shared__ __align__(16) uint8_t reg[2 * kRBytes];  // above copy, left copy
+  __shared__ float acc[kMaxGroupBlocks * kModes];
+  __shared__ uint32_t dcv[kMaxGroupBlocks];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int q = blockIdx.x & (kGroups - 1), sbx = blockIdx.x >> 2, sby = row0 + blockIdx.y;
   const int ox = 64 * sbx, oy = 64 * sby;
-  for (int k = tid; k < kRegionSide * kRegionSide; k += 256) {
-    const int ry = k / kRegionSide - 1, rx = k % kRegionSide - 1;
-    const int yy = clampi(oy + ry, 0, src.h - 1), xx = clampi(ox + rx, 0, src.w - 1);
-    reg[(ry + 1) * kRegionStride + rx + 1] = src.p[static_cast<ptrdiff_t>(yy) * src.stride + xx];
+  const int p0 = g_group_pos[q], np = g_group_pos[q + 1] - p0;
+
+  // Stage: region row r (frame row oy + r - 1, clamped), dword d = columns ox - 4 + 4d .. + 3 (a
+  // dword entirely right of the frame reads the replicated border at w + 60).
+  uint32_t* ra = reinterpret_cast<uint32_t*>(reg);
+  uint32_t* rl = reinterpret_cast<uint32_t*>(reg + kRBytes);
+  for (int k = tid; k < kRRows * (kRS / 4); k += 256) {
+    const int r = k / (kRS / 4), d = k - r * (kRS / 4);
+    const int yy = clampi(oy + r - 1, 0, src.h - 1), xx = min(ox - 4 + 4 * d, src.w + 60);
+    const uint32_t v = *reinterpret_cast<const uint32_t*>(src.p + static_cast<ptrdiff_t>(yy) * src.stride + xx);
+    ra[k] = v, rl[k] = v;
   }
-  for (int k = tid; k < kBlocks * kModes; k += 256) acc[k] = 0;
+  for (int k = tid; k < np * kModes; k += 256) acc[k] = 0.f;
   __syncthreads();
-  const Region s{reg};
-  // dc_value (intra_common.h:120-128) of every block of levels 1-3.
-  for (int b = 5 + tid; b < kBlocks; b += 256) {
+  if (oy == 0 && tid < kRS / 4) ra[tid] = 0x80808080u;    // above copy: row -1 is 128 at the frame top
+  if (ox == 0 && tid < kRRows) reg[kRBytes + tid * kRS + 3] = 128;  // left copy: column -1 at the frame left
+  __syncthreads();
+
+  // dc_value (intra_common.h:120-128) of the group's blocks.
+  for (int i = tid; i < np; i += 256) {
     int x, y, w, h;
-    block_geom(b, x, y, w, h);
-    int sum = 0, n = 0;
-    if (oy + y > 0)
-      for (int j = 0; j < w; j++) sum += s.at(y - 1, x + j), n++;
-    if (ox + x > 0)
-      for (int i = 0; i < h; i++) sum += s.at(y + i, x - 1), n++;
-    dcv[b] = n ? (sum + n / 2) / n : 128;
+    block_geom(g_pos_block[p0 + i], x, y, w, h);
+    const int blk = (y + 1) * kRS + x + 4;
+    uint32_t sum = 0, n = 0;
+    if (oy + y > 0) {
+      for (int j = 0; j < w; j += 4) sum = sad4(*reinterpret_cast<const uint32_t*>(reg + blk - kRS + j), 0u, sum);
+      n += w;
+    }
+    if (ox + x > 0) {
+      for (int r = 0; r < h; r++) sum += reg[kRBytes + blk + r * kRS - 1];
+      n += h;
+    }
+    dcv[i] = n ? (sum + n / 2) / n : 128u;
   }
   __syncthreads();
-  for (int m = 0; m < kModes; m++) {
-    for (int slot = tid; slot < kIntraTiles; slot += 256) {
-      const uint32_t e = g_intra_tiles[slot];
-      const int b = static_cast<int>(e & 0xffff), tx = static_cast<int>((e >> 16) & 255), ty = static_cast<int>(e >> 24);
-      int x, y, w, h;
-      block_geom(b, x, y, w, h);
-      int d[16];
+
+  const half4 hh = hadamard_operand(lane, false), nhh = hadamard_operand(lane, true);
+  const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+  const int j16 = lane & 15, g = lane >> 4;
+  for (int step = kGroupSteps * q + wave; step < kGroupSteps * (q + 1); step += 4) {
+    const ShapeInfo sh = g_shape[step >> 4];
+    const uint32_t e = g_itile[16 * step + j16];
+    const int x = 4 * static_cast<int>(e & 15), y = 4 * static_cast<int>((e >> 4) & 15);
+    const int pj0 = 4 * static_cast<int>((e >> 8) & 7), pi = 4 * static_cast<int>((e >> 11) & 7) + g;
+    const int lp = static_cast<int>(e >> 14) - p0;
+    const int w = sh.w, h = sh.h;
+    const int blk = (y + 1) * kRS + x + 4;  // offset of sample (y, x) in the above copy
+    const uint32_t cur = *reinterpret_cast<const uint32_t*>(reg + blk + pi * kRS + pj0);
+    const uint32_t a4 = *reinterpret_cast<const uint32_t*>(reg + blk - kRS + pj0);  // above[pj0 .. pj0 + 3]
+    const int lft = reg[kRBytes + blk + pi * kRS - 1];                              // left[pi]
+    const int tr = reg[blk - kRS + w - 1], bl = reg[kRBytes + blk + (h - 1) * kRS - 1];
+    const int tl = reg[blk - kRS - 1];
+    const int dc = static_cast<int>(dcv[lp]);
+    const f32x4 ccur = mfma(hh, as_half4(halves01(cur), halves23(cur)), zero);
+    int a[4];
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+    for (int jj = 0; jj < 4; jj++) a[jj] = static_cast<int>((a4 >> (8 * jj)) & 255);
+
+    auto score = [&](int m, uint32_t p01, uint32_t p23) {
+      const f32x4 d = mfma(nhh, as_half4(p01, p23), ccur);
+      float s = __builtin_fabsf(d[0]) + __builtin_fabsf(d[1]) + __builtin_fabsf(d[2]) + __builtin_fabsf(d[3]);
+      // Sum over the block's tiles (an aligned group of sh.seg columns of this row).
+      if (sh.seg >= 2) s += as_f32(dpp<kDppXor1>(as_u32f(s)));
+      if (sh.seg >= 4) s += as_f32(dpp<kDppXor2>(as_u32f(s)));
+      if (sh.seg >= 8) s += as_f32(dpp<kDppHalfMirror>(as_u32f(s)));
+      if (sh.seg >= 16) s += as_f32(dpp<kDppMirror>(as_u32f(s)));
+      if ((j16 & (sh.seg - 1)) == 0) atomicAdd(&acc[lp * kModes + m], s);
+    };
+
+    // 0 DC, 1 V, 2 H (intra_common.h:89-96).
+    {
+      const uint32_t pd = (static_cast<uint32_t>(dc) | 0x6400u) * 0x10001u;
+      score(0, pd, pd);
+      score(1, halves01(a4), halves23(a4));
+      const uint32_t ph = (static_cast<uint32_t>(lft) | 0x6400u) * 0x10001u;
+      score(2, ph, ph);
+    }
+    // 3 Smooth: (h-i) a[j] + (i+1) bl + (w-j) l[i] + (j+1) tr, rounded, / (w + h + 2).
+    {
+      const int base = (pi + 1) * bl + (w - pj0) * lft + (pj0 + 1) * tr + sh.den / 2, dl = tr - lft, al = h - pi;
+      uint32_t v[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int pi = 4 * ty + i, pj = 4 * tx + j;
-          d[4 * i + j] = s.at(y + pi, x + pj) - predict(s, ox, oy, x, y, w, h, m, pi, pj, dcv[b]);
-        }
-      atomicAdd(&acc[b * kModes + m], static_cast<uint32_t>(wht4_abs_sum(d)));
+      for (int jj = 0; jj < 4; jj++) v[jj] = __umulhi(static_cast<uint32_t>(al * a[jj] + base + jj * dl), sh.magic);
+      score(3, pack_lo16(v[0], v[1]) | 0x64006400u, pack_lo16(v[2], v[3]) | 0x64006400u);
+    }
+    // 4 Smooth V: ((h-1-i) a[j] + (i+1) left[h-1] + h/2) / h.
+    {
+      const int c = (pi + 1) * bl + h / 2 + (0x6400 << sh.lh), al = h - 1 - pi;
+      uint32_t v[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) v[jj] = static_cast<uint32_t>((al * a[jj] + c) >> sh.lh);
+      score(4, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+    }
+    // 5 Smooth H: ((w-1-j) l[i] + (j+1) above[w-1] + w/2) / w.
+    {
+      const int c = (w - 1 - pj0) * lft + (pj0 + 1) * tr + w / 2 + (0x6400 << sh.lw), dl = tr - lft;
+      uint32_t v[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) v[jj] = static_cast<uint32_t>((c + jj * dl) >> sh.lw);
+      score(5, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+    }
+    // 6 Paeth: the neighbour nearest to above + left - top-left, ties to left, then above.
+    {
+      const int pa = abs(lft - tl);
+      uint32_t v[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        const int pl = abs(a[jj] - tl), ptl = abs(a[jj] + lft - 2 * tl);
+        const int p = (pl <= pa && pl <= ptl) ? lft : (pa <= ptl ? a[jj] : tl);
+        v[jj] = static_cast<uint32_t>(p) | 0x6400u;
+      }
+      score(6, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+    }
+    // 7..12 directional (intra_common.h:61-85): two samples and a 5-bit weight per predicted sample.
+#pragma unroll
+    for (int mm = 0; mm < kDirModes; mm++) {
+      const uint4 t = *reinterpret_cast<const uint4*>(g_dir + sh.dir + (mm * h + pi) * w + pj0);
+      const uint32_t te[4] = {t.x, t.y, t.z, t.w};
+      uint32_t v[4];
+#pragma unroll
+      for (int jj = 0; jj < 4; jj++) {
+        const int o = blk + static_cast<int>(static_cast<int16_t>(te[jj] & 0xffffu));
+        const int f = static_cast<int>((te[jj] >> 16) & 31), st = static_cast<int>(te[jj] >> 21);
+        const int e0 = reg[o], e1 = reg[o + st];
+        v[jj] = static_cast<uint32_t>(e0 + (((e1 - e0) * f + 16) >> 5) + 0x6400);
+      }
+      score(7 + mm, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
     }
   }
   __syncthreads();
   uint32_t* o = out + (static_cast<size_t>(sby) * sb_cols + sbx) * kBlocks * kModes;
-  for (int k = 5 * kModes + tid; k < kBlocks * kModes; k += 256) o[k] = acc[k];
+  for (int k = tid; k < np * kModes; k += 256) {
+    const int i = k / kModes, m = k - i * kModes;
+    o[g_pos_block[p0 + i] * kModes + m] = static_cast<uint32_t>(acc[k]);
+  }
+}
+
+// dir_pred's sample choice (intra_common.h:61-85) for sample (i, j) of a w x h block: the edge
+// (0 above, 1 left), the two indices along it, and the 5-bit weight of the second.
+struct DirSample {
+  int edge, k0, k1, f;
+};
+DirSample dir_sample(int w, int h, int angle, int i, int j) {
+  const int last = w + h - 1;
+  auto deriv = [](int a) { return a == 23 ? 151 : a == 45 ? 64 : 27; };
+  auto fdiv64 = [](int v) { return v >= 0 ? v / 64 : -((-v + 63) / 64); };
+  if (angle < 90) {
+    const int pos = (i + 1) * deriv(angle), base = (pos >> 6) + j;
+    return {0, std::min(base, last), std::min(base + 1, last), (pos & 63) >> 1};
+  }
+  if (angle > 180) {
+    const int pos = (j + 1) * deriv(270 - angle), base = (pos >> 6) + i;
+    return {1, std::min(base, last), std::min(base + 1, last), (pos & 63) >> 1};
+  }
+  const int px = 64 * j - (i + 1) * deriv(180 - angle), bx = fdiv64(px);
+  if (bx >= -1) return {0, bx, bx + 1, (px - 64 * bx) >> 1};
+  const int py = 64 * i - (j + 1) * deriv(angle - 90), by = fdiv64(py);
+  return {1, std::max(by, -1), std::max(by + 1, -1), (py - 64 * by) >> 1};
 }
 
 }  // namespace
 
 void launch_intra(const DPlane& src, int sb_cols, int row0, int nrows, uint32_t* out, hipStream_t s) {
   if (nrows <= 0) return;
-  k_intra<<<dim3(sb_cols, nrows), 256, 0, s>>>(src, sb_cols, row0, out);
+  k_intra<<<dim3(kGroups * sb_cols, nrows), 256, 0, s>>>(src, sb_cols, row0, out);
 }
 
-// The static (block, tile) list of levels 1-3 for k_intra.
+// The shape-sorted tile list, the block order, the group ranges and the directional sample table.
 void upload_intra_tiles(hipStream_t s) {
-  static uint32_t table[kIntraTiles];
-  int n = 0;
-  for (int b = 5; b < kBlocks; b++) {
-    const int sq = b / 5, kind = b % 5;
-    const int level = sq >= 21 ? 3 : sq >= 5 ? 2 : 1;
-    const int size = 64 >> level;
-    const int w = (kind == 3 || kind == 4) ? size / 2 : size, h = (kind == 1 || kind == 2) ? size / 2 : size;
-    for (int ty = 0; ty < h / 4; ty++)
-      for (int tx = 0; tx < w / 4; tx++) table[n++] = static_cast<uint32_t>(b) | (static_cast<uint32_t>(tx) << 16) | (static_cast<uint32_t>(ty) << 24);
+  static const int kShapeWH[kShapes][2] = {{32, 32}, {32, 16}, {16, 32}, {16, 16}, {16, 8}, {8, 16}, {8, 8}, {8, 4}, {4, 8}};
+  static const int kAngles[kDirModes] = {45, 67, 113, 135, 157, 203};
+  static uint32_t tiles[kIntraTiles];
+  static uint16_t pos_block[420];
+  static uint32_t dir[kDirEntries];
+  static ShapeInfo shapes[kShapes];
+  static int group_pos[kGroups + 1];
+  std::vector<int> tile_pos;
+  int nt = 0, np = 0, nd = 0;
+  for (int si = 0; si < kShapes; si++) {
+    const int w = kShapeWH[si][0], h = kShapeWH[si][1];
+    ShapeInfo& sh = shapes[si];
+    sh.w = w, sh.h = h, sh.lw = __builtin_ctz(w), sh.lh = __builtin_ctz(h), sh.den = w + h + 2;
+    sh.magic = static_cast<uint32_t>(((uint64_t(1) << 32) + sh.den - 1) / sh.den);
+    sh.seg = std::min(16, (w / 4) * (h / 4)), sh.dir = nd;
+    for (int b = 5; b < kBlocks; b++) {
+      const int sq = b / 5, kind = b % 5, level = sq >= 21 ? 3 : sq >= 5 ? 2 : 1;
+      const int size = 64 >> level, first = ((1 << (2 * level)) - 1) / 3, i = sq - first, n = 1 << level;
+      const int bw = (kind == 3 || kind == 4) ? size / 2 : size, bh = (kind == 1 || kind == 2) ? size / 2 : size;
+      if (bw != w || bh != h) continue;
+      const int bx = (i & (n - 1)) * size + (kind == 4 ? size / 2 : 0), by = (i >> level) * size + (kind == 2 ? size / 2 : 0);
+      for (int ty = 0; ty < h / 4; ty++)
+        for (int tx = 0; tx < w / 4; tx++) {
+          tiles[nt++] = static_cast<uint32_t>(bx / 4) | static_cast<uint32_t>(by / 4) << 4 | static_cast<uint32_t>(tx) << 8 |
+                        static_cast<uint32_t>(ty) << 11 | static_cast<uint32_t>(np) << 14;
+          tile_pos.push_back(np);
+        }
+      pos_block[np++] = static_cast<uint16_t>(b);
+    }
+    if (nt != kTilesPerShape * (si + 1)) throw std::logic_error("intra shape tile count");
+    for (int mm = 0; mm < kDirModes; mm++)
+      for (int i = 0; i < h; i++)
+        for (int j = 0; j < w; j++) {
+          const DirSample d = dir_sample(w, h, kAngles[mm], i, j);
+          // Offsets from sample (0, 0) of the block in the above copy; the left copy follows it.
+          const int off = d.edge == 0 ? -kRS + d.k0 : kRBytes + d.k0 * kRS - 1;
+          const int step = d.edge == 0 ? d.k1 - d.k0 : (d.k1 - d.k0) * kRS;
+          if (off < -32768 || off > 32767 || step < 0 || step > 127 || d.f < 0 || d.f > 31) throw std::logic_error("intra direction entry");
+          dir[nd++] = (static_cast<uint32_t>(off) & 0xffffu) | static_cast<uint32_t>(d.f) << 16 | static_cast<uint32_t>(step) << 21;
+        }
   }
-  if (n != kIntraTiles) throw std::logic_error("intra tile table size");
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_intra_tiles), table, sizeof(table), 0, hipMemcpyHostToDevice, s);
+  if (nt != kIntraTiles || np != 420 || nd != kDirEntries) throw std::logic_error("intra table size");
+  for (int q = 0; q <= kGroups; q++) group_pos[q] = q < kGroups ? tile_pos[16 * kGroupSteps * q] : np;
+  for (int q = 0; q < kGroups; q++) {
+    // A block's tiles must not straddle two groups, and a group must fit the LDS accumulators.
+    const int last = tile_pos[16 * kGroupSteps * (q + 1) - 1];
+    if (last + 1 != group_pos[q + 1] || group_pos[q + 1] - group_pos[q] > kMaxGroupBlocks) throw std::logic_error("intra group split");
+  }
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_itile), tiles, sizeof(tiles), 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_pos_block), pos_block, sizeof(pos_block), 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_dir), dir, sizeof(dir), 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_shape), shapes, sizeof(shapes), 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_group_pos), group_pos, sizeof(group_pos), 0, hipMemcpyHostToDevice, s);
 }
 
 }  // namespace celv2hip

@@ -49,9 +49,12 @@ def lib():
         L.celv2hip_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), I, ctypes.POINTER(LL), I]
         L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.celv2hip_fetch_intra.argtypes = [P, P, ctypes.c_longlong]
+        L.celv2hip_fetch_intra.restype = ctypes.c_longlong
         L.celv2hip_device_alloc.argtypes = [I, LL, ctypes.POINTER(P)]
         L.celv2hip_device_free.argtypes = [I, P]
         L.celv2hip_copy_to_device.argtypes = [I, P, P, LL]
+        L.celv2hip_copy_on_device.argtypes = [I, P, P, LL]
         L.celv2_engine_last_error.restype = ctypes.c_char_p
         L.celv2_engine_create.argtypes = [I, I, I, I, I]
         L.celv2_engine_create.restype = P
@@ -238,6 +241,15 @@ class RawEngine:
         k = c.value * r.value
         return mx[:k].reshape(r.value, c.value).copy(), my[:k].reshape(r.value, c.value).copy()
 
+    def fetch_intra(self):
+        """Intra SATDs of the last launch's source: (superblocks, 425, 13); blocks 0..4 undefined."""
+        n = self.sb_cols * self.sb_rows * 425 * 13
+        out = np.zeros(n, np.uint32)
+        got = lib().celv2hip_fetch_intra(self.h, out.ctypes.data, n)
+        if got < 0:
+            self._check(int(got), "fetch_intra")
+        return out.reshape(self.sb_cols * self.sb_rows, 425, 13)
+
     def stats_reset(self):
         self._check(lib().celv2hip_stats_reset(self.h), "stats_reset")
 
@@ -265,6 +277,13 @@ class DeviceBuffer:
         self.ptr = p.value
         if lib().celv2hip_copy_to_device(device, self.ptr, np.ascontiguousarray(array).ctypes.data, array.nbytes) != 0:
             raise EngineError(last_error())
+
+    def copy_from(self, other):
+        """Device-to-device copy of other's bytes into this buffer (a copy kernel of known size)."""
+        n = min(self.nbytes, other.nbytes)
+        if lib().celv2hip_copy_on_device(self.device, self.ptr, other.ptr, n) != 0:
+            raise EngineError(last_error())
+        return n
 
     def free(self):
         if self.ptr:

@@ -155,10 +155,18 @@ int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint
 /* Motion field of reference `ref` at `level` (cols x rows, level 0 in quarter pels). */
 int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, int32_t* mvy, int* cols, int* rows);
 
+/* Intra scores of the last launch's source: per superblock (row-major), per candidate block 5..424,
+ * the 13 modes' SATD (block_mode_satd, src/gpu/intra_common.h:149-163); entries of blocks 0..4 (the
+ * 64x64 level, inter only) are undefined. `cap` counts uint32 entries; returns the entries written
+ * (superblocks x 425 x 13) or a negative status. */
+long long celv2hip_fetch_intra(celv2hip_engine* e, uint32_t* out, long long cap);
+
 /* ---- Device memory helpers for callers without another allocator --------------------------- */
 int celv2hip_device_alloc(int device, long long bytes, void** out);
 int celv2hip_device_free(int device, void* p);
 int celv2hip_copy_to_device(int device, void* dst, const void* src, long long bytes);
+/* Device-to-device copy of `bytes` (a plain copy kernel of known size, for calibrating traffic counters). */
+int celv2hip_copy_on_device(int device, void* dst, const void* src, long long bytes);
 
 
 /* ---- The C++ engine (celv2::HipEngine, include/celv2/hip_engine.h) behind a handle ------------

@@ -1,5 +1,7 @@
 """Each stage of the HIP engine against its CPU rule (CelV2 checks each Vulkan kernel the same way,
 tests/test_vk_kernels.cpp: downsample_matches_ref, subpel_matches_ref, me_matches_ref)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -53,3 +55,26 @@ def test_fields_match(launched, size):
             gx, gy = e.fetch_field(r, l)
             bad = np.argwhere((gx != want[l][0]) | (gy != want[l][1]))
             assert bad.size == 0, (r, l, bad[:5].tolist(), [(int(gx[i, j]), int(gy[i, j]), int(want[l][0][i, j]), int(want[l][1][i, j])) for i, j in bad[:5]])
+
+
+@pytest.mark.parametrize("size", [(352, 288), (202, 122)])
+def test_intra_scores_match(launched, size):
+    """All 13 modes' SATD of every candidate block of levels 1-3 (block_mode_satd,
+    src/gpu/intra_common.h:149-163), frame edges and the partial right/bottom superblocks included."""
+    e, y8 = launched[size]
+    O = ol.oracle()
+    src = np.ascontiguousarray(y8[2])
+    H, W = src.shape
+    got = e.fetch_intra()
+    xywh = (ctypes.c_int * 4)()
+    bad = []
+    for sb in range(got.shape[0]):
+        ox, oy = 64 * (sb % e.sb_cols), 64 * (sb // e.sb_cols)
+        for b in range(5, 425):
+            O.orc_block_geom(b, xywh)
+            x, y, bw, bh = xywh
+            for m in range(13):
+                want = O.orc_intra_satd(src.ctypes.data, W, H, ox + x, oy + y, bw, bh, m)
+                if got[sb, b, m] != want:
+                    bad.append((sb, b, m, int(got[sb, b, m]), want))
+    assert not bad, (len(bad), bad[:8])
