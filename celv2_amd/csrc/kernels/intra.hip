@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "common.h"
+#include "hadamard.h"
 #include "launch.h"
 
 namespace celv2hip {
@@ -56,44 +57,6 @@ __constant__ int g_group_pos[kGroups + 1];  // block positions (shape-sorted ord
 __device__ uint32_t g_itile[kIntraTiles];   // x/4 | y/4 << 4 | tx << 8 | ty << 11 | pos << 14
 __device__ uint16_t g_pos_block[420];       // block index of each position
 __device__ __align__(16) uint32_t g_dir[kDirEntries];  // uint16 offset | f << 16 | step << 21
-
-typedef _Float16 half4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ half4 as_half4(uint32_t lo, uint32_t hi) {
-  return __builtin_bit_cast(half4, static_cast<uint64_t>(hi) << 32 | lo);
-}
-__device__ __forceinline__ float as_f32(uint32_t v) { return __builtin_bit_cast(float, v); }
-__device__ __forceinline__ uint32_t as_u32f(float v) { return __builtin_bit_cast(uint32_t, v); }
-
-// Bytes 0,1 (lo) or 2,3 (hi) of x as halves 1024 + byte.
-constexpr uint32_t kHalfBias = 0x64646464u;
-__device__ __forceinline__ uint32_t halves01(uint32_t x) { return __builtin_amdgcn_perm(kHalfBias, x, 0x04010400u); }
-__device__ __forceinline__ uint32_t halves23(uint32_t x) { return __builtin_amdgcn_perm(kHalfBias, x, 0x04030402u); }
-// Low 16 bits of a and b as one dword (a low).
-__device__ __forceinline__ uint32_t pack_lo16(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(b, a, 0x05040100u); }
-
-__device__ __forceinline__ f32x4 mfma(half4 a, half4 b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, c, 0, 0, 0);
-}
-
-// Lane (i, g)'s part of the Hadamard operand: HH[i][4g + k], k = 0..3, as halves +-1.
-__device__ __forceinline__ half4 hadamard_operand(int lane, bool negate) {
-  const int i = lane & 15, g = lane >> 4;
-  uint32_t w[2];
-#pragma unroll
-  for (int p = 0; p < 2; p++) {
-    uint32_t v = 0;
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-      const int k = 2 * p + h;
-      const int par = (__builtin_popcount((i >> 2) & g) + __builtin_popcount((i & 3) & k) + (negate ? 1 : 0)) & 1;
-      v |= (par ? 0xBC00u : 0x3C00u) << (16 * h);
-    }
-    w[p] = v;
-  }
-  return as_half4(w[0], w[1]);
-}
 
 __global__ void __launch_bounds__(256) k_intra(DPlane src, int sb_cols, int row0, uint32_t* __restrict__ out) {
   __shared__ __align__(16) uint8_t reg[2 * kRBytes];  // above copy, left copy
@@ -156,14 +119,13 @@ __global__ void __launch_bounds__(256) k_intra(DPlane src, int sb_cols, int row0
     const int tr = reg[blk - kRS + w - 1], bl = reg[kRBytes + blk + (h - 1) * kRS - 1];
     const int tl = reg[blk - kRS - 1];
     const int dc = static_cast<int>(dcv[lp]);
-    const f32x4 ccur = mfma(hh, as_half4(halves01(cur), halves23(cur)), zero);
+    const f32x4 ccur = mfma16(hh, row_halves(cur), zero);
     int a[4];
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) a[jj] = static_cast<int>((a4 >> (8 * jj)) & 255);
 
     auto score = [&](int m, uint32_t p01, uint32_t p23) {
-      const f32x4 d = mfma(nhh, as_half4(p01, p23), ccur);
-      float s = __builtin_fabsf(d[0]) + __builtin_fabsf(d[1]) + __builtin_fabsf(d[2]) + __builtin_fabsf(d[3]);
+      float s = abs_sum4(mfma16(nhh, as_half4(p01, p23), ccur));
       // Sum over the block's tiles (an aligned group of sh.seg columns of this row).
       if (sh.seg >= 2) s += as_f32(dpp<kDppXor1>(as_u32f(s)));
       if (sh.seg >= 4) s += as_f32(dpp<kDppXor2>(as_u32f(s)));

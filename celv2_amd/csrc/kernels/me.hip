@@ -4,16 +4,17 @@
 // src/gpu/inter_common.h:306-358):
 //   1. the six candidate centres' SADs, 8 lanes per centre (one block row each), summed by DPP;
 //      the centre ranking is then the reference's sequential rule, wave-uniform (scalar code);
-//   2. each searched centre's 17 x 17 positions: lane = (dx, third of the dy range), 51 lanes; a lane
-//      loads its 13 window rows with one aligned buffer_load_dwordx3 each (the row's offset in an
-//      SGPR) and scores its 5-6 positions with v_sad_u8 against the block held in SGPRs, the
-//      distance penalty folded into the first v_sad_u8's accumulator; the first least cost in
-//      dy-major order is the minimum of the key cost << 9 | position index (DPP + 4 readlanes).
-//      With two centres, the second window's loads are issued before the first is scored;
-//   3. level 0: the half-pel then quarter-pel refinement, 18 lanes = 9 positions x 2 tile rows,
-//      each lane two 4x4 tiles in packed 16-bit arithmetic, keyed (cost << 4 | order) so the
-//      centre wins ties as in me_refine.
+//   2. each searched centre's 24 x 24 window is read from the plane once (168 coalesced dwords) into
+//      the wave's LDS area; lane = (dx, third of the dy range), 51 lanes, takes its 13 rows from LDS
+//      and scores its 5-6 positions with v_sad_u8 against the block held in SGPRs, the distance
+//      penalty folded into the first v_sad_u8's accumulator; the first least cost in dy-major order
+//      is the minimum of the key cost << 9 | position index (DPP + 4 readlanes);
+//   3. level 0: the half-pel then quarter-pel refinement: per step the 9 positions x 4 tiles are 36
+//      columns of three 16x16x16 Hadamard products on the matrix cores, accumulated onto the block's
+//      own transform (hadamard.h); the key (cost << 4 | order) makes the centre win ties as in
+//      me_refine.
 #include "common.h"
+#include "hadamard.h"
 #include "launch.h"
 
 namespace celv2hip {
@@ -21,46 +22,54 @@ namespace celv2hip {
 namespace {
 
 constexpr int kWin = 2 * kRange + 8;  // 24: a search window's side
+constexpr int kWinDwords = 7;         // an aligned row of a window: 28 bytes cover 24 from any offset
+constexpr int kWinStride = 8;         // dwords per window row in LDS
+// 25 rows: the last dy group's sixth position (past the range, its key forced to the maximum) reads
+// row 24, which stays inside the array.
+constexpr int kWinLds = (kWin + 1) * kWinStride;
 
-// One lane's share of a search window: rows dy0 .. dy0 + 12 of the window at (X, Y), 8 bytes from
-// column dxi, as 13 x 2 words.
-struct Window {
-  uint32_t w[13][2];
-};
-
-__device__ __forceinline__ void load_window(const BPlane& rb, int X, int Y, int dxi, int dy0, Window& win) {
-  const int off = corner_offset(rb, Y + dy0, X + dxi);
-  const int base = off & ~3, sh = off & 3;
+// Reads a window's rows (aligned dwords from the row's first byte rounded down) into LDS; the
+// lane's share is dword k = lane + 64 i of the 24 x 7.
+__device__ __forceinline__ void stage_window(const BPlane& rb, int X, int Y, const int (&koff)[3], const int (&kdst)[3],
+                                             uint32_t* win) {
+  const int base = corner_offset(rb, Y, X) & ~3;
 #pragma unroll
-  for (int rr = 0; rr < 13; rr++) {
-    uint32_t d0, d1, d2;
-    bload12(rb, base, rr * rb.stride, d0, d1, d2);
-    win.w[rr][0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
-    win.w[rr][1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
-  }
+  for (int i = 0; i < 3; i++)
+    if (kdst[i] >= 0) win[kdst[i]] = bload4(rb, base + koff[i], 0);
 }
 
-// The lane's best key over its positions: pen[k] / idx[k] hold each position's penalty and index
-// (kill5: the lane has only 5 positions).
-__device__ __forceinline__ uint32_t score_window(const Window& win, const uint32_t c[16], const uint32_t pen[6],
-                                                 const uint32_t idx[6], uint32_t kill5) {
+// The lane's best key over its positions from a staged window (byte shift sh of the window's first
+// sample): pen[k] / idx[k] hold each position's penalty and index (kill5: the lane has 5 positions).
+__device__ __forceinline__ uint32_t score_window(const uint32_t* win, int sh, int dxi, int dy0, const uint32_t c[16],
+                                                 const uint32_t pen[6], const uint32_t idx[6], uint32_t kill5) {
+  const int o = sh + dxi, q = o >> 2, s = o & 3;
+  uint32_t w[13][2];
+#pragma unroll
+  for (int rr = 0; rr < 13; rr++) {
+    const uint32_t* row = win + (dy0 + rr) * kWinStride + q;
+    const uint32_t d0 = row[0], d1 = row[1], d2 = row[2];
+    w[rr][0] = __builtin_amdgcn_alignbyte(d1, d0, s);
+    w[rr][1] = __builtin_amdgcn_alignbyte(d2, d1, s);
+  }
   uint32_t best = 0xffffffffu;
 #pragma unroll
   for (int k = 0; k < 6; k++) {
-    uint32_t s = pen[k];
+    uint32_t acc = pen[k];
 #pragma unroll
-    for (int i = 0; i < 8; i++) s = sad4(c[2 * i + 1], win.w[k + i][1], sad4(c[2 * i], win.w[k + i][0], s));
-    uint32_t key = (s << 9) | idx[k];
+    for (int i = 0; i < 8; i++) acc = sad4(c[2 * i + 1], w[k + i][1], sad4(c[2 * i], w[k + i][0], acc));
+    uint32_t key = (acc << 9) | idx[k];
     if (k == 5) key |= kill5;
     best = min(best, key);
   }
   return best;
 }
 
-__constant__ int8_t kRefineDx[9] = {0, -1, 0, 1, -1, 1, -1, 0, 1};
-__constant__ int8_t kRefineDy[9] = {0, -1, -1, -1, 0, 0, 1, 1, 1};
+// me_refine's 9 positions (src/gpu/inter_common.h:306-319), 2 bits each as offset + 1; 12..15 pad.
+constexpr uint32_t kRefDx = (1u << 0) | (0u << 2) | (1u << 4) | (2u << 6) | (0u << 8) | (2u << 10) | (0u << 12) | (1u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
+constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u << 8) | (1u << 10) | (2u << 12) | (2u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 
 __global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
+  __shared__ uint32_t wins[4][2][kWinLds];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addressing
   const int per_ref = a.rows * a.cols, nitems = a.nrefs * per_ref, ngroups = (nitems + 3) >> 2;
@@ -125,25 +134,39 @@ __global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
     }
   }
 
-  // Full searches (me_search, src/gpu/inter_common.h:273-284).
+  // Full searches (me_search, src/gpu/inter_common.h:273-284) from windows staged in LDS.
+  int koff[3], kdst[3];
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const int k = lane + 64 * i, row = k / kWinDwords, d = k - row * kWinDwords;
+    koff[i] = row * rb.stride + 4 * d;
+    kdst[i] = k < kWin * kWinDwords ? row * kWinStride + d : -1;
+  }
+  uint32_t* win1 = wins[wave][0];
+  uint32_t* win2 = wins[wave][1];
+  const int X1 = clamp_origin(8 * bx + p1x - kRange, ref.w, kWin), Y1 = clamp_origin(8 * by + p1y - kRange, ref.h, kWin);
+  const int X2 = clamp_origin(8 * bx + p2x - kRange, ref.w, kWin), Y2 = clamp_origin(8 * by + p2y - kRange, ref.h, kWin);
+  stage_window(rb, X1, Y1, koff, kdst, win1);
+  if (two) stage_window(rb, X2, Y2, koff, kdst, win2);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
   const bool act = lane < 51;
   const int li = act ? lane : 0;
-  const int dxi = li % 17, g = li / 17, dy0 = 6 * g;
+  const int dxi = li % 17, g3 = li / 17, dy0 = 6 * g3;
   uint32_t pen[6], idx[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     pen[k] = kPelPenalty * (abs(dxi - kRange) + abs(dy0 + k - kRange));
     idx[k] = static_cast<uint32_t>((dy0 + k) * 17 + dxi);
   }
-  const uint32_t kill5 = g == 2 ? 0xffffffffu : 0u;
-  Window w1, w2;
-  load_window(rb, clamp_origin(8 * bx + p1x - kRange, ref.w, kWin), clamp_origin(8 * by + p1y - kRange, ref.h, kWin), dxi, dy0, w1);
-  if (two) load_window(rb, clamp_origin(8 * bx + p2x - kRange, ref.w, kWin), clamp_origin(8 * by + p2y - kRange, ref.h, kWin), dxi, dy0, w2);
-  const uint32_t k1 = wave_min_u32(act ? score_window(w1, c, pen, idx, kill5) : 0xffffffffu);
+  const uint32_t kill5 = g3 == 2 ? 0xffffffffu : 0u;
+  const uint32_t k1 = wave_min_u32(act ? score_window(win1, corner_offset(rb, Y1, X1) & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
   const int i1 = static_cast<int>(k1 & 511);
   int mvx = p1x + i1 % 17 - kRange, mvy = p1y + i1 / 17 - kRange;
   if (two) {
-    const uint32_t k2 = wave_min_u32(act ? score_window(w2, c, pen, idx, kill5) : 0xffffffffu);
+    const uint32_t k2 = wave_min_u32(act ? score_window(win2, corner_offset(rb, Y2, X2) & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
     if ((k2 >> 9) < (k1 >> 9)) {
       const int i2 = static_cast<int>(k2 & 511);
       mvx = p2x + i2 % 17 - kRange, mvy = p2y + i2 / 17 - kRange;
@@ -151,36 +174,46 @@ __global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
   }
 
   if (a.level0) {
-    // me_refine (src/gpu/inter_common.h:306-319) from the integer MV, in quarter pels: lanes 2o, 2o+1
-    // score position o's top and bottom tile rows (two 4x4 tiles each, packed), 18 lanes in all.
-    int qcx = 4 * mvx, qcy = 4 * mvy;
-    const bool on = lane < 18;
-    const int o = on ? (lane >> 1) : 8, ty = lane & 1;
-    uint32_t ca[4], cbw[4];
+    // me_refine (src/gpu/inter_common.h:306-319) from the integer MV, in quarter pels. Column
+    // j' = 16 c + j of product c is position o = j' >> 2, tile t = j' & 3 (t = 2 ty + tx); lane
+    // (j, g) supplies row g of that tile. Positions 9..11 are padding (never chosen).
+    const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;
+    const half4 hh = hadamard_operand(lane, false), nhh = hadamard_operand(lane, true);
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    const uint32_t crow = bload4(cb, corner_offset(cb, 8 * by + 4 * ty + g, 8 * bx + 4 * tx), 0);
+    const f32x4 ccur = mfma16(hh, row_halves(crow), zero);
+    int odx[3], ody[3];
 #pragma unroll
-    for (int i = 0; i < 4; i++) bload8(cb, corner_offset(cb, 8 * by + 4 * ty + i, 8 * bx), 0, ca[i], cbw[i]);
-    const TilePair cp = tile_pair(ca, cbw);
+    for (int cc = 0; cc < 3; cc++) {
+      const int o = 4 * cc + (j >> 2);
+      odx[cc] = static_cast<int>((kRefDx >> (2 * o)) & 3u) - 1, ody[cc] = static_cast<int>((kRefDy >> (2 * o)) & 3u) - 1;
+    }
+    int qcx = 4 * mvx, qcy = 4 * mvy;
 #pragma unroll
     for (int step = 2; step >= 1; step--) {
-      const int qx = qcx + kRefineDx[o] * step, qy = qcy + kRefineDy[o] * step;
-      const int X = clamp_origin(8 * bx + (qx >> 2), ref.w, 8);
-      const int Y = clamp_origin(8 * by + 4 * ty + (qy >> 2), ref.h, 4);
-      const int off = static_cast<int>(a.phase_bytes) * ((qy & 3) * 4 + (qx & 3)) + corner_offset(rb, Y, X);
-      const int base = off & ~3, sh = off & 3;
-      uint32_t pa[4], pb[4];
+      uint32_t best = 0xffffffffu;
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        uint32_t d0, d1, d2;
-        bload12(rb, base, i * rb.stride, d0, d1, d2);
-        pa[i] = __builtin_amdgcn_alignbyte(d1, d0, sh), pb[i] = __builtin_amdgcn_alignbyte(d2, d1, sh);
+      for (int cc = 0; cc < 3; cc++) {
+        const int qx = qcx + odx[cc] * step, qy = qcy + ody[cc] * step;
+        const int X = clamp_origin(8 * bx + (qx >> 2), ref.w, 8) + 4 * tx;
+        const int Y = clamp_origin(8 * by + 4 * ty + (qy >> 2), ref.h, 4) + g;
+        const int off = static_cast<int>(a.phase_bytes) * ((qy & 3) * 4 + (qx & 3)) + corner_offset(rb, Y, X);
+        uint32_t d0, d1;
+        bload8(rb, off & ~3, 0, d0, d1);
+        const uint32_t prow = __builtin_amdgcn_alignbyte(d1, d0, off & 3);
+        float s = abs_sum4(mfma16(nhh, row_halves(prow), ccur));
+        s += as_f32(dpp<kDppXor1>(as_u32f(s)));  // the position's 4 tiles (lanes 4o' .. 4o'+3)
+        s += as_f32(dpp<kDppXor2>(as_u32f(s)));
+        s += __shfl_xor(s, 16, 64);                // its 4 rows of coefficients
+        s += __shfl_xor(s, 32, 64);
+        const int o = 4 * cc + (j >> 2);
+        const uint32_t cost = static_cast<uint32_t>(s) + (o ? kQpelPenalty : 0);
+        if (o < 9) best = min(best, (cost << 4) | static_cast<uint32_t>(o));
       }
-      const uint32_t two_t = satd4x4_pair(cp, pa, pb);
-      uint32_t s = (two_t & 0xffffu) + (two_t >> 16);
-      s += dpp<kDppXor1>(s);  // the other tile row of the same position
-      const uint32_t cost = s + (o ? kQpelPenalty : 0);
-      const uint32_t key = wave_min_u32(on ? ((cost << 4) | static_cast<uint32_t>(o)) : 0xffffffffu);
+      const uint32_t key = wave_min_u32(best);
       const int ob = static_cast<int>(key & 15);
-      qcx += kRefineDx[ob] * step, qcy += kRefineDy[ob] * step;
+      qcx += (static_cast<int>((kRefDx >> (2 * ob)) & 3u) - 1) * step;
+      qcy += (static_cast<int>((kRefDy >> (2 * ob)) & 3u) - 1) * step;
     }
     mvx = qcx, mvy = qcy;
   }
