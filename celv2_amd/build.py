@@ -30,6 +30,9 @@ HIP_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    # MFMA results straight into VGPRs (no v_accvgpr_read after each product).
+    "-mllvm",
+    "-amdgpu-mfma-vgpr-form",
     "-Wall",
     "-Wno-unused-function",
 ]

@@ -1,8 +1,10 @@
 // The C ABI of libcelv2hip.so (include/celv2hip.h): device memory, reference-key caches, the
-// per-launch queue of kernels on one HIP stream, and row completion events.
+// per-launch queue of kernels on the engine's main HIP stream, prepare() on a side stream, and row
+// completion events.
 //
 // Queue order of a launch (VulkanEngine::launch, src/vk/vk_engine.cpp:207-312, re-planned for HIP):
-//   source: the prepare()d pyramid and intra scores under src_key, else import + pyramid + intra now;
+//   source: the prepare()d pyramid and intra scores under src_key (built on the side stream, which
+//   the main stream waits for), else import + pyramid + intra now;
 //   references: a slot holding the key, else a free slot: import + 15 phase planes + pyramid;
 //   motion search: levels 4..0, each one launch over every reference (kernel boundaries order them);
 //   rows in row-stream order (or all rows at once): inter scores, finalize, copy of the rows' records
@@ -114,13 +116,18 @@ struct SrcSlot {
   Pyramid p;
   Upload up;
   uint32_t* intra = nullptr;
+  // prepare() fills a slot on the side stream while launches run on the main stream: each stream
+  // waits for the other's last work on the slot before touching it.
+  hipEvent_t side_done = nullptr, main_done = nullptr;
+  bool side_pending = false, main_pending = false;
 };
 
 }  // namespace
 
 struct celv2hip_engine {
   int width, height, sb_size, device, pw, ph, sb_cols, sb_rows;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // launches
+  hipStream_t side = nullptr;    // prepare(): the next source's planes and intra scores
   std::string name;
   int row_streams = 1;
   bool resident = false;
@@ -151,6 +158,7 @@ struct celv2hip_engine {
   ~celv2hip_engine() {
     if (device >= 0) (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    if (side) (void)hipStreamSynchronize(side);
     auto free_up = [](Upload& u) {
       if (u.host) (void)hipHostFree(u.host);
       if (u.dev) (void)hipFree(u.dev);
@@ -159,6 +167,8 @@ struct celv2hip_engine {
     for (SrcSlot& s : srcs) {
       if (s.p.mem) (void)hipFree(s.p.mem);
       if (s.intra) (void)hipFree(s.intra);
+      if (s.side_done) (void)hipEventDestroy(s.side_done);
+      if (s.main_done) (void)hipEventDestroy(s.main_done);
       free_up(s.up);
     }
     for (auto& s : slots) {
@@ -179,6 +189,12 @@ struct celv2hip_engine {
     for (auto& a : pring)
       for (hipEvent_t e : a) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+    if (side) (void)hipStreamDestroy(side);
+  }
+
+  void sync_all() {
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    check(hipStreamSynchronize(side), "hipStreamSynchronize side");
   }
 
   size_t nsb() const { return static_cast<size_t>(sb_cols) * sb_rows; }
@@ -192,7 +208,7 @@ struct celv2hip_engine {
 
   // Puts a frame's luma where the import kernel reads it: in place if on the device, else through
   // the slot's pinned staging buffer (copied now, so the caller may reuse its memory at once).
-  const void* stage(const celv2_luma& f, Upload& up) {
+  const void* stage(const celv2_luma& f, Upload& up, hipStream_t st) {
     const bool wide = f.y16 != nullptr;
     if (!wide && !f.y8) throw Failure(CELV2_ERR_ARG, "frame has no luma");
     if (wide && (f.bit_depth < 9 || f.bit_depth > 16)) throw Failure(CELV2_ERR_ARG, "bit depth must be 9..16 with 16-bit samples");
@@ -210,28 +226,38 @@ struct celv2hip_engine {
       check(hipEventSynchronize(up.done), "hipEventSynchronize staging");
     }
     std::memcpy(up.host, wide ? static_cast<const void*>(f.y16) : static_cast<const void*>(f.y8), n);
-    check(hipMemcpyAsync(up.dev, up.host, n, hipMemcpyHostToDevice, stream), "hipMemcpyAsync upload");
-    check(hipEventRecord(up.done, stream), "hipEventRecord");
+    check(hipMemcpyAsync(up.dev, up.host, n, hipMemcpyHostToDevice, st), "hipMemcpyAsync upload");
+    check(hipEventRecord(up.done, st), "hipEventRecord");
     return up.dev;
   }
 
-  void build(const celv2_luma& f, Pyramid& p, Upload& up, bool phases) {
+  void build(const celv2_luma& f, Pyramid& p, Upload& up, bool phases, hipStream_t st) {
     ensure_pyramid(p, phases);
-    const void* src = stage(f, up);
-    if (f.y16) launch_import16(static_cast<const uint16_t*>(src), pw, f.bit_depth, p.lv[0], stream);
-    else launch_import8(static_cast<const uint8_t*>(src), pw, p.lv[0], stream);
-    if (phases) launch_subpel(p.lv[0], p.phase_bytes, stream);
-    for (int l = 1; l < kLevels; l++) launch_downsample(p.lv[l - 1], p.lv[l], stream);
+    const void* src = stage(f, up, st);
+    if (f.y16) launch_import16(static_cast<const uint16_t*>(src), pw, f.bit_depth, p.lv[0], st);
+    else launch_import8(static_cast<const uint8_t*>(src), pw, p.lv[0], st);
+    if (phases) launch_subpel(p.lv[0], p.phase_bytes, st);
+    for (int l = 1; l < kLevels; l++) launch_downsample(p.lv[l - 1], p.lv[l], st);
     check(hipGetLastError(), "plane kernels");
   }
 
-  void intra_scores(SrcSlot& s) {
+  void intra_scores(SrcSlot& s, hipStream_t st) {
     if (!s.intra) check(hipMalloc(&s.intra, nsb() * kBlocks * kModes * sizeof(uint32_t)), "hipMalloc intra");
     std::array<hipEvent_t, 2>& pe = pring[prepares++ % kRing];
-    check(hipEventRecord(pe[0], stream), "hipEventRecord");
-    launch_intra(s.p.lv[0], sb_cols, 0, sb_rows, s.intra, stream);
+    check(hipEventRecord(pe[0], st), "hipEventRecord");
+    launch_intra(s.p.lv[0], sb_cols, 0, sb_rows, s.intra, st);
     check(hipGetLastError(), "intra kernel");
-    check(hipEventRecord(pe[1], stream), "hipEventRecord");
+    check(hipEventRecord(pe[1], st), "hipEventRecord");
+  }
+
+  // Orders stream `st`'s coming work on slot s after the other stream's last work on it.
+  void claim(SrcSlot& s, bool on_side) {
+    if (on_side && s.main_pending) check(hipStreamWaitEvent(side, s.main_done, 0), "hipStreamWaitEvent");
+    if (!on_side && s.side_pending) check(hipStreamWaitEvent(stream, s.side_done, 0), "hipStreamWaitEvent");
+  }
+  void release(SrcSlot& s, bool on_side) {
+    if (on_side) check(hipEventRecord(s.side_done, side), "hipEventRecord"), s.side_pending = true;
+    else check(hipEventRecord(s.main_done, stream), "hipEventRecord"), s.main_pending = true;
   }
 };
 
@@ -282,6 +308,11 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     e->name = prop.name[0] ? prop.name : prop.gcnArchName;
     if (e->name.empty()) e->name = "HIP device " + std::to_string(device);
     check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking), "hipStreamCreate");
+    for (SrcSlot& s : e->srcs) {
+      check(hipEventCreateWithFlags(&s.side_done, hipEventDisableTiming), "hipEventCreate");
+      check(hipEventCreateWithFlags(&s.main_done, hipEventDisableTiming), "hipEventCreate");
+    }
     int w = e->pw, h = e->ph;
     for (int l = 0; l < kLevels; l++) {
       e->fcols[l] = (w + 7) / 8, e->frows[l] = (h + 7) / 8;
@@ -337,11 +368,14 @@ int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key) 
   if (int rc = validate(e)) return rc;
   return guarded([&] {
     if (key < 0 || !next) return;
-    // The slot the last launch does not use: its earlier work is behind us on the stream.
+    // The slot the last launch does not use, filled on the side stream once the main stream's
+    // earlier launches are done with it; the launch that takes it waits for the side stream.
     SrcSlot& s = e->srcs[1 - e->cur_set];
     s.key = -1;
-    e->build(*next, s.p, s.up, false);
-    e->intra_scores(s);
+    e->claim(s, true);
+    e->build(*next, s.p, s.up, false, e->side);
+    e->intra_scores(s, e->side);
+    e->release(s, true);
     s.key = key;
   });
 }
@@ -358,12 +392,13 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
     int use = -1;
     for (int i = 0; i < 2; i++)
       if (src_key >= 0 && e->srcs[i].key == src_key) use = i;
+    e->claim(e->srcs[use < 0 ? 1 - e->cur_set : use], false);
     if (use < 0) {
       use = 1 - e->cur_set;
       SrcSlot& s = e->srcs[use];
       s.key = -1;
-      e->build(*src, s.p, s.up, false);
-      e->intra_scores(s);
+      e->build(*src, s.p, s.up, false, e->stream);
+      e->intra_scores(s, e->stream);
     }
     e->srcs[use].key = -1;  // taken: a later prepare() may overwrite it
     e->cur_set = use;
@@ -385,7 +420,7 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
         }
         if (!slot) e->slots.push_back(std::make_unique<RefSlot>()), slot = e->slots.back().get();
         slot->key = -1;
-        e->build(refs[ri], slot->p, slot->up, true);
+        e->build(refs[ri], slot->p, slot->up, true, e->stream);
         slot->key = key;
       }
       slot->used = now;
@@ -458,6 +493,7 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
       check(hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
     }
     check(hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
+    e->release(cs, false);
     e->nrefs_last = nrefs;
     e->last_refs = used;
     e->launched = true;
@@ -486,7 +522,7 @@ int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out) {
   if (int rc = validate(e)) return rc;
   return guarded([&] {
     if (row < 0 || row >= e->sb_rows) throw Failure(CELV2_ERR_ARG, "HipEngine: no such row");
-    check(hipStreamSynchronize(e->stream), "sync");
+    e->sync_all();
     check(hipMemcpy(out, static_cast<char*>(e->rec_dev) + static_cast<size_t>(row) * e->sb_cols * sizeof(celv2_sb),
                     e->sb_cols * sizeof(celv2_sb), hipMemcpyDeviceToHost), "hipMemcpy row");
   });
@@ -504,7 +540,7 @@ int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint
     if (phase && !p->phase_bytes) throw Failure(CELV2_ERR_ARG, "no phases");
     const DPlane& d = p->lv[level];
     *w = d.w, *h = d.h;
-    check(hipStreamSynchronize(e->stream), "sync");
+    e->sync_all();
     check(hipMemcpy2D(out, d.w, d.p + static_cast<ptrdiff_t>(phase) * p->phase_bytes, d.stride, d.w, d.h, hipMemcpyDeviceToHost), "hipMemcpy2D");
   });
 }
@@ -515,7 +551,7 @@ int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, i
     if (!e->launched || ref < 0 || ref >= e->nrefs_last || level < 0 || level >= kLevels) throw Failure(CELV2_ERR_ARG, "no such field");
     const int n = e->fcols[level] * e->frows[level];
     std::vector<uint32_t> packed(n);
-    check(hipStreamSynchronize(e->stream), "sync");
+    e->sync_all();
     check(hipMemcpy(packed.data(), e->fields[ref][level], n * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy field");
     for (int i = 0; i < n; i++) mvx[i] = static_cast<int16_t>(packed[i] & 0xffffu), mvy[i] = static_cast<int16_t>(packed[i] >> 16);
     *cols = e->fcols[level], *rows = e->frows[level];
@@ -530,7 +566,7 @@ long long celv2hip_fetch_intra(celv2hip_engine* e, uint32_t* out, long long cap)
     if (!e->launched || !s.intra) throw Failure(CELV2_ERR_STATE, "no launch");
     n = static_cast<long long>(e->nsb()) * kBlocks * kModes;
     if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "intra buffer too small");
-    check(hipStreamSynchronize(e->stream), "sync");
+    e->sync_all();
     check(hipMemcpy(out, s.intra, static_cast<size_t>(n) * sizeof(uint32_t), hipMemcpyDeviceToHost), "hipMemcpy intra");
   });
   return rc ? rc : n;
@@ -538,13 +574,13 @@ long long celv2hip_fetch_intra(celv2hip_engine* e, uint32_t* out, long long cap)
 
 int celv2hip_sync(celv2hip_engine* e) {
   if (int rc = validate(e)) return rc;
-  return guarded([&] { check(hipStreamSynchronize(e->stream), "hipStreamSynchronize"); });
+  return guarded([&] { e->sync_all(); });
 }
 
 int celv2hip_stats_reset(celv2hip_engine* e) {
   if (int rc = validate(e)) return rc;
   return guarded([&] {
-    check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    e->sync_all();
     check(hipMemset(e->counters, 0, 80 * 16 * sizeof(unsigned long long)), "hipMemset counters");
     e->window_launch = e->launches, e->window_prepare = e->prepares;
   });
@@ -554,7 +590,7 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
   if (int rc = validate(e)) return -rc;
   long long n = 0;
   const int rc = guarded([&] {
-    check(hipStreamSynchronize(e->stream), "hipStreamSynchronize");
+    e->sync_all();
     n = e->launches - e->window_launch;
     const long long np = e->prepares - e->window_prepare;
     if (n > celv2hip_engine::kRing || np > celv2hip_engine::kRing) throw Failure(CELV2_ERR_STATE, "stats window longer than the event ring");

@@ -3,13 +3,17 @@
 // partition decision (decide_square, src/gpu/kernels_ref.cpp:178-251) and the exported superblock
 // record (celv2_sb, include/celv2hip.h).
 //
-// One workgroup per superblock. Compound SATDs are tiled: lanes take 4x4 tiles from a static list of
-// all 425 blocks' tiles and add into per-block LDS sums (integer atomics, order-free); the partition
-// DP runs level by level with one thread per square, and one thread walks the chosen tree.
+// One workgroup per superblock. Compound SATDs run on the matrix cores (hadamard.h): the 3072 tiles of
+// all 425 blocks, sorted by block size so that every 16-tile product holds whole blocks or a part of
+// one block in an aligned group of columns; lane (j, g) loads row g of tile j from the source and from
+// both predictions and averages the predictions bytewise. Per-block sums are LDS float atomics of
+// exact integers (order-free). The partition DP runs level by level with one thread per square, and
+// one thread walks the chosen tree.
 #include <stdexcept>
 
 #include "../../../include/celv2hip.h"
 #include "common.h"
+#include "hadamard.h"
 #include "launch.h"
 
 namespace celv2hip {
@@ -18,39 +22,31 @@ namespace {
 
 constexpr int kAllTiles = 3072;  // 4x4 tiles of all 425 blocks: per level 256 (whole) + 4 x 128 (halves)
 constexpr int64_t kMax = INT64_MAX;
+constexpr int kThreads = 512;  // 8 waves: two superblocks per CU still fit in one round
 
 __device__ uint32_t g_all_tiles[kAllTiles];  // b | tx << 16 | ty << 24
+__device__ uint8_t g_tile_seg[kAllTiles / 16];  // tiles per block (at most 16) of each 16-tile step
 
 __device__ __forceinline__ bool pair_allowed(int a, int b, int n) {  // compound_pair_ok, inter_common.h:364-367
   const int lo = a < b ? a : b, lim = n - 2 < 2 ? n - 2 : 2;
   return a != b && lo <= lim;
 }
 
-// The compound prediction's 4x4 SATD: predictions averaged (p0 + p1 + 1) >> 1
-// (block_compound_satd, src/gpu/inter_common.h:370-385).
-__device__ __forceinline__ int satd4x4_avg(const uint32_t c[4], const uint32_t p[4], const uint32_t q[4]) {
-  int d[16];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int a = static_cast<int>((p[i] >> (8 * j)) & 255), b = static_cast<int>((q[i] >> (8 * j)) & 255);
-      d[4 * i + j] = static_cast<int>((c[i] >> (8 * j)) & 255) - ((a + b + 1) >> 1);
-    }
-  return wht4_abs_sum(d);
+// (p + q + 1) >> 1 of each byte pair (the compound average, block_compound_satd,
+// src/gpu/inter_common.h:370-385): (p | q) - ((p ^ q) >> 1).
+__device__ __forceinline__ uint32_t avg_bytes(uint32_t p, uint32_t q) { return (p | q) - (((p ^ q) >> 1) & 0x7f7f7f7fu); }
+
+// Row g of the 4x4 tile at (x, y) of a bordered plane, the tile's origin clamped as a 4x4 window.
+__device__ __forceinline__ uint32_t tile_row(const uint8_t* plane, int stride, int w, int h, int x, int y, int g) {
+  const int X = clamp_origin(x, w, 4), Y = clamp_origin(y, h, 4);
+  return load4u(plane + static_cast<ptrdiff_t>(Y + g) * stride + X);
 }
 
-__device__ __forceinline__ void tile_rows(const DPlane& pl, const uint8_t* plane, int x, int y, uint32_t out[4]) {
-  const int X = clamp_origin(x, pl.w, 4), Y = clamp_origin(y, pl.h, 4);
-#pragma unroll
-  for (int i = 0; i < 4; i++) out[i] = load4u(plane + static_cast<ptrdiff_t>(Y + i) * pl.stride + X);
-}
-
-__global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
+__global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a) {
   __shared__ int64_t rc[kMaxRefs][kBlocks];    // per-reference inter cost
   __shared__ uint32_t rmv[kMaxRefs][kBlocks];  // per-reference MV (quarter pel, packed)
   __shared__ int8_t ra[kBlocks], rs[kBlocks];  // best and second reference (-1: none)
-  __shared__ uint32_t comp[kBlocks];           // compound SATD (blocks with an allowed pair)
+  __shared__ float comp[kBlocks];              // compound SATD (blocks with an allowed pair)
   __shared__ int64_t bcost[kBlocks];           // the block's choice's cost
   __shared__ uint8_t bmode[kBlocks];           // its intra mode, kModes = inter
   __shared__ uint8_t bref[kBlocks], bref2[kBlocks];
@@ -60,6 +56,7 @@ __global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
   __shared__ uint8_t tree[CELV2_SB_CAP];
   __shared__ uint16_t leaf[CELV2_SB_CAP];
   __shared__ int counts[2];
+  __shared__ const uint8_t* refp[kMaxRefs];  // reference planes (level 0, phase 0), indexable per lane
 
   const int tid = threadIdx.x;
   const int sbx = blockIdx.x, sby = a.row0 + blockIdx.y;
@@ -67,9 +64,10 @@ __global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
   const int ox = 64 * sbx, oy = 64 * sby, n = a.nrefs;
   const int64_t lam = a.lam;
   celv2_sb* const out = static_cast<celv2_sb*>(a.out) + sb;
+  if (tid < kMaxRefs) refp[tid] = tid < n ? a.ref[tid].p : nullptr;
 
   // Per block: each reference's best (cost and MV), the best and the runner-up reference.
-  for (int b = tid; b < kBlocks; b += 256) {
+  for (int b = tid; b < kBlocks; b += kThreads) {
     int best = -1, second = -1;
     for (int r = 0; r < n; r++) {
       const uint2 e = a.best[(sb * kMaxRefs + r) * kBlocks + b];
@@ -80,35 +78,61 @@ __global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
     for (int r = 0; r < n; r++)
       if (r != best && (second < 0 || rc[r][b] < rc[second][b])) second = r;
     ra[b] = static_cast<int8_t>(best), rs[b] = static_cast<int8_t>(second);
-    comp[b] = 0;
+    comp[b] = 0.f;
   }
   __syncthreads();
 
-  // Compound SATDs of the blocks whose two best references form an allowed pair.
+  // Compound SATDs of the blocks whose two best references form an allowed pair (the products of
+  // other blocks' tiles are computed and dropped).
   if (n >= 2) {
-    for (int slot = tid; slot < kAllTiles; slot += 256) {
-      const uint32_t e = g_all_tiles[slot];
-      const int b = static_cast<int>(e & 0xffff), tx = static_cast<int>((e >> 16) & 255), ty = static_cast<int>(e >> 24);
-      const int p = ra[b], q = rs[b];
-      if (q < 0 || !pair_allowed(p, q, n)) continue;
-      int x, y, w, h;
-      block_geom(b, x, y, w, h);
-      const int px = ox + x + 4 * tx, py = oy + y + 4 * ty;
-      uint32_t c[4], pp[4], qq[4];
+    const int lane = tid & 63, j = lane & 15, g = lane >> 4;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const half4 hh = hadamard_operand(lane, false);
+    const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
+    const DPlane& r0 = a.ref[0];
+    // Four steps at a time: every load of the four is in flight before the first product.
+    constexpr int kWaves = kThreads / 64, kBatch = 4;
+    static_assert((kAllTiles / 16) % (kWaves * kBatch) == 0, "compound steps per batch");
+    for (int step0 = wave; step0 < kAllTiles / 16; step0 += kWaves * kBatch) {
+      uint32_t cur[kBatch], avg[kBatch];
+      int blk[kBatch];
+      bool on[kBatch];
 #pragma unroll
-      for (int i = 0; i < 4; i++) c[i] = *reinterpret_cast<const uint32_t*>(a.src.p + static_cast<ptrdiff_t>(py + i) * a.src.stride + px);
-      const uint32_t mp = rmv[p][b], mq = rmv[q][b];
-      const DPlane& rp = a.ref[p];
-      const DPlane& rq = a.ref[q];
-      tile_rows(rp, rp.p + a.phase_bytes * ((mv_y(mp) & 3) * 4 + (mv_x(mp) & 3)), px + (mv_x(mp) >> 2), py + (mv_y(mp) >> 2), pp);
-      tile_rows(rq, rq.p + a.phase_bytes * ((mv_y(mq) & 3) * 4 + (mv_x(mq) & 3)), px + (mv_x(mq) >> 2), py + (mv_y(mq) >> 2), qq);
-      atomicAdd(&comp[b], static_cast<uint32_t>(satd4x4_avg(c, pp, qq)));
+      for (int u = 0; u < kBatch; u++) {
+        const uint32_t e = g_all_tiles[16 * (step0 + kWaves * u) + j];
+        const int b = static_cast<int>(e & 0xffff), tx = static_cast<int>((e >> 16) & 255), ty = static_cast<int>(e >> 24);
+        int x, y, w, h;
+        block_geom(b, x, y, w, h);
+        const int px = ox + x + 4 * tx, py = oy + y + 4 * ty;
+        const int p = ra[b], q = rs[b];
+        blk[u] = b, on[u] = q >= 0 && pair_allowed(p, q, n);
+        cur[u] = *reinterpret_cast<const uint32_t*>(a.src.p + static_cast<ptrdiff_t>(py + g) * a.src.stride + px);
+        avg[u] = cur[u];
+        if (on[u]) {
+          const uint32_t mp = rmv[p][b], mq = rmv[q][b];
+          const uint8_t* pp = refp[p] + a.phase_bytes * ((mv_y(mp) & 3) * 4 + (mv_x(mp) & 3));
+          const uint8_t* qp = refp[q] + a.phase_bytes * ((mv_y(mq) & 3) * 4 + (mv_x(mq) & 3));
+          avg[u] = avg_bytes(tile_row(pp, r0.stride, r0.w, r0.h, px + (mv_x(mp) >> 2), py + (mv_y(mp) >> 2), g),
+                             tile_row(qp, r0.stride, r0.w, r0.h, px + (mv_x(mq) >> 2), py + (mv_y(mq) >> 2), g));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; u++) {
+        const int seg = g_tile_seg[step0 + kWaves * u];
+        float s = abs_sum4(mfma16(hh, row_halves(cur[u]) - row_halves(avg[u]), zero));
+        // Sum over the block's tiles in this product (an aligned group of seg columns of this row).
+        if (seg >= 2) s += as_f32(dpp<kDppXor1>(as_u32f(s)));
+        if (seg >= 4) s += as_f32(dpp<kDppXor2>(as_u32f(s)));
+        if (seg >= 8) s += as_f32(dpp<kDppHalfMirror>(as_u32f(s)));
+        if (seg >= 16) s += as_f32(dpp<kDppMirror>(as_u32f(s)));
+        if (on[u] && (j & (seg - 1)) == 0) atomicAdd(&comp[blk[u]], s);
+      }
     }
   }
   __syncthreads();
 
   // Each block's choices, its candidate record, and its cost for the DP.
-  for (int b = tid; b < kBlocks; b += 256) {
+  for (int b = tid; b < kBlocks; b += kThreads) {
     int x, y, w, h;
     block_geom(b, x, y, w, h);
     const int p = ra[b], q = rs[b];
@@ -117,7 +141,7 @@ __global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
     int r1 = 0, r2 = CELV2_SINGLE_REF;
     if (p >= 0) ci = rc[p][b], m1 = rmv[p][b], r1 = p;
     if (q >= 0 && pair_allowed(p, q, n)) {
-      const int64_t cc = static_cast<int64_t>(comp[b]) * 256 + lam * (20 + p + q);
+      const int64_t cc = static_cast<int64_t>(comp[b]) * 256 + lam * (20 + p + q);  // comp: an exact integer
       if (cc < ci) ci = cc, r2 = q, m2 = rmv[q][b];
     }
     int64_t c = kMax;
@@ -203,8 +227,8 @@ __global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
   __syncthreads();
   const int nt = counts[0], nl = counts[1];
   if (tid == 0) out->ntree = nt, out->nblocks = nl, out->ncands = kBlocks, out->pad_ = 0;
-  for (int i = tid; i < CELV2_SB_CAP; i += 256) out->tree[i] = i < nt ? tree[i] : 0;
-  for (int i = tid; i < CELV2_SB_CAP; i += 256) {
+  for (int i = tid; i < CELV2_SB_CAP; i += kThreads) out->tree[i] = i < nt ? tree[i] : 0;
+  for (int i = tid; i < CELV2_SB_CAP; i += kThreads) {
     celv2_block d = {};
     if (i < nl) {
       const int b = leaf[i];
@@ -234,22 +258,32 @@ __global__ void __launch_bounds__(256) k_finalize(FinalizeArgs a) {
 
 void launch_finalize(const FinalizeArgs& a, hipStream_t s) {
   if (a.nrows <= 0) return;
-  k_finalize<<<dim3(a.sb_cols, a.nrows), 256, 0, s>>>(a);
+  k_finalize<<<dim3(a.sb_cols, a.nrows), kThreads, 0, s>>>(a);
 }
 
+// All blocks' tiles, blocks in decreasing tile count (256 .. 2, every class a multiple of 16 tiles),
+// each block's tiles contiguous; seg[step] = min(16, tiles per block) of the step's blocks.
 void upload_final_tiles(hipStream_t s) {
   static uint32_t table[kAllTiles];
+  static uint8_t seg[kAllTiles / 16];
   int n = 0;
-  for (int b = 0; b < kBlocks; b++) {
-    const int sq = b / 5, kind = b % 5;
-    const int level = sq >= 21 ? 3 : sq >= 5 ? 2 : sq >= 1 ? 1 : 0;
-    const int size = 64 >> level;
-    const int w = (kind == 3 || kind == 4) ? size / 2 : size, h = (kind == 1 || kind == 2) ? size / 2 : size;
-    for (int ty = 0; ty < h / 4; ty++)
-      for (int tx = 0; tx < w / 4; tx++) table[n++] = static_cast<uint32_t>(b) | (static_cast<uint32_t>(tx) << 16) | (static_cast<uint32_t>(ty) << 24);
-  }
+  for (int count = 256; count >= 2; count /= 2)
+    for (int b = 0; b < kBlocks; b++) {
+      const int sq = b / 5, kind = b % 5;
+      const int level = sq >= 21 ? 3 : sq >= 5 ? 2 : sq >= 1 ? 1 : 0;
+      const int size = 64 >> level;
+      const int w = (kind == 3 || kind == 4) ? size / 2 : size, h = (kind == 1 || kind == 2) ? size / 2 : size;
+      if ((w / 4) * (h / 4) != count) continue;
+      if (n % 16 != 0 && count >= 16) throw std::logic_error("compound tile alignment");
+      for (int ty = 0; ty < h / 4; ty++)
+        for (int tx = 0; tx < w / 4; tx++) {
+          if (n % 16 == 0) seg[n / 16] = static_cast<uint8_t>(count < 16 ? count : 16);
+          table[n++] = static_cast<uint32_t>(b) | (static_cast<uint32_t>(tx) << 16) | (static_cast<uint32_t>(ty) << 24);
+        }
+    }
   if (n != kAllTiles) throw std::logic_error("compound tile table size");
   (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_all_tiles), table, sizeof(table), 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_tile_seg), seg, sizeof(seg), 0, hipMemcpyHostToDevice, s);
 }
 
 }  // namespace celv2hip

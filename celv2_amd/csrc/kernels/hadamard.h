@@ -6,6 +6,8 @@
 // operand and the tiles as the 16 columns of B. A sample v in 0..255 becomes the exact half
 // 1024 + v by writing the byte 0x64 above it; the bias cancels in cur - pred, every partial sum is an
 // integer below 2^24, so the f32 coefficients are exact. |x| is a free source modifier on v_add_f32.
+// Where both operands are at hand, (1024 + c) - (1024 + p) = c - p is itself an exact half
+// (|c - p| <= 255), so one v_pk_add_f16 per pair gives the difference tile directly.
 //
 // Operand layout of v_mfma_f32_16x16x16_f16 for lane l = j + 16 g (j = l & 15, g = l >> 4):
 //   A: HH[j][4g .. 4g + 3]; B: column j's elements 4g .. 4g + 3 (for a tile: its row g);
@@ -58,6 +60,19 @@ __device__ __forceinline__ half4 hadamard_operand(int lane, bool negate) {
 // Sum of |coefficients| this lane holds (a quarter of one tile's SATD).
 __device__ __forceinline__ float abs_sum4(f32x4 d) {
   return __builtin_fabsf(d[0]) + __builtin_fabsf(d[1]) + __builtin_fabsf(d[2]) + __builtin_fabsf(d[3]);
+}
+
+// Cross-row sums without LDS (gfx950 v_permlane16_swap / v_permlane32_swap exchange rows of 16
+// lanes between two registers). sum_rows16(a, b): row 0 gets a0 + a1, row 1 b0 + b1, row 2 a2 + a3,
+// row 3 b2 + b3 (an = row n of a). sum_halves32(a, b): lanes 0..31 get a(l) + a(l + 32), lanes
+// 32..63 get b(l - 32) + b(l).
+__device__ __forceinline__ float sum_rows16(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane16_swap(as_u32f(a), as_u32f(b), false, false);
+  return as_f32(r[0]) + as_f32(r[1]);
+}
+__device__ __forceinline__ float sum_halves32(float a, float b) {
+  const auto r = __builtin_amdgcn_permlane32_swap(as_u32f(a), as_u32f(b), false, false);
+  return as_f32(r[0]) + as_f32(r[1]);
 }
 
 }  // namespace celv2hip

@@ -204,8 +204,8 @@ __global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
         float s = abs_sum4(mfma16(nhh, row_halves(prow), ccur));
         s += as_f32(dpp<kDppXor1>(as_u32f(s)));  // the position's 4 tiles (lanes 4o' .. 4o'+3)
         s += as_f32(dpp<kDppXor2>(as_u32f(s)));
-        s += __shfl_xor(s, 16, 64);                // its 4 rows of coefficients
-        s += __shfl_xor(s, 32, 64);
+        s = sum_rows16(s, s);                       // its 4 rows of coefficients
+        s = sum_halves32(s, s);
         const int o = 4 * cc + (j >> 2);
         const uint32_t cost = static_cast<uint32_t>(s) + (o ? kQpelPenalty : 0);
         if (o < 9) best = min(best, (cost << 4) | static_cast<uint32_t>(o));
