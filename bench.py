@@ -32,6 +32,13 @@ from celv2_amd.records import SB_DTYPE, compare_maps  # noqa: E402
 METRIC = "1080p10 frames/s on 1 GPU: plain HIP vs learned-pruned at ≤0.5% BD-rate loss"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, one int32 op per lane per clock
 SAD_OPS, SATD_OPS = 3, 7  # int ops per pixel (sub, abs, add; diff, 2 butterfly passes, abs, add)
+# The motion search's ceiling: v_sad_u8 takes 4 byte pairs per lane per instruction, so the chip's
+# published clock gives 256 CUs x 64 lanes x 4 x 2.4 GHz pixel-SADs per second; a microbenchmark of
+# independent v_sad_u8 chains on the MI355X reached 121.6 T/s (profiles/sad_u8_rate.md).
+SAD_PEAK_TPX, SAD_MEASURED_TPX = 256 * 64 * 4 * 2.4e9 / 1e12, 121.6
+# HBM bytes the five motion-search launches of one frame move (rocprofv3 FETCH_SIZE and WRITE_SIZE
+# per dispatch, scaled by a 64 MiB device copy: profiles/me_traffic_1080p10_7refs.md).
+ME_TRAFFIC_BYTES = 289.7e6
 
 
 def log(*a):
@@ -39,7 +46,8 @@ def log(*a):
 
 
 def cpu_baseline(y8, frame, refs, lam, gpu_map, threads):
-    """The oracle's threaded CPU engine on one frame (test infrastructure, timed here only)."""
+    """The oracle's threaded CPU engine on one frame (test infrastructure, timed here only); with
+    gpu_map None, only the time."""
     so = os.path.join(ROOT, "oracle", "liboracle.so")
     O = ctypes.CDLL(so)
     O.orc_engine_create.restype = ctypes.c_void_p
@@ -50,7 +58,7 @@ def cpu_baseline(y8, frame, refs, lam, gpu_map, threads):
     h, w = y8[0].shape
     e = O.orc_engine_create(w, h, 64, threads, -1)
     arr = (ctypes.c_void_p * len(refs))(*[y8[i].ctypes.data for i in refs])
-    out = np.zeros(gpu_map.shape[0], dtype=SB_DTYPE)
+    out = np.zeros(((w + 63) // 64) * ((h + 63) // 64), dtype=SB_DTYPE)
     t0 = time.perf_counter()
     O.orc_engine_launch(e, w, h, y8[frame].ctypes.data, arr, len(refs), lam)
     rc = O.orc_engine_collect(e, out.ctypes.data, None)
@@ -58,7 +66,7 @@ def cpu_baseline(y8, frame, refs, lam, gpu_map, threads):
     O.orc_engine_destroy(e)
     if rc != 0:
         raise RuntimeError("CPU engine failed")
-    return dt, compare_maps(gpu_map, out, (w + 63) // 64)
+    return dt, (compare_maps(gpu_map, out, (w + 63) // 64) if gpu_map is not None else None)
 
 
 def main():
@@ -126,6 +134,15 @@ def main():
     log(f"timed {args.steps} frames in {wall:.3f}s: {fps:.1f} frames/s; per frame ms {json.dumps({k: round(v, 3) for k, v in per.items()})}")
     log(f"counts over the window: {cnt}")
     gpu_map = E.fetch()
+    # Non-repeated inter candidates per frame, counted on 4 more frames outside the timed window.
+    E.set_counting(True)
+    E.stats_reset()
+    for s_ in range(args.warmup + args.steps, args.warmup + args.steps + 4):
+        step(s_)
+    E.sync()
+    _, cnt4 = E.stats()
+    E.set_counting(False)
+    inter_cands = cnt4["inter_candidates"] / max(1, cnt4["launches"])
 
     # Algorithmic integer work per frame (weights of the reference's cost rules, counted as executed).
     sbs = E.sb_cols * E.sb_rows
@@ -142,18 +159,21 @@ def main():
     me_ops = SAD_OPS * me_sad_px + SATD_OPS * me_satd_px
     inter_ops = SATD_OPS * cnt["tile_evals"] / n * 4096
     intra_ops = SATD_OPS * sbs * 479232
-    stage_ops = {"me": me_ops, "inter": inter_ops, "intra": intra_ops}
+    # The dominant kernel is the motion search (5 launches per frame); its bound is the v_sad_u8 rate.
     dom = max(("me", "inter", "intra", "finalize"), key=lambda k: per[k])
-    if dom in stage_ops:
-        achieved = stage_ops[dom] / (per[dom] / 1e3) / 1e12
-        roofline = {"bound": "valu", "kernel": {"me": "k_me_level (5 launches)", "inter": "k_inter", "intra": "k_intra"}[dom],
-                    "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "Tops/s",
-                    "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
-                    "ops_per_launch": stage_ops[dom], "ms_per_launch": round(per[dom], 4)}
-    else:
-        roofline = {"bound": "valu", "kernel": "k_finalize", "achieved": None, "peak": round(VALU_PEAK_TOPS, 2),
-                    "unit": "Tops/s", "frac": None, "traffic": None, "ms_per_launch": round(per[dom], 4)}
+    me_s = per["me"] / 1e3
+    achieved = me_sad_px / me_s / 1e12
+    roofline = {"bound": "valu", "kernel": "k_me_level (5 launches per frame)", "dominant_stage": dom,
+                "achieved": round(achieved, 3), "peak": round(SAD_PEAK_TPX, 2), "unit": "T pixel-SAD/s (v_sad_u8)",
+                "frac": round(achieved / SAD_PEAK_TPX, 4),
+                "traffic": ME_TRAFFIC_BYTES, "traffic_gbps": round(ME_TRAFFIC_BYTES / me_s / 1e9, 1),
+                "measured_ceiling": SAD_MEASURED_TPX, "frac_of_measured_ceiling": round(achieved / SAD_MEASURED_TPX, 4),
+                "work_per_frame": me_sad_px, "ms_per_frame": round(per["me"], 4),
+                "int_ops_model_tops": round(me_ops / me_s / 1e12, 2)}
     total_ops = me_ops + inter_ops + intra_ops
+    # Scored candidates (intra mode evaluations + non-repeated inter (block, MV, reference) candidates
+    # + compound evaluations) per frame.
+    cands = sbs * 420 * 13 + inter_cands + cnt["compound_blocks"] / n
 
     # End to end through the C++ HipEngine (host frames, upload, DecisionMap); never the metric.
     e2e = None
@@ -182,9 +202,16 @@ def main():
         parity = diff is None
         if diff:
             log("PARITY FAILURE vs the CPU engine:", diff)
+        # One thread (as engine_bench runs it) on a bounded sample: the frame's top two superblock
+        # rows, scaled to the frame's rows.
+        crop = [np.ascontiguousarray(f[:128]) for f in y8]
+        dt1, _ = cpu_baseline(crop, last, refs, lam, None, 1)
+        one = dt1 * E.sb_rows / 2
         cpu = {"value": round(1.0 / dt, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"1 frame (frame {last}) of the same 1080p 10-bit 7-reference workload, oracle CpuEngine with {threads} threads: {dt:.1f} s"}
-        log(f"cpu engine: {dt:.2f}s for one frame ({threads} threads); parity {'ok' if parity else 'FAILED'}")
+               "sample": f"1 frame (frame {last}) of the same 1080p 10-bit 7-reference workload, oracle CpuEngine with {threads} threads: {dt:.1f} s",
+               "one_thread": {"value": round(1.0 / one, 5), "unit": "frames/s", "cores": 1,
+                              "sample": f"rows 0-127 of frame {last} (2 of {E.sb_rows} superblock rows), 1 thread: {dt1:.1f} s, scaled x{E.sb_rows / 2:g}"}}
+        log(f"cpu engine: {dt:.2f}s for one frame ({threads} threads), 1 thread {dt1:.1f}s for 2 SB rows; parity {'ok' if parity else 'FAILED'}")
     for b in dev:
         b.free()
 
@@ -209,6 +236,9 @@ def main():
         "parity_vs_cpu_engine": parity,
         "stage_ms_per_frame": {k: round(v, 4) for k, v in per.items()},
         "counts_per_frame": {k: round(v / max(1, n), 1) for k, v in cnt.items() if k != "launches"},
+        "scored_candidates_per_frame": round(cands),
+        "scored_candidates_per_s": round(cands * fps),
+        "me_positions_per_s": round(cnt["me_centres"] / n * 289 * fps),
         "int_ops_per_frame": total_ops,
         "all_kernels_tops": round(total_ops * fps / 1e12, 3),
         "engine_path_fps_host_buffers": e2e,

@@ -2,6 +2,7 @@
 
   celv2_amd/lib/libcelv2hip.so    HIP kernels for gfx950 + the C++ engine + C ABI (hipcc)
   celv2_amd/lib/libcelv2synth.so  synthetic clip generator (g++)
+  celv2_amd/lib/sad_rate          v_sad_u8 issue-rate microbenchmark (the motion search's ceiling)
 
 `python -m celv2_amd.build` or `build()` compiles every time it is called.
 """
@@ -75,6 +76,13 @@ def build_synth():
     os.replace(tmp, os.path.join(LIB, "libcelv2synth.so"))
 
 
+def build_probes():
+    os.makedirs(LIB, exist_ok=True)
+    tmp = os.path.join(LIB, "sad_rate.tmp")
+    _run([HIPCC, *[f for f in HIP_FLAGS if f != "-fPIC"], "-o", tmp, os.path.join(CSRC, "probes", "sad_rate.hip")])
+    os.replace(tmp, os.path.join(LIB, "sad_rate"))
+
+
 def build_oracle():
     """The test oracle (oracle/liboracle.so) and, where the reference tree is mounted, oracle/_ref."""
     _run(["make", "-C", os.path.join(ROOT, "oracle"), "-s", "-B", "liboracle.so"])
@@ -86,6 +94,7 @@ def build_all():
     build_synth()
     build_oracle()
     build_hip()
+    build_probes()
 
 
 if __name__ == "__main__":

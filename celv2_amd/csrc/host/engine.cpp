@@ -131,6 +131,7 @@ struct celv2hip_engine {
   std::string name;
   int row_streams = 1;
   bool resident = false;
+  bool counting = false;  // count non-repeated inter candidates (costs the inter kernel LDS atomics)
   SrcSlot srcs[2];
   int cur_set = 0;
   std::vector<std::unique_ptr<RefSlot>> slots;
@@ -149,7 +150,7 @@ struct celv2hip_engine {
   std::vector<std::array<hipEvent_t, 2>> pring;
   hipEvent_t* ev = nullptr;  // the current launch's 6 events
   long long prepares = 0, window_launch = 0, window_prepare = 0;
-  unsigned long long* counters = nullptr;  // device: [0] unique MVs scored, [1] compound blocks, [16..]: ME searches
+  unsigned long long* counters = nullptr;  // device: [0] unique MVs scored, [1] compound blocks, [2] inter candidates, [16..]: ME searches
   unsigned long long* counters_host = nullptr;
   int nrefs_last = 0;
   bool launched = false;
@@ -364,6 +365,12 @@ int celv2hip_set_resident(celv2hip_engine* e, int on) {
   return CELV2_OK;
 }
 
+int celv2hip_set_counting(celv2hip_engine* e, int on) {
+  if (!e) return CELV2_ERR_ARG;
+  e->counting = on != 0;
+  return CELV2_OK;
+}
+
 int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key) {
   if (int rc = validate(e)) return rc;
   return guarded([&] {
@@ -457,7 +464,7 @@ int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key
     ia.nrefs = nrefs, ia.cols = e->fcols[0], ia.rows = e->frows[0], ia.sb_cols = e->sb_cols;
     ia.phase_bytes = nrefs ? used[0]->p.phase_bytes : 0;
     ia.best = e->best;
-    ia.counter = e->counters;
+    ia.counter = e->counters, ia.candidates = e->counting ? e->counters + 2 : nullptr;
     FinalizeArgs fa{};
     fa.src = cs.p.lv[0];
     fa.nrefs = nrefs, fa.sb_cols = e->sb_cols, fa.phase_bytes = ia.phase_bytes, fa.lam = lambda_q8;
@@ -614,9 +621,9 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
     check(hipMemcpy(e->counters_host, e->counters, 80 * 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "hipMemcpy counters");
     unsigned long long searches = 0;
     for (int i = 0; i < 64; i++) searches += e->counters_host[16 + 16 * i];
-    const long long v[4] = {n, static_cast<long long>(e->counters_host[0]), static_cast<long long>(searches),
-                            static_cast<long long>(e->counters_host[1])};
-    for (int k = 0; k < 4 && k < cap_counts; k++) counts[k] = v[k];
+    const long long v[5] = {n, static_cast<long long>(e->counters_host[0]), static_cast<long long>(searches),
+                            static_cast<long long>(e->counters_host[1]), static_cast<long long>(e->counters_host[2])};
+    for (int k = 0; k < 5 && k < cap_counts; k++) counts[k] = v[k];
   });
   return rc ? -rc : static_cast<int>(n);
 }

@@ -36,6 +36,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
   __shared__ int cuid[kRegion * kRegion];
   __shared__ int urep[kRegion * kRegion];
   __shared__ uint32_t best[kBlocks];
+  __shared__ unsigned long long seen[kBlocks][2];  // MV classes among each block's candidates (counting only)
   __shared__ uint32_t sc[4][kBlocks];
   __shared__ int nuniq;
 
@@ -52,7 +53,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
     const int cx = clampi(8 * sbx - 1 + tid % kRegion, 0, a.cols - 1), cy = clampi(8 * sby - 1 + tid / kRegion, 0, a.rows - 1);
     mv[tid] = field[cy * a.cols + cx];
   }
-  for (int b = tid; b < kBlocks; b += 256) best[b] = 0xffffffffu;
+  for (int b = tid; b < kBlocks; b += 256) best[b] = 0xffffffffu, seen[b][0] = seen[b][1] = 0;
   __syncthreads();
   // Distinct MVs, numbered in cell order (wave 0); each cell's class is its first cell with that MV.
   const int c0 = lane, c1 = lane + 64;
@@ -163,6 +164,7 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
             const uint32_t e = g_cell_tasks[i];
             const int b = static_cast<int>(e >> 7);
             atomicMin(&best[b], (mysc[b] << 7) | (e & 127u));
+            if (a.candidates) atomicOr(&seen[b][u >> 6], 1ull << (u & 63));
           }
         }
       }
@@ -176,6 +178,14 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
     out[b] = make_uint2(key >> 7, mv[cell]);
   }
   if (tid == 0 && a.counter) atomicAdd(a.counter, static_cast<unsigned long long>(U));
+  if (a.candidates) {
+    // Non-repeated candidates: per block, the distinct MVs among its candidates (a repeat is skipped
+    // by the reference, repeated_candidate, src/gpu/inter_common.h:65-89), marked in the task loop.
+    unsigned long long n = 0;
+    for (int b = tid; b < kBlocks; b += 256) n += static_cast<unsigned long long>(__popcll(seen[b][0]) + __popcll(seen[b][1]));
+    for (int m = 32; m >= 1; m >>= 1) n += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n), m, 64));
+    if (lane == 0) atomicAdd(a.candidates, n);
+  }
 }
 
 }  // namespace
@@ -223,16 +233,16 @@ static int cand_total(int b) {
 void upload_inter_tables(hipStream_t s) {
   static uint16_t first[kRegion * kRegion + 1], tasks[kTasks];
   static uint8_t cand[kBlocks * kMaxCands];
-  int count[kRegion * kRegion] = {};
+  int ccount[kRegion * kRegion] = {};
   int total = 0;
   for (int b = 0; b < kBlocks; b++)
     for (int k = 0; k < cand_total(b); k++) {
       const int c = region_cell(b, k);
       cand[b * kMaxCands + k] = static_cast<uint8_t>(c);
-      count[c]++, total++;
+      ccount[c]++, total++;
     }
   int off = 0;
-  for (int c = 0; c < kRegion * kRegion; c++) first[c] = static_cast<uint16_t>(off), off += count[c];
+  for (int c = 0; c < kRegion * kRegion; c++) first[c] = static_cast<uint16_t>(off), off += ccount[c];
   first[kRegion * kRegion] = static_cast<uint16_t>(off);
   int fill[kRegion * kRegion] = {};
   for (int b = 0; b < kBlocks; b++)

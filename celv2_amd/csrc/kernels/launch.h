@@ -40,7 +40,8 @@ struct InterArgs {
   int nrefs, cols, rows, sb_cols, row0, nrows;
   long long phase_bytes;
   uint2* best;
-  unsigned long long* counter;  // unique MVs scored (may be null)
+  unsigned long long* counter;     // unique MVs scored (may be null)
+  unsigned long long* candidates;  // non-repeated (block, MV) candidates scored (may be null)
 };
 void launch_inter(const InterArgs& a, hipStream_t s);
 

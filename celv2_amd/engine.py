@@ -250,18 +250,21 @@ class RawEngine:
             self._check(int(got), "fetch_intra")
         return out.reshape(self.sb_cols * self.sb_rows, 425, 13)
 
+    def set_counting(self, on):
+        self._check(lib().celv2hip_set_counting(self.h, 1 if on else 0), "set_counting")
+
     def stats_reset(self):
         self._check(lib().celv2hip_stats_reset(self.h), "stats_reset")
 
     def stats(self):
         """(stage ms sums over the window, counts) - see celv2hip_stats."""
         ms = (ctypes.c_double * 7)()
-        cnt = (LL * 4)()
-        n = lib().celv2hip_stats(self.h, ms, 7, cnt, 4)
+        cnt = (LL * 5)()
+        n = lib().celv2hip_stats(self.h, ms, 7, cnt, 5)
         if n < 0:
             raise EngineError(last_error())
         keys = ["build", "me", "inter", "finalize", "copy", "total", "intra_prepare"]
-        ckeys = ["launches", "tile_evals", "me_centres", "compound_blocks"]
+        ckeys = ["launches", "tile_evals", "me_centres", "compound_blocks", "inter_candidates"]
         return dict(zip(keys, list(ms))), dict(zip(ckeys, list(cnt)))
 
 

@@ -131,7 +131,8 @@ int celv2hip_sync(celv2hip_engine* e);
  * scores), [1] motion search (5 levels), [2] inter scores, [3] compound + finalize, [4] record copy,
  * [5] launch start to end, [6] intra scores queued by prepare(). counts: [0] launches, [1] distinct
  * (MV, reference, superblock) tile evaluations, [2] motion-search centres searched, [3] compound
- * blocks scored. Returns the number of launches (negative status on failure). */
+ * blocks scored, [4] non-repeated (block, MV, reference) inter candidates scored (while counting is on,
+ * celv2hip_set_counting). Returns the number of launches (negative status on failure). */
 int celv2hip_stats_reset(celv2hip_engine* e);
 int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts, int cap_counts);
 
@@ -140,6 +141,10 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
 /* Keeps results on the device: wait_row is not available and no device-to-host copy is queued.
  * on != 0: device-resident mode; 0: normal. */
 int celv2hip_set_resident(celv2hip_engine* e, int on);
+
+/* on != 0: later launches count the non-repeated inter candidates they score (stats counts[4]);
+ * off by default, as the count costs the inter kernel extra LDS atomics. */
+int celv2hip_set_counting(celv2hip_engine* e, int on);
 
 /* Copies the records of superblock row `row` of the last launch in resident mode (tests). */
 int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out);
