@@ -128,6 +128,7 @@ struct celv2hip_engine {
   int width, height, sb_size, device, pw, ph, sb_cols, sb_rows;
   hipStream_t stream = nullptr;  // launches
   hipStream_t side = nullptr;    // prepare(): the next source's planes and intra scores
+  hipEvent_t order_ev = nullptr;  // orders this engine's stream around batched launches
   std::string name;
   int row_streams = 1;
   bool resident = false;
@@ -191,6 +192,7 @@ struct celv2hip_engine {
       for (hipEvent_t e : a) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     if (side) (void)hipStreamDestroy(side);
+    if (order_ev) (void)hipEventDestroy(order_ev);
   }
 
   void sync_all() {
@@ -232,33 +234,33 @@ struct celv2hip_engine {
     return up.dev;
   }
 
-  void build(const celv2_luma& f, Pyramid& p, Upload& up, bool phases, hipStream_t st) {
+  // A plane build for the batched plane launches: import (rounding >8-bit luma), the 15 phase planes
+  // of a reference, the 4 coarser levels.
+  struct BuildJob {
+    ImportItem imp;
+    Pyramid* p;
+    bool phases;
+  };
+  BuildJob build_job(const celv2_luma& f, Pyramid& p, Upload& up, bool phases, hipStream_t st) {
     ensure_pyramid(p, phases);
     const void* src = stage(f, up, st);
-    if (f.y16) launch_import16(static_cast<const uint16_t*>(src), pw, f.bit_depth, p.lv[0], st);
-    else launch_import8(static_cast<const uint8_t*>(src), pw, p.lv[0], st);
-    if (phases) launch_subpel(p.lv[0], p.phase_bytes, st);
-    for (int l = 1; l < kLevels; l++) launch_downsample(p.lv[l - 1], p.lv[l], st);
-    check(hipGetLastError(), "plane kernels");
+    return BuildJob{ImportItem{src, pw, f.y16 ? f.bit_depth : 8, f.y16 ? 1 : 0, p.lv[0]}, &p, phases};
   }
 
-  void intra_scores(SrcSlot& s, hipStream_t st) {
+  IntraItem intra_item(SrcSlot& s) {
     if (!s.intra) check(hipMalloc(&s.intra, nsb() * kBlocks * kModes * sizeof(uint32_t)), "hipMalloc intra");
-    std::array<hipEvent_t, 2>& pe = pring[prepares++ % kRing];
-    check(hipEventRecord(pe[0], st), "hipEventRecord");
-    launch_intra(s.p.lv[0], sb_cols, 0, sb_rows, s.intra, st);
-    check(hipGetLastError(), "intra kernel");
-    check(hipEventRecord(pe[1], st), "hipEventRecord");
+    return IntraItem{s.p.lv[0], sb_cols, 0, sb_rows, s.intra};
   }
 
-  // Orders stream `st`'s coming work on slot s after the other stream's last work on it.
-  void claim(SrcSlot& s, bool on_side) {
-    if (on_side && s.main_pending) check(hipStreamWaitEvent(side, s.main_done, 0), "hipStreamWaitEvent");
-    if (!on_side && s.side_pending) check(hipStreamWaitEvent(stream, s.side_done, 0), "hipStreamWaitEvent");
+  // Orders stream st's coming work on slot s after the other stream's last work on it (the side
+  // stream is prepare()'s, any other stream is a launch's).
+  void claim(SrcSlot& s, hipStream_t st) {
+    if (st == side && s.main_pending) check(hipStreamWaitEvent(side, s.main_done, 0), "hipStreamWaitEvent");
+    if (st != side && s.side_pending) check(hipStreamWaitEvent(st, s.side_done, 0), "hipStreamWaitEvent");
   }
-  void release(SrcSlot& s, bool on_side) {
-    if (on_side) check(hipEventRecord(s.side_done, side), "hipEventRecord"), s.side_pending = true;
-    else check(hipEventRecord(s.main_done, stream), "hipEventRecord"), s.main_pending = true;
+  void release(SrcSlot& s, hipStream_t st) {
+    if (st == side) check(hipEventRecord(s.side_done, side), "hipEventRecord"), s.side_pending = true;
+    else check(hipEventRecord(s.main_done, st), "hipEventRecord"), s.main_pending = true;
   }
 };
 
@@ -310,6 +312,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     if (e->name.empty()) e->name = "HIP device " + std::to_string(device);
     check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming), "hipEventCreate");
     for (SrcSlot& s : e->srcs) {
       check(hipEventCreateWithFlags(&s.side_done, hipEventDisableTiming), "hipEventCreate");
       check(hipEventCreateWithFlags(&s.main_done, hipEventDisableTiming), "hipEventCreate");
@@ -371,6 +374,50 @@ int celv2hip_set_counting(celv2hip_engine* e, int on) {
   return CELV2_OK;
 }
 
+namespace {
+
+// Plane builds of any engines in batched launches: every import, then every reference's phase
+// planes, then each coarser level of every pyramid.
+void run_builds(const std::vector<celv2hip_engine::BuildJob>& jobs, hipStream_t st) {
+  for (size_t j0 = 0; j0 < jobs.size(); j0 += kMaxItems) {
+    const size_t j1 = std::min(jobs.size(), j0 + kMaxItems);
+    Batch<ImportItem> bi;
+    Batch<SubpelItem> bs;
+    for (size_t j = j0; j < j1; j++) {
+      add_import(bi, jobs[j].imp);
+      if (jobs[j].phases) add_subpel(bs, SubpelItem{jobs[j].p->lv[0], jobs[j].p->phase_bytes});
+    }
+    launch_import(bi, st);
+    launch_subpel(bs, st);
+    for (int l = 1; l < kLevels; l++) {
+      Batch<DownItem> bd;
+      for (size_t j = j0; j < j1; j++) add_downsample(bd, DownItem{jobs[j].p->lv[l - 1], jobs[j].p->lv[l]});
+      launch_downsample(bd, st);
+    }
+  }
+  check(hipGetLastError(), "plane kernels");
+}
+
+// Intra scores of the given engines' source slots, with each engine's prepare-ring events around.
+void run_intra(const std::vector<std::pair<celv2hip_engine*, SrcSlot*>>& items, hipStream_t st) {
+  for (size_t j0 = 0; j0 < items.size(); j0 += kMaxItems) {
+    const size_t j1 = std::min(items.size(), j0 + kMaxItems);
+    Batch<IntraItem> b;
+    std::vector<std::array<hipEvent_t, 2>*> rings;
+    for (size_t j = j0; j < j1; j++) {
+      celv2hip_engine* e = items[j].first;
+      add_intra(b, e->intra_item(*items[j].second));
+      rings.push_back(&e->pring[e->prepares++ % celv2hip_engine::kRing]);
+    }
+    for (auto* r : rings) check(hipEventRecord((*r)[0], st), "hipEventRecord");
+    launch_intra(b, st);
+    check(hipGetLastError(), "intra kernel");
+    for (auto* r : rings) check(hipEventRecord((*r)[1], st), "hipEventRecord");
+  }
+}
+
+}  // namespace
+
 int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key) {
   if (int rc = validate(e)) return rc;
   return guarded([&] {
@@ -379,131 +426,250 @@ int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key) 
     // earlier launches are done with it; the launch that takes it waits for the side stream.
     SrcSlot& s = e->srcs[1 - e->cur_set];
     s.key = -1;
-    e->claim(s, true);
-    e->build(*next, s.p, s.up, false, e->side);
-    e->intra_scores(s, e->side);
-    e->release(s, true);
+    e->claim(s, e->side);
+    run_builds({e->build_job(*next, s.p, s.up, false, e->side)}, e->side);
+    run_intra({{e, &s}}, e->side);
+    e->release(s, e->side);
     s.key = key;
   });
 }
 
-int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key, const celv2_luma* refs,
-                    const long long* ref_keys, int nrefs, int64_t lambda_q8) {
-  if (int rc = validate(e)) return rc;
-  return guarded([&] {
-    if (!src) throw Failure(CELV2_ERR_ARG, "HipEngine: no source");
-    if (nrefs < 0 || nrefs > kMaxRefs) throw Failure(CELV2_ERR_SIZE, "HipEngine: too many references");
-    if (nrefs > 0 && !refs) throw Failure(CELV2_ERR_ARG, "HipEngine: no references");
+namespace {
+
+struct LaunchIn {
+  celv2hip_engine* e;
+  const celv2_luma* src;
+  long long src_key;
+  const celv2_luma* refs;
+  const long long* ref_keys;
+  int nrefs;
+  int64_t lam;
+};
+
+// One frame of each engine: every stage is one launch (or one per kMaxItems engines) on the first
+// engine's stream; each engine's own stream orders its earlier work before and its later work after.
+void launch_batch(const LaunchIn* in, int n) {
+  celv2hip_engine* const e0 = in[0].e;
+  const hipStream_t S = e0->stream;
+  for (int i = 0; i < n; i++) {
+    const LaunchIn& x = in[i];
+    if (!x.src) throw Failure(CELV2_ERR_ARG, "HipEngine: no source");
+    if (x.nrefs < 0 || x.nrefs > kMaxRefs) throw Failure(CELV2_ERR_SIZE, "HipEngine: too many references");
+    if (x.nrefs > 0 && !x.refs) throw Failure(CELV2_ERR_ARG, "HipEngine: no references");
+    if (x.e->device != e0->device) throw Failure(CELV2_ERR_ARG, "batched engines must share a device");
+    for (int k = 0; k < i; k++)
+      if (in[k].e == x.e) throw Failure(CELV2_ERR_ARG, "an engine appears twice in one batch");
+  }
+  for (int i = 1; i < n; i++) {
+    check(hipEventRecord(in[i].e->order_ev, in[i].e->stream), "hipEventRecord");
+    check(hipStreamWaitEvent(S, in[i].e->order_ev, 0), "hipStreamWaitEvent");
+  }
+  std::vector<celv2hip_engine::BuildJob> jobs;
+  std::vector<std::pair<celv2hip_engine*, SrcSlot*>> intra_now;
+  std::vector<SrcSlot*> cur(n);
+  std::vector<std::vector<RefSlot*>> used(n);
+  for (int i = 0; i < n; i++) {
+    const LaunchIn& x = in[i];
+    celv2hip_engine* e = x.e;
     e->ev = e->ring[e->launches % celv2hip_engine::kRing].data();
-    check(hipEventRecord(e->ev[0], e->stream), "hipEventRecord");
+    check(hipEventRecord(e->ev[0], S), "hipEventRecord");
     int use = -1;
-    for (int i = 0; i < 2; i++)
-      if (src_key >= 0 && e->srcs[i].key == src_key) use = i;
-    e->claim(e->srcs[use < 0 ? 1 - e->cur_set : use], false);
+    for (int k = 0; k < 2; k++)
+      if (x.src_key >= 0 && e->srcs[k].key == x.src_key) use = k;
+    e->claim(e->srcs[use < 0 ? 1 - e->cur_set : use], S);
     if (use < 0) {
       use = 1 - e->cur_set;
       SrcSlot& s = e->srcs[use];
       s.key = -1;
-      e->build(*src, s.p, s.up, false, e->stream);
-      e->intra_scores(s, e->stream);
+      jobs.push_back(e->build_job(*x.src, s.p, s.up, false, S));
+      intra_now.push_back({e, &s});
     }
     e->srcs[use].key = -1;  // taken: a later prepare() may overwrite it
     e->cur_set = use;
-    SrcSlot& cs = e->srcs[use];
+    cur[i] = &e->srcs[use];
     const long long now = ++e->launches;
-    std::vector<RefSlot*> used(nrefs, nullptr);
-    for (int ri = 0; ri < nrefs; ri++) {
-      const long long key = ref_keys ? ref_keys[ri] : -1;
+    used[i].assign(x.nrefs, nullptr);
+    for (int ri = 0; ri < x.nrefs; ri++) {
+      const long long key = x.ref_keys ? x.ref_keys[ri] : -1;
       RefSlot* slot = nullptr;
-      for (auto& s : e->slots)
-        if (key >= 0 && s->key == key) slot = s.get();
+      for (auto& sl : e->slots)
+        if (key >= 0 && sl->key == key) slot = sl.get();
       if (!slot) {
-        for (auto& s : e->slots) {
-          const bool wanted = ref_keys && std::find(ref_keys, ref_keys + nrefs, s->key) != ref_keys + nrefs && s->key >= 0;
-          if (s->used != now && !wanted) {
-            slot = s.get();
+        for (auto& sl : e->slots) {
+          const bool wanted = x.ref_keys && std::find(x.ref_keys, x.ref_keys + x.nrefs, sl->key) != x.ref_keys + x.nrefs && sl->key >= 0;
+          if (sl->used != now && !wanted) {
+            slot = sl.get();
             break;
           }
         }
         if (!slot) e->slots.push_back(std::make_unique<RefSlot>()), slot = e->slots.back().get();
         slot->key = -1;
-        e->build(refs[ri], slot->p, slot->up, true, e->stream);
+        jobs.push_back(e->build_job(x.refs[ri], slot->p, slot->up, true, S));
         slot->key = key;
       }
       slot->used = now;
-      used[ri] = slot;
+      used[i][ri] = slot;
     }
-    check(hipEventRecord(e->ev[1], e->stream), "hipEventRecord");
-    // Motion search, top level first, every reference per launch.
-    for (int l = kLevels - 1; l >= 0 && nrefs > 0; l--) {
+  }
+  run_builds(jobs, S);
+  run_intra(intra_now, S);
+  for (int i = 0; i < n; i++) check(hipEventRecord(in[i].e->ev[1], S), "hipEventRecord");
+
+  // Motion search, top level first, every reference of every engine per launch.
+  for (int l = kLevels - 1; l >= 0; l--) {
+    Batch<MeLevelArgs> b;
+    for (int i = 0; i < n; i++) {
+      celv2hip_engine* e = in[i].e;
+      const int nr = in[i].nrefs;
+      if (!nr) continue;
       MeLevelArgs a{};
-      a.cur = cs.p.lv[l];
-      a.nrefs = nrefs, a.cols = e->fcols[l], a.rows = e->frows[l];
+      a.cur = cur[i]->p.lv[l];
+      a.nrefs = nr, a.cols = e->fcols[l], a.rows = e->frows[l];
       a.pcols = l + 1 < kLevels ? e->fcols[l + 1] : 0, a.prows = l + 1 < kLevels ? e->frows[l + 1] : 0;
-      a.level0 = l == 0, a.phase_bytes = used[0]->p.phase_bytes;
+      a.level0 = l == 0, a.phase_bytes = used[i][0]->p.phase_bytes;
       a.searches = e->counters + 16;
-      for (int r = 0; r < nrefs; r++) {
+      for (int r = 0; r < nr; r++) {
         if (!e->fields[r][l]) check(hipMalloc(&e->fields[r][l], static_cast<size_t>(e->fcols[l]) * e->frows[l] * sizeof(uint32_t)), "hipMalloc field");
-        a.ref[r] = used[r]->p.lv[l];
+        a.ref[r] = used[i][r]->p.lv[l];
         a.coarser[r] = l + 1 < kLevels ? e->fields[r][l + 1] : nullptr;
         a.out[r] = e->fields[r][l];
       }
-      launch_me_level(a, e->stream);
-      check(hipGetLastError(), "me kernel");
+      add_me_level(b, a);
+      if (b.n == kMaxItems) launch_me_level(b, S), b = Batch<MeLevelArgs>();
     }
-    check(hipEventRecord(e->ev[2], e->stream), "hipEventRecord");
-    // Rows: row streams as VulkanEngine orders them (src/vk/vk_engine.cpp:288-294).
+    launch_me_level(b, S);
+    check(hipGetLastError(), "me kernel");
+  }
+  for (int i = 0; i < n; i++) check(hipEventRecord(in[i].e->ev[2], S), "hipEventRecord");
+
+  auto inter_args = [&](int i) {
+    celv2hip_engine* e = in[i].e;
+    InterArgs ia{};
+    ia.src = cur[i]->p.lv[0];
+    ia.nrefs = in[i].nrefs, ia.cols = e->fcols[0], ia.rows = e->frows[0], ia.sb_cols = e->sb_cols;
+    ia.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
+    ia.best = e->best;
+    ia.counter = e->counters, ia.candidates = e->counting ? e->counters + 2 : nullptr;
+    for (int r = 0; r < in[i].nrefs; r++) ia.ref[r] = used[i][r]->p.lv[0], ia.field[r] = e->fields[r][0];
+    return ia;
+  };
+  auto finalize_args = [&](int i) {
+    celv2hip_engine* e = in[i].e;
+    FinalizeArgs fa{};
+    fa.src = cur[i]->p.lv[0];
+    fa.nrefs = in[i].nrefs, fa.sb_cols = e->sb_cols, fa.lam = in[i].lam;
+    fa.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
+    fa.best = e->best, fa.intra = cur[i]->intra, fa.out = e->rec_dev, fa.counter = e->counters + 1;
+    for (int r = 0; r < in[i].nrefs; r++) fa.ref[r] = used[i][r]->p.lv[0];
+    return fa;
+  };
+  // Rows: row streams as VulkanEngine orders them (src/vk/vk_engine.cpp:288-294).
+  for (int i = 0; i < n; i++) {
+    celv2hip_engine* e = in[i].e;
     const int streams = std::max(1, std::min(e->row_streams, e->sb_rows));
     const int run = (e->sb_rows + streams - 1) / streams;
     e->order.clear();
-    for (int i = 0; i < run; i++)
-      for (int s = 0; s < streams; s++)
-        if (s * run + i < e->sb_rows) e->order.push_back(s * run + i);
-    InterArgs ia{};
-    ia.src = cs.p.lv[0];
-    ia.nrefs = nrefs, ia.cols = e->fcols[0], ia.rows = e->frows[0], ia.sb_cols = e->sb_cols;
-    ia.phase_bytes = nrefs ? used[0]->p.phase_bytes : 0;
-    ia.best = e->best;
-    ia.counter = e->counters, ia.candidates = e->counting ? e->counters + 2 : nullptr;
-    FinalizeArgs fa{};
-    fa.src = cs.p.lv[0];
-    fa.nrefs = nrefs, fa.sb_cols = e->sb_cols, fa.phase_bytes = ia.phase_bytes, fa.lam = lambda_q8;
-    fa.best = e->best, fa.intra = cs.intra, fa.out = e->rec_dev, fa.counter = e->counters + 1;
-    for (int r = 0; r < nrefs; r++) ia.ref[r] = fa.ref[r] = used[r]->p.lv[0], ia.field[r] = e->fields[r][0];
-    const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
-    if (e->resident || streams == 1) {
-      // All rows in one launch per kernel (resident: results stay on the device).
-      ia.row0 = 0, ia.nrows = e->sb_rows;
-      if (nrefs) launch_inter(ia, e->stream);
-      check(hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
-      fa.row0 = 0, fa.nrows = e->sb_rows;
-      launch_finalize(fa, e->stream);
-      check(hipGetLastError(), "row kernels");
-      check(hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
-      if (!e->resident) {
-        check(hipMemcpyAsync(e->rec_host, e->rec_dev, row_bytes * e->sb_rows, hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync records");
-        for (int r = 0; r < e->sb_rows; r++) check(hipEventRecord(e->row_done[r], e->stream), "hipEventRecord");
-      }
-    } else {
-      for (int r : e->order) {
-        ia.row0 = r, ia.nrows = 1;
-        if (nrefs) launch_inter(ia, e->stream);
-        fa.row0 = r, fa.nrows = 1;
-        launch_finalize(fa, e->stream);
-        check(hipGetLastError(), "row kernels");
-        const size_t off = static_cast<size_t>(r) * row_bytes;
-        check(hipMemcpyAsync(static_cast<char*>(e->rec_host) + off, static_cast<char*>(e->rec_dev) + off, row_bytes,
-                             hipMemcpyDeviceToHost, e->stream), "hipMemcpyAsync row");
-        check(hipEventRecord(e->row_done[r], e->stream), "hipEventRecord");
-      }
-      check(hipEventRecord(e->ev[3], e->stream), "hipEventRecord");
-      check(hipEventRecord(e->ev[4], e->stream), "hipEventRecord");
+    for (int k = 0; k < run; k++)
+      for (int t = 0; t < streams; t++)
+        if (t * run + k < e->sb_rows) e->order.push_back(t * run + k);
+  }
+  const bool per_row = n == 1 && !e0->resident && std::max(1, std::min(e0->row_streams, e0->sb_rows)) > 1;
+  if (!per_row) {
+    // All rows in one launch per kernel.
+    Batch<InterArgs> bi;
+    for (int i = 0; i < n; i++) {
+      if (!in[i].nrefs) continue;
+      InterArgs ia = inter_args(i);
+      ia.row0 = 0, ia.nrows = in[i].e->sb_rows;
+      add_inter(bi, ia);
+      if (bi.n == kMaxItems) launch_inter(bi, S), bi = Batch<InterArgs>();
     }
-    check(hipEventRecord(e->ev[5], e->stream), "hipEventRecord");
-    e->release(cs, false);
-    e->nrefs_last = nrefs;
-    e->last_refs = used;
+    launch_inter(bi, S);
+    for (int i = 0; i < n; i++) check(hipEventRecord(in[i].e->ev[3], S), "hipEventRecord");
+    Batch<FinalizeArgs> bf;
+    for (int i = 0; i < n; i++) {
+      FinalizeArgs fa = finalize_args(i);
+      fa.row0 = 0, fa.nrows = in[i].e->sb_rows;
+      add_finalize(bf, fa);
+      if (bf.n == kMaxItems) launch_finalize(bf, S), bf = Batch<FinalizeArgs>();
+    }
+    launch_finalize(bf, S);
+    check(hipGetLastError(), "row kernels");
+    for (int i = 0; i < n; i++) {
+      celv2hip_engine* e = in[i].e;
+      check(hipEventRecord(e->ev[4], S), "hipEventRecord");
+      if (!e->resident) {
+        const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
+        check(hipMemcpyAsync(e->rec_host, e->rec_dev, row_bytes * e->sb_rows, hipMemcpyDeviceToHost, S), "hipMemcpyAsync records");
+        for (int r = 0; r < e->sb_rows; r++) check(hipEventRecord(e->row_done[r], S), "hipEventRecord");
+      }
+    }
+  } else {
+    // One engine, rows interleaved: each row's inter scores, finalize and record copy in turn.
+    celv2hip_engine* e = e0;
+    const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
+    for (int r : e->order) {
+      if (in[0].nrefs) {
+        Batch<InterArgs> bi;
+        InterArgs ia = inter_args(0);
+        ia.row0 = r, ia.nrows = 1;
+        add_inter(bi, ia);
+        launch_inter(bi, S);
+      }
+      Batch<FinalizeArgs> bf;
+      FinalizeArgs fa = finalize_args(0);
+      fa.row0 = r, fa.nrows = 1;
+      add_finalize(bf, fa);
+      launch_finalize(bf, S);
+      check(hipGetLastError(), "row kernels");
+      const size_t off = static_cast<size_t>(r) * row_bytes;
+      check(hipMemcpyAsync(static_cast<char*>(e->rec_host) + off, static_cast<char*>(e->rec_dev) + off, row_bytes,
+                           hipMemcpyDeviceToHost, S), "hipMemcpyAsync row");
+      check(hipEventRecord(e->row_done[r], S), "hipEventRecord");
+    }
+    check(hipEventRecord(e->ev[3], S), "hipEventRecord");
+    check(hipEventRecord(e->ev[4], S), "hipEventRecord");
+  }
+  for (int i = 0; i < n; i++) {
+    celv2hip_engine* e = in[i].e;
+    check(hipEventRecord(e->ev[5], S), "hipEventRecord");
+    e->release(*cur[i], S);
+    e->nrefs_last = in[i].nrefs;
+    e->last_refs = used[i];
     e->launched = true;
+  }
+  for (int i = 1; i < n; i++) {
+    check(hipEventRecord(e0->order_ev, S), "hipEventRecord");
+    check(hipStreamWaitEvent(in[i].e->stream, e0->order_ev, 0), "hipStreamWaitEvent");
+  }
+}
+
+}  // namespace
+
+int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key, const celv2_luma* refs,
+                    const long long* ref_keys, int nrefs, int64_t lambda_q8) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    const LaunchIn x{e, src, src_key, refs, ref_keys, nrefs, lambda_q8};
+    launch_batch(&x, 1);
+  });
+}
+
+int celv2hip_launch_batch(celv2hip_engine* const* engines, int n, const celv2_luma* srcs, const long long* src_keys,
+                          const celv2_luma* const* refs, const long long* const* ref_keys, const int* nrefs,
+                          const int64_t* lambdas_q8) {
+  if (!engines || n <= 0 || n > 64 || !srcs || !nrefs || !lambdas_q8) {
+    g_error = "launch_batch: bad arguments";
+    return CELV2_ERR_ARG;
+  }
+  for (int i = 0; i < n; i++)
+    if (int rc = validate(engines[i])) return rc;
+  return guarded([&] {
+    std::vector<LaunchIn> in(n);
+    for (int i = 0; i < n; i++)
+      in[i] = LaunchIn{engines[i], &srcs[i], src_keys ? src_keys[i] : -1, refs ? refs[i] : nullptr,
+                       ref_keys ? ref_keys[i] : nullptr, nrefs[i], lambdas_q8[i]};
+    launch_batch(in.data(), n);
   });
 }
 

@@ -42,7 +42,7 @@ __device__ __forceinline__ uint32_t tile_row(const uint8_t* plane, int stride, i
   return load4u(plane + static_cast<ptrdiff_t>(Y + g) * stride + X);
 }
 
-__global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a) {
+__global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch) {
   __shared__ int64_t rc[kMaxRefs][kBlocks];    // per-reference inter cost
   __shared__ uint32_t rmv[kMaxRefs][kBlocks];  // per-reference MV (quarter pel, packed)
   __shared__ int8_t ra[kBlocks], rs[kBlocks];  // best and second reference (-1: none)
@@ -59,7 +59,10 @@ __global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a) {
   __shared__ const uint8_t* refp[kMaxRefs];  // reference planes (level 0, phase 0), indexable per lane
 
   const int tid = threadIdx.x;
-  const int sbx = blockIdx.x, sby = a.row0 + blockIdx.y;
+  const int bi = batch_item(batch, static_cast<int>(blockIdx.x));
+  const FinalizeArgs& a = batch.item[bi];
+  const int lw = static_cast<int>(blockIdx.x) - batch.first[bi];
+  const int sbx = lw % a.sb_cols, sby = a.row0 + lw / a.sb_cols;
   const size_t sb = static_cast<size_t>(sby) * a.sb_cols + sbx;
   const int ox = 64 * sbx, oy = 64 * sby, n = a.nrefs;
   const int64_t lam = a.lam;
@@ -256,9 +259,11 @@ __global__ void __launch_bounds__(kThreads) k_finalize(FinalizeArgs a) {
 
 }  // namespace
 
-void launch_finalize(const FinalizeArgs& a, hipStream_t s) {
-  if (a.nrows <= 0) return;
-  k_finalize<<<dim3(a.sb_cols, a.nrows), kThreads, 0, s>>>(a);
+void add_finalize(Batch<FinalizeArgs>& b, const FinalizeArgs& a) {
+  if (a.nrows > 0) b.add(a, a.sb_cols * a.nrows);
+}
+void launch_finalize(const Batch<FinalizeArgs>& b, hipStream_t s) {
+  if (b.total() > 0) k_finalize<<<b.total(), kThreads, 0, s>>>(b);
 }
 
 // All blocks' tiles, blocks in decreasing tile count (256 .. 2, every class a multiple of 16 tiles),

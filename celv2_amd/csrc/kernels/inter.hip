@@ -31,7 +31,7 @@ __device__ __forceinline__ uint32_t xor_add(uint32_t v, int m) {
 // row by DPP: lane ^ 1 (next square across), lane ^ 8 (next square down), lane ^ 2, and lane ^ 4 for
 // values already equal over each quad.
 
-__global__ void __launch_bounds__(256) k_inter(InterArgs a) {
+__global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
   __shared__ uint32_t mv[kRegion * kRegion];
   __shared__ int cuid[kRegion * kRegion];
   __shared__ int urep[kRegion * kRegion];
@@ -41,9 +41,12 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
   __shared__ int nuniq;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int nwg = a.nrefs * a.nrows * a.sb_cols;
-  const int lg = xcd_logical(blockIdx.x, nwg);
-  if (lg >= nwg) return;  // whole workgroup
+  const int total = batch.total();
+  const int glg = xcd_logical(blockIdx.x, total);
+  if (glg >= total) return;  // whole workgroup
+  const int bi = batch_item(batch, glg);
+  const InterArgs& a = batch.item[bi];
+  const int lg = glg - batch.first[bi];
   const int r = lg / (a.nrows * a.sb_cols), rem = lg - r * a.nrows * a.sb_cols;
   const int sby = a.row0 + rem / a.sb_cols, sbx = rem % a.sb_cols;
   const int ox = 64 * sbx, oy = 64 * sby;
@@ -190,10 +193,13 @@ __global__ void __launch_bounds__(256) k_inter(InterArgs a) {
 
 }  // namespace
 
-void launch_inter(const InterArgs& a, hipStream_t s) {
+void add_inter(Batch<InterArgs>& b, const InterArgs& a) {
   const int nwg = a.nrefs * a.nrows * a.sb_cols;
-  if (nwg <= 0) return;
-  k_inter<<<((nwg + 7) / 8) * 8, 256, 0, s>>>(a);
+  if (nwg > 0) b.add(a, nwg);
+}
+void launch_inter(const Batch<InterArgs>& b, hipStream_t s) {
+  if (b.total() <= 0) return;
+  k_inter<<<((b.total() + 7) / 8) * 8, 256, 0, s>>>(b);
 }
 
 // Candidate k of block b as a cell of the 10 x 10 region around the superblock (region column / row

@@ -58,13 +58,18 @@ __device__ uint32_t g_itile[kIntraTiles];   // x/4 | y/4 << 4 | tx << 8 | ty << 
 __device__ uint16_t g_pos_block[420];       // block index of each position
 __device__ __align__(16) uint32_t g_dir[kDirEntries];  // uint16 offset | f << 16 | step << 21
 
-__global__ void __launch_bounds__(256) k_intra(DPlane src, int sb_cols, int row0, uint32_t* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
   __shared__ __align__(16) uint8_t reg[2 * kRBytes];  // above copy, left copy
   __shared__ float acc[kMaxGroupBlocks * kModes];
   __shared__ uint32_t dcv[kMaxGroupBlocks];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int q = blockIdx.x & (kGroups - 1), sbx = blockIdx.x >> 2, sby = row0 + blockIdx.y;
+  const int bi = batch_item(batch, static_cast<int>(blockIdx.x));
+  const IntraItem& it = batch.item[bi];
+  const DPlane& src = it.src;
+  const int lw = static_cast<int>(blockIdx.x) - batch.first[bi], sb_cols = it.sb_cols;
+  const int q = lw & (kGroups - 1), sbx = (lw >> 2) % sb_cols, sby = it.row0 + (lw >> 2) / sb_cols;
+  uint32_t* __restrict__ const out = it.out;
   const int ox = 64 * sbx, oy = 64 * sby;
   const int p0 = g_group_pos[q], np = g_group_pos[q + 1] - p0;
 
@@ -227,9 +232,11 @@ DirSample dir_sample(int w, int h, int angle, int i, int j) {
 
 }  // namespace
 
-void launch_intra(const DPlane& src, int sb_cols, int row0, int nrows, uint32_t* out, hipStream_t s) {
-  if (nrows <= 0) return;
-  k_intra<<<dim3(kGroups * sb_cols, nrows), 256, 0, s>>>(src, sb_cols, row0, out);
+void add_intra(Batch<IntraItem>& b, const IntraItem& t) {
+  if (t.nrows > 0) b.add(t, kGroups * t.sb_cols * t.nrows);
+}
+void launch_intra(const Batch<IntraItem>& b, hipStream_t s) {
+  if (b.total() > 0) k_intra<<<b.total(), 256, 0, s>>>(b);
 }
 
 // The shape-sorted tile list, the block order, the group ranges and the directional sample table.

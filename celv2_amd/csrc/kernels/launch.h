@@ -3,15 +3,30 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "batch.h"
 #include "common.h"
 
 namespace celv2hip {
 
-// planes.hip
-void launch_import8(const uint8_t* src, int sstride, const DPlane& out, hipStream_t s);
-void launch_import16(const uint16_t* src, int sstride, int bit_depth, const DPlane& out, hipStream_t s);
-void launch_downsample(const DPlane& in, const DPlane& out, hipStream_t s);
-void launch_subpel(const DPlane& level0, long long phase_bytes, hipStream_t s);
+// planes.hip: every launch takes a batch of planes (add_* appends one with its workgroup count).
+struct ImportItem {  // a frame's luma (8-bit, or 9..16-bit in uint16) into a bordered level-0 plane
+  const void* src;
+  int sstride, bit_depth, wide;
+  DPlane out;
+};
+struct DownItem {  // level l+1 from level l
+  DPlane in, out;
+};
+struct SubpelItem {  // the 15 quarter-pel phase planes of a level-0 plane, phase_bytes apart
+  DPlane in;
+  long long phase_bytes;
+};
+void add_import(Batch<ImportItem>& b, const ImportItem& t);
+void add_downsample(Batch<DownItem>& b, const DownItem& t);
+void add_subpel(Batch<SubpelItem>& b, const SubpelItem& t);
+void launch_import(const Batch<ImportItem>& b, hipStream_t s);
+void launch_downsample(const Batch<DownItem>& b, hipStream_t s);
+void launch_subpel(const Batch<SubpelItem>& b, hipStream_t s);
 
 // me.hip: one pyramid level's motion fields for n references at once. cur: the source's level;
 // ref[r]: reference r's level (at level 0 with its phases, phase_bytes apart); coarser[r]: the level
@@ -25,11 +40,18 @@ struct MeLevelArgs {
   long long phase_bytes;
   unsigned long long* searches;  // centres searched, spread over 64 slots 16 words apart (may be null)
 };
-void launch_me_level(const MeLevelArgs& a, hipStream_t s);
+void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a);
+void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s);
 
 // intra.hip: SATD of the 13 intra modes of the 420 candidate blocks of levels 1-3 of superblock
 // rows [row0, row0 + nrows): out[(sb * 425 + b) * 13 + m] (blocks 0..4 untouched).
-void launch_intra(const DPlane& src, int sb_cols, int row0, int nrows, uint32_t* out, hipStream_t s);
+struct IntraItem {
+  DPlane src;
+  int sb_cols, row0, nrows;
+  uint32_t* out;
+};
+void add_intra(Batch<IntraItem>& b, const IntraItem& t);
+void launch_intra(const Batch<IntraItem>& b, hipStream_t s);
 
 // inter.hip: per superblock (rows [row0, row0 + nrows)) and reference, every candidate block's best
 // single-reference SATD and quarter-pel MV: best[((sb * kMaxRefs) + r) * 425 + b] = {satd, packed mv}.
@@ -43,7 +65,8 @@ struct InterArgs {
   unsigned long long* counter;     // unique MVs scored (may be null)
   unsigned long long* candidates;  // non-repeated (block, MV) candidates scored (may be null)
 };
-void launch_inter(const InterArgs& a, hipStream_t s);
+void add_inter(Batch<InterArgs>& b, const InterArgs& a);
+void launch_inter(const Batch<InterArgs>& b, hipStream_t s);
 
 // finalize.hip: per superblock, the reference choice, compound, intra choice, partition DP and the
 // exported record (include/celv2hip.h celv2_sb) at out[sb].
@@ -58,7 +81,8 @@ struct FinalizeArgs {
   void* out;               // celv2_sb records, one per superblock of the frame
   unsigned long long* counter;  // compound blocks scored (may be null)
 };
-void launch_finalize(const FinalizeArgs& a, hipStream_t s);
+void add_finalize(Batch<FinalizeArgs>& b, const FinalizeArgs& a);
+void launch_finalize(const Batch<FinalizeArgs>& b, hipStream_t s);
 
 // Static tables the kernels read (built once per process on the host, copied to the device).
 void upload_tables(hipStream_t s);

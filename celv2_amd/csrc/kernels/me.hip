@@ -68,13 +68,17 @@ __device__ __forceinline__ uint32_t score_window(const uint32_t* win, int sh, in
 constexpr uint32_t kRefDx = (1u << 0) | (0u << 2) | (1u << 4) | (2u << 6) | (0u << 8) | (2u << 10) | (0u << 12) | (1u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u << 8) | (1u << 10) | (2u << 12) | (2u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 
-__global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
+__global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
   __shared__ uint32_t wins[4][2][kWinLds];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addressing
-  const int per_ref = a.rows * a.cols, nitems = a.nrefs * per_ref, ngroups = (nitems + 3) >> 2;
-  const int lg = xcd_logical(blockIdx.x, ngroups);
-  if (lg >= ngroups) return;
+  const int total = batch.total();
+  const int glg = xcd_logical(blockIdx.x, total);
+  if (glg >= total) return;
+  const int bi = batch_item(batch, glg);
+  const MeLevelArgs& a = batch.item[bi];
+  const int lg = glg - batch.first[bi];
+  const int per_ref = a.rows * a.cols, nitems = a.nrefs * per_ref;
   const int item = 4 * lg + wave;
   if (item >= nitems) return;  // whole wave
   const int r = item / per_ref, rem = item - r * per_ref, by = rem / a.cols, bx = rem - by * a.cols;
@@ -225,12 +229,13 @@ __global__ void __launch_bounds__(256) k_me_level(MeLevelArgs a) {
 
 }  // namespace
 
-void launch_me_level(const MeLevelArgs& a, hipStream_t s) {
+void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a) {
   const int nitems = a.nrefs * a.rows * a.cols;
-  if (nitems <= 0) return;
-  const int ngroups = (nitems + 3) / 4;
-  const int grid = ((ngroups + 7) / 8) * 8;
-  k_me_level<<<grid, 256, 0, s>>>(a);
+  if (nitems > 0) b.add(a, (nitems + 3) / 4);
+}
+void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s) {
+  if (b.total() <= 0) return;
+  k_me_level<<<((b.total() + 7) / 8) * 8, 256, 0, s>>>(b);
 }
 
 }  // namespace celv2hip

@@ -4,6 +4,7 @@
 // Every kernel writes the whole bordered plane, computing a border sample from the clamped position
 // (so the border equals what the reference's clamped reads return). Threads own 4 horizontally
 // adjacent samples of one row and store them as one dword (bordered rows start 64-byte aligned).
+#include "batch.h"
 #include "common.h"
 #include "launch.h"
 
@@ -20,24 +21,43 @@ __device__ __forceinline__ uint32_t to8(uint32_t v, int bit_depth) {
   return r > 255 ? 255 : r;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_import(const T* __restrict__ src, int sstride, int bit_depth, DPlane out) {
-  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);  // group of 4 samples
-  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
+// Workgroups of one plane: 64 x 4-sample groups of 4 bordered rows each.
+__device__ __forceinline__ int plane_cols(const DPlane& p) { return ((p.w + 2 * kBorder + 3) / 4 + 63) / 64; }
+
+// This workgroup's item of a batch over planes and its plane-local block (bx, by).
+#define CELV2_PLANE_BLOCK(b, plane_of, t, bx, by)                                   \
+  const int item_ = batch_item(b, static_cast<int>(blockIdx.x));                 \
+  const auto& t = b.item[item_];                                                  \
+  const int lw_ = static_cast<int>(blockIdx.x) - b.first[item_], cols_ = plane_cols(plane_of); \
+  const int by = lw_ / cols_, bx = lw_ - by * cols_;
+
+__global__ void __launch_bounds__(256) k_import(Batch<ImportItem> b) {
+  CELV2_PLANE_BLOCK(b, t.out, t, bx, by)
+  const DPlane& out = t.out;
+  const int gx = bx * 64 + (threadIdx.x & 63);  // group of 4 samples
+  const int y = by * 4 + (threadIdx.x >> 6) - kBorder;
   const int x0 = 4 * gx - kBorder;
   if (y >= out.h + kBorder || x0 >= out.w + kBorder) return;
   const int sy = clampi(y, 0, out.h - 1);
-  const T* row = src + static_cast<size_t>(sy) * sstride;
   uint32_t word = 0;
+  if (t.wide) {
+    const uint16_t* row = static_cast<const uint16_t*>(t.src) + static_cast<size_t>(sy) * t.sstride;
 #pragma unroll
-  for (int k = 0; k < 4; k++) word |= to8(static_cast<uint32_t>(row[clampi(x0 + k, 0, out.w - 1)]), bit_depth) << (8 * k);
+    for (int k = 0; k < 4; k++) word |= to8(row[clampi(x0 + k, 0, out.w - 1)], t.bit_depth) << (8 * k);
+  } else {
+    const uint8_t* row = static_cast<const uint8_t*>(t.src) + static_cast<size_t>(sy) * t.sstride;
+#pragma unroll
+    for (int k = 0; k < 4; k++) word |= static_cast<uint32_t>(row[clampi(x0 + k, 0, out.w - 1)]) << (8 * k);
+  }
   *reinterpret_cast<uint32_t*>(out.p + static_cast<ptrdiff_t>(y) * out.stride + x0) = word;
 }
 
 // Level l+1 from level l: the rounded 2x2 mean (downsample_px, src/gpu/inter_common.h:256-259).
-__global__ void __launch_bounds__(256) k_downsample(DPlane in, DPlane out) {
-  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
+__global__ void __launch_bounds__(256) k_downsample(Batch<DownItem> b) {
+  CELV2_PLANE_BLOCK(b, t.out, t, bx, by)
+  const DPlane &in = t.in, &out = t.out;
+  const int gx = bx * 64 + (threadIdx.x & 63);
+  const int y = by * 4 + (threadIdx.x >> 6) - kBorder;
   const int x0 = 4 * gx - kBorder;
   if (y >= out.h + kBorder || x0 >= out.w + kBorder) return;
   const int cy = clampi(y, 0, out.h - 1);
@@ -124,9 +144,12 @@ __device__ __forceinline__ void subpel_group(const DPlane& in, int cy, int c0, u
   }
 }
 
-__global__ void __launch_bounds__(256) k_subpel(DPlane in, long long phase_bytes) {
-  const int gx = blockIdx.x * 64 + (threadIdx.x & 63);
-  const int y = static_cast<int>(blockIdx.y) * 4 + (threadIdx.x >> 6) - kBorder;
+__global__ void __launch_bounds__(256) k_subpel(Batch<SubpelItem> b) {
+  CELV2_PLANE_BLOCK(b, t.in, t, bx, by)
+  const DPlane& in = t.in;
+  const long long phase_bytes = t.phase_bytes;
+  const int gx = bx * 64 + (threadIdx.x & 63);
+  const int y = by * 4 + (threadIdx.x >> 6) - kBorder;
   const int x0 = 4 * gx - kBorder;
   if (y >= in.h + kBorder || x0 >= in.w + kBorder) return;
   const int cy = clampi(y, 0, in.h - 1);
@@ -138,24 +161,24 @@ __global__ void __launch_bounds__(256) k_subpel(DPlane in, long long phase_bytes
   for (int ph = 1; ph < 16; ph++) *reinterpret_cast<uint32_t*>(out + phase_bytes * ph) = word[ph];
 }
 
-dim3 plane_grid(const DPlane& p) {
+int plane_groups(const DPlane& p) {
   const int groups = (p.w + 2 * kBorder + 3) / 4;
-  return dim3((groups + 63) / 64, (p.h + 2 * kBorder + 3) / 4);
+  return ((groups + 63) / 64) * ((p.h + 2 * kBorder + 3) / 4);
 }
 
 }  // namespace
 
-void launch_import8(const uint8_t* src, int sstride, const DPlane& out, hipStream_t s) {
-  k_import<uint8_t><<<plane_grid(out), 256, 0, s>>>(src, sstride, 8, out);
+void launch_import(const Batch<ImportItem>& b, hipStream_t s) {
+  if (b.total()) k_import<<<b.total(), 256, 0, s>>>(b);
 }
-void launch_import16(const uint16_t* src, int sstride, int bit_depth, const DPlane& out, hipStream_t s) {
-  k_import<uint16_t><<<plane_grid(out), 256, 0, s>>>(src, sstride, bit_depth, out);
+void launch_downsample(const Batch<DownItem>& b, hipStream_t s) {
+  if (b.total()) k_downsample<<<b.total(), 256, 0, s>>>(b);
 }
-void launch_downsample(const DPlane& in, const DPlane& out, hipStream_t s) {
-  k_downsample<<<plane_grid(out), 256, 0, s>>>(in, out);
+void launch_subpel(const Batch<SubpelItem>& b, hipStream_t s) {
+  if (b.total()) k_subpel<<<b.total(), 256, 0, s>>>(b);
 }
-void launch_subpel(const DPlane& level0, long long phase_bytes, hipStream_t s) {
-  k_subpel<<<plane_grid(level0), 256, 0, s>>>(level0, phase_bytes);
-}
+void add_import(Batch<ImportItem>& b, const ImportItem& t) { b.add(t, plane_groups(t.out)); }
+void add_downsample(Batch<DownItem>& b, const DownItem& t) { b.add(t, plane_groups(t.out)); }
+void add_subpel(Batch<SubpelItem>& b, const SubpelItem& t) { b.add(t, plane_groups(t.in)); }
 
 }  // namespace celv2hip

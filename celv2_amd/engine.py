@@ -41,6 +41,9 @@ def lib():
         L.celv2hip_set_resident.argtypes = [P, I]
         L.celv2hip_prepare.argtypes = [P, ctypes.POINTER(Luma), LL]
         L.celv2hip_launch.argtypes = [P, ctypes.POINTER(Luma), LL, ctypes.POINTER(Luma), ctypes.POINTER(LL), I, ctypes.c_int64]
+        L.celv2hip_launch_batch.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(Luma), ctypes.POINTER(LL),
+                                            ctypes.POINTER(ctypes.POINTER(Luma)), ctypes.POINTER(ctypes.POINTER(LL)),
+                                            ctypes.POINTER(I), ctypes.POINTER(ctypes.c_int64)]
         L.celv2hip_row_order.argtypes = [P, ctypes.POINTER(I), I]
         L.celv2hip_wait_row.argtypes = [P, I, P]
         L.celv2hip_fetch_row.argtypes = [P, I, P]
@@ -204,6 +207,23 @@ class RawEngine:
         arr = (Luma * max(1, len(refs)))(*refs)
         k = (LL * max(1, len(refs)))(*keys) if keys is not None else None
         self._check(lib().celv2hip_launch(self.h, ctypes.byref(src), src_key, arr, k, len(refs), lam), "launch")
+
+    @staticmethod
+    def launch_batch(engines, srcs, refs, lams, keys=None, src_keys=None):
+        """One frame of each engine in shared launches (celv2hip_launch_batch)."""
+        n = len(engines)
+        eng = (P * n)(*[e.h for e in engines])
+        sarr = (Luma * n)(*srcs)
+        ref_arrs = [(Luma * max(1, len(r)))(*r) for r in refs]
+        rp = (ctypes.POINTER(Luma) * n)(*[ctypes.cast(a, ctypes.POINTER(Luma)) for a in ref_arrs])
+        key_arrs = [(LL * max(1, len(k)))(*k) for k in keys] if keys is not None else None
+        kp = (ctypes.POINTER(LL) * n)(*[ctypes.cast(a, ctypes.POINTER(LL)) for a in key_arrs]) if keys is not None else None
+        sk = (LL * n)(*src_keys) if src_keys is not None else None
+        nr = (I * n)(*[len(r) for r in refs])
+        lm = (ctypes.c_int64 * n)(*lams)
+        rc = lib().celv2hip_launch_batch(eng, n, sarr, sk, rp, kp, nr, lm)
+        if rc != 0:
+            raise EngineError(f"launch_batch: {last_error()}")
 
     def collect(self):
         out = np.zeros(self.sb_cols * self.sb_rows, dtype=SB_DTYPE)

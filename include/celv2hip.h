@@ -115,6 +115,16 @@ int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key);
 int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key, const celv2_luma* refs,
                     const long long* ref_keys, int nrefs, int64_t lambda_q8);
 
+/* One frame of each of n engines (1..64, one device) in shared launches: each stage runs once over
+ * every engine's work (at most 8 engines per kernel launch), on the first engine's stream; each
+ * engine's decisions are exactly those of celv2hip_launch with the same inputs. Per engine i:
+ * srcs[i], src_keys[i] (src_keys may be NULL: no keys), nrefs[i] references refs[i][0..nrefs[i]) with
+ * keys ref_keys[i] (ref_keys or ref_keys[i] may be NULL), lambdas_q8[i]. No counterpart in the
+ * reference's DecisionEngine: the batched form for several streams sharing one GPU. */
+int celv2hip_launch_batch(celv2hip_engine* const* engines, int n, const celv2_luma* srcs, const long long* src_keys,
+                          const celv2_luma* const* refs, const long long* const* ref_keys, const int* nrefs,
+                          const int64_t* lambdas_q8);
+
 /* The superblock rows in the order the last launch queued them (row streams); returns the count. */
 int celv2hip_row_order(const celv2hip_engine* e, int* rows, int cap);
 

@@ -100,3 +100,26 @@ def test_resident_device_frames(gpu):
             b.free()
     want = ol.decide_frame(y8[2], [y8[1], y8[0]], lam, w, h)
     check(got, want, e.sb_cols)
+
+
+def test_batched_launch_matches_single_launches(gpu):
+    """Three engines of different sizes in shared launches (celv2hip_launch_batch) decide exactly as
+    each alone; a second batched frame reuses the cached references by key."""
+    sizes = [(352, 288), (202, 122), (160, 96)]
+    clips = [synth.clip_frames(synth.params(0, 11 + i, w, h), 4)[1] for i, (w, h) in enumerate(sizes)]
+    lam = gpu.lambda_q8(120)
+    batch = [gpu.RawEngine(w, h) for (w, h) in sizes]
+    alone = [gpu.RawEngine(w, h) for (w, h) in sizes]
+    for f in (2, 3):
+        srcs = [gpu.RawEngine.luma(c[f]) for c in clips]
+        refs = [[gpu.RawEngine.luma(c[f - 1]), gpu.RawEngine.luma(c[f - 2])] for c in clips]
+        keys = [[f - 1, f - 2] for _ in clips]
+        gpu.RawEngine.launch_batch(batch, srcs, refs, [lam] * 3, keys=keys, src_keys=[f] * 3)
+        got = [e.collect() for e in batch]
+        for i, e in enumerate(alone):
+            e.launch(srcs[i], refs[i], lam, keys=keys[i], src_key=f)
+            want = e.collect()
+            diff = compare_maps(got[i], want, e.sb_cols)
+            assert diff is None, (f, sizes[i], diff)
+    for e in batch + alone:
+        e.close()
