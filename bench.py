@@ -69,6 +69,87 @@ def cpu_baseline(y8, frame, refs, lam, gpu_map, threads):
     return dt, (compare_maps(gpu_map, out, (w + 63) // 64) if gpu_map is not None else None)
 
 
+# The mixed-resolution workload: six streams, one frame of each per step.
+C2_SIZES = [(640, 360), (854, 480), (1280, 720), (1280, 720), (1920, 1080), (2560, 1440)]
+
+
+def bench_batched(NR, lam, steps=20, warmup=3, nf=16):
+    """Six streams of 16 10-bit frames (seeded generator, workload 1 at each size), 7 references,
+    engine_bench method per stream, frames resident in HBM. Timed twice over the same steps: each
+    engine launched on its own (the plain form), then all six in shared launches
+    (celv2hip_launch_batch); both prepare() each stream's next frame. Returns the two rates and
+    whether the last step's decisions agree between the forms."""
+    t = time.perf_counter()
+    streams = []
+    for k, (w, h) in enumerate(C2_SIZES):
+        y16, _ = synth.clip_frames(synth.params(1, 100 + k, w, h), nf)
+        streams.append([engine.DeviceBuffer(f) for f in y16])
+    log(f"mixed workload: {len(C2_SIZES)} streams x {nf} frames in {time.perf_counter() - t:.1f}s")
+    engines = [engine.RawEngine(w, h) for (w, h) in C2_SIZES]
+    for e in engines:
+        e.set_resident(True)
+    lums = [[engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev] for dev in streams]
+    decidable = list(range(NR, nf))
+    sbs = sum(e.sb_cols * e.sb_rows for e in engines)
+
+    def frame(s):
+        return decidable[s % len(decidable)], decidable[(s + 1) % len(decidable)]
+
+    def step_plain(s):
+        i, nxt = frame(s)
+        refs = [i - k for k in range(1, NR + 1)]
+        for e, lum in zip(engines, lums):
+            e.launch(lum[i], [lum[r] for r in refs], lam, keys=refs, src_key=i)
+            e.prepare(lum[nxt], nxt)
+
+    def step_batch(s, groups):
+        i, nxt = frame(s)
+        refs = [i - k for k in range(1, NR + 1)]
+        for g in groups:
+            engine.RawEngine.launch_batch([engines[k] for k in g], [lums[k][i] for k in g],
+                                          [[lums[k][r] for r in refs] for k in g], [lam] * len(g), keys=[refs] * len(g),
+                                          src_keys=[i] * len(g))
+        for e, lum in zip(engines, lums):
+            e.prepare(lum[nxt], nxt)
+
+    forms = [("plain", step_plain), ("batched", lambda s: step_batch(s, [range(6)])),
+             ("batched_2", lambda s: step_batch(s, [(0, 2, 5), (1, 3, 4)])),
+             ("batched_3", lambda s: step_batch(s, [(0, 5), (1, 4), (2, 3)]))]
+    rates, maps = {}, {}
+    for name, step in forms:
+        for e, lum in zip(engines, lums):
+            e.prepare(lum[frame(0)[0]], frame(0)[0])
+        for s in range(warmup):
+            step(s)
+        for e in engines:
+            e.sync()
+        t0 = time.perf_counter()
+        for s in range(warmup, warmup + steps):
+            step(s)
+        for e in engines:
+            e.sync()
+        wall = time.perf_counter() - t0
+        rates[name] = (steps * len(engines) / wall, steps * sbs / wall, wall / steps * 1e3)
+        maps[name] = [e.fetch() for e in engines]
+        log(f"mixed workload, {name}: {rates[name][0]:.1f} frames/s, {rates[name][1]:.0f} superblocks/s, {rates[name][2]:.3f} ms per step")
+    same = all(compare_maps(a, b, e.sb_cols) is None for f in maps if f != "plain"
+               for a, b, e in zip(maps["plain"], maps[f], engines))
+    best = max((f for f in rates if f != "plain"), key=lambda f: rates[f][0])
+    for e in engines:
+        e.close()
+    for dev in streams:
+        for b in dev:
+            b.free()
+    return {"workload": "6 streams (640x360, 854x480, 1280x720 x2, 1920x1080, 2560x1440), 16 frames each, 10-bit, "
+                        "7 references, engine_bench method per stream, one frame of every stream per step",
+            "superblocks_per_step": sbs, "steps": steps,
+            "best_grouping": best, "frames_per_s": round(rates[best][0], 2), "superblocks_per_s": round(rates[best][1]),
+            "ms_per_step": round(rates[best][2], 4),
+            "by_grouping": {f: {"frames_per_s": round(r[0], 2), "ms_per_step": round(r[2], 4)} for f, r in rates.items()},
+            "plain_frames_per_s": round(rates["plain"][0], 2), "plain_superblocks_per_s": round(rates["plain"][1]),
+            "speedup": round(rates[best][0] / rates["plain"][0], 3), "decisions_equal_plain": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +160,7 @@ def main():
     ap.add_argument("--qindex", type=int, default=130)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the mixed-resolution batched form")
     ap.add_argument("--e2e-steps", type=int, default=12)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
@@ -193,6 +275,8 @@ def main():
         e2e = round(1.0 / (sum(ts[2:]) / len(ts[2:])), 2)
         HE.close()
 
+    batched = None if args.no_batched else bench_batched(NR, lam)
+
     cpu = None
     parity = None
     if not args.no_cpu:
@@ -242,6 +326,8 @@ def main():
         "int_ops_per_frame": total_ops,
         "all_kernels_tops": round(total_ops * fps / 1e12, 3),
         "engine_path_fps_host_buffers": e2e,
+        "forms": {"plain": {"frames_per_s": round(fps, 2), "scored_candidates_per_s": round(cands * fps)},
+                  "batched": batched},
     }
     print(json.dumps(out), flush=True)
 
