@@ -140,6 +140,9 @@ struct celv2hip_engine {
   uint32_t* fields[kMaxRefs][kLevels] = {};
   int fcols[kLevels], frows[kLevels];
   uint2* best = nullptr;
+  uint8_t* kmap = nullptr;       // per-superblock reference limits on the device (when kmap_on)
+  bool kmap_on = false;
+  int64_t* sb_cost = nullptr;    // per-superblock decision cost of the last launch
   void* rec_dev = nullptr;   // celv2_sb per superblock
   void* rec_host = nullptr;  // pinned
   std::vector<int> order;
@@ -181,6 +184,8 @@ struct celv2hip_engine {
       for (uint32_t* f : r)
         if (f) (void)hipFree(f);
     if (best) (void)hipFree(best);
+    if (kmap) (void)hipFree(kmap);
+    if (sb_cost) (void)hipFree(sb_cost);
     if (rec_dev) (void)hipFree(rec_dev);
     if (rec_host) (void)hipHostFree(rec_host);
     if (counters) (void)hipFree(counters);
@@ -323,6 +328,8 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
       w /= 2, h /= 2;
     }
     check(hipMalloc(&e->best, e->nsb() * kMaxRefs * kBlocks * sizeof(uint2)), "hipMalloc best");
+    check(hipMalloc(&e->kmap, e->nsb()), "hipMalloc kmap");
+    check(hipMalloc(&e->sb_cost, e->nsb() * sizeof(int64_t)), "hipMalloc sb_cost");
     check(hipMalloc(&e->rec_dev, e->nsb() * sizeof(celv2_sb)), "hipMalloc records");
     check(hipHostMalloc(&e->rec_host, e->nsb() * sizeof(celv2_sb), hipHostMallocDefault), "hipHostMalloc records");
     check(hipMalloc(&e->counters, 80 * 16 * sizeof(unsigned long long)), "hipMalloc counters");
@@ -366,6 +373,35 @@ int celv2hip_set_resident(celv2hip_engine* e, int on) {
   if (!e) return CELV2_ERR_ARG;
   e->resident = on != 0;
   return CELV2_OK;
+}
+
+int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    if (!k || count == 0) {
+      e->kmap_on = false;
+      return;
+    }
+    if (count != static_cast<int>(e->nsb())) throw Failure(CELV2_ERR_SIZE, "one limit per superblock");
+    for (int i = 0; i < count; i++)
+      if (k[i] < 1 || k[i] > kMaxRefs) throw Failure(CELV2_ERR_ARG, "reference limits are 1..7");
+    e->sync_all();
+    check(hipMemcpy(e->kmap, k, static_cast<size_t>(count), hipMemcpyHostToDevice), "hipMemcpy kmap");
+    e->kmap_on = true;
+  });
+}
+
+long long celv2hip_fetch_sb_costs(celv2hip_engine* e, int64_t* out, long long cap) {
+  if (int rc = validate(e)) return rc;
+  long long n = 0;
+  const int rc = guarded([&] {
+    if (!e->launched) throw Failure(CELV2_ERR_STATE, "no launch");
+    n = static_cast<long long>(e->nsb());
+    if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "cost buffer too small");
+    e->sync_all();
+    check(hipMemcpy(out, e->sb_cost, static_cast<size_t>(n) * sizeof(int64_t), hipMemcpyDeviceToHost), "hipMemcpy sb_cost");
+  });
+  return rc ? rc : n;
 }
 
 int celv2hip_set_counting(celv2hip_engine* e, int on) {
@@ -528,6 +564,7 @@ void launch_batch(const LaunchIn* in, int n) {
       a.pcols = l + 1 < kLevels ? e->fcols[l + 1] : 0, a.prows = l + 1 < kLevels ? e->frows[l + 1] : 0;
       a.level0 = l == 0, a.phase_bytes = used[i][0]->p.phase_bytes;
       a.searches = e->counters + 16;
+      a.kmap = e->kmap_on ? e->kmap : nullptr, a.sb_cols = e->sb_cols, a.sb_rows = e->sb_rows, a.level = l;
       for (int r = 0; r < nr; r++) {
         if (!e->fields[r][l]) check(hipMalloc(&e->fields[r][l], static_cast<size_t>(e->fcols[l]) * e->frows[l] * sizeof(uint32_t)), "hipMalloc field");
         a.ref[r] = used[i][r]->p.lv[l];
@@ -550,6 +587,7 @@ void launch_batch(const LaunchIn* in, int n) {
     ia.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
     ia.best = e->best;
     ia.counter = e->counters, ia.candidates = e->counting ? e->counters + 2 : nullptr;
+    ia.kmap = e->kmap_on ? e->kmap : nullptr;
     for (int r = 0; r < in[i].nrefs; r++) ia.ref[r] = used[i][r]->p.lv[0], ia.field[r] = e->fields[r][0];
     return ia;
   };
@@ -560,6 +598,7 @@ void launch_batch(const LaunchIn* in, int n) {
     fa.nrefs = in[i].nrefs, fa.sb_cols = e->sb_cols, fa.lam = in[i].lam;
     fa.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
     fa.best = e->best, fa.intra = cur[i]->intra, fa.out = e->rec_dev, fa.counter = e->counters + 1;
+    fa.kmap = e->kmap_on ? e->kmap : nullptr, fa.sb_cost = e->sb_cost;
     for (int r = 0; r < in[i].nrefs; r++) fa.ref[r] = used[i][r]->p.lv[0];
     return fa;
   };

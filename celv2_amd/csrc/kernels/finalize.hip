@@ -64,7 +64,9 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
   const int lw = static_cast<int>(blockIdx.x) - batch.first[bi];
   const int sbx = lw % a.sb_cols, sby = a.row0 + lw / a.sb_cols;
   const size_t sb = static_cast<size_t>(sby) * a.sb_cols + sbx;
-  const int ox = 64 * sbx, oy = 64 * sby, n = a.nrefs;
+  const int ox = 64 * sbx, oy = 64 * sby;
+  // With a reference limit, the superblock is decided exactly as with its nearest kmap[sb] references.
+  const int n = a.kmap ? min(a.nrefs, static_cast<int>(a.kmap[sb])) : a.nrefs;
   const int64_t lam = a.lam;
   celv2_sb* const out = static_cast<celv2_sb*>(a.out) + sb;
   if (tid < kMaxRefs) refp[tid] = tid < n ? a.ref[tid].p : nullptr;
@@ -206,6 +208,7 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
 
   // Preorder walk of the chosen tree (one thread): tree entries and the leaves' blocks.
   if (tid == 0) {
+    if (a.sb_cost) a.sb_cost[sb] = sq_cost[0];
     int st_level[16], st_qx[16], st_qy[16], sp = 1, nt = 0, nl = 0;
     st_level[0] = 0, st_qx[0] = 0, st_qy[0] = 0;
     while (sp > 0) {

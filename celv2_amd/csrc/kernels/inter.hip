@@ -50,6 +50,7 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
   const int r = lg / (a.nrows * a.sb_cols), rem = lg - r * a.nrows * a.sb_cols;
   const int sby = a.row0 + rem / a.sb_cols, sbx = rem % a.sb_cols;
   const int ox = 64 * sbx, oy = 64 * sby;
+  if (a.kmap && r >= a.kmap[sby * a.sb_cols + sbx]) return;  // reference pruned here: whole workgroup
   const uint32_t* field = a.field[r];
 
   if (tid < kRegion * kRegion) {

@@ -39,6 +39,11 @@ struct MeLevelArgs {
   int nrefs, cols, rows, pcols, prows, level0;
   long long phase_bytes;
   unsigned long long* searches;  // centres searched, spread over 64 slots 16 words apart (may be null)
+  // Per-superblock reference limits (may be null): superblock s uses references 0 .. kmap[s] - 1.
+  // At levels 0 and 1 a block is searched for reference r only when a superblock within one
+  // superblock of it keeps r (every field cell a kept superblock reads, and their centres' cells).
+  const uint8_t* kmap;
+  int sb_cols, sb_rows, level;
 };
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a);
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s);
@@ -64,6 +69,7 @@ struct InterArgs {
   uint2* best;
   unsigned long long* counter;     // unique MVs scored (may be null)
   unsigned long long* candidates;  // non-repeated (block, MV) candidates scored (may be null)
+  const uint8_t* kmap;             // per-superblock reference limits (may be null)
 };
 void add_inter(Batch<InterArgs>& b, const InterArgs& a);
 void launch_inter(const Batch<InterArgs>& b, hipStream_t s);
@@ -80,6 +86,8 @@ struct FinalizeArgs {
   const uint32_t* intra;   // from launch_intra
   void* out;               // celv2_sb records, one per superblock of the frame
   unsigned long long* counter;  // compound blocks scored (may be null)
+  const uint8_t* kmap;     // per-superblock reference limits (may be null): n = min(nrefs, kmap[sb])
+  int64_t* sb_cost;        // the superblock's decision cost (the 64x64 square's DP cost) (may be null)
 };
 void add_finalize(Batch<FinalizeArgs>& b, const FinalizeArgs& a);
 void launch_finalize(const Batch<FinalizeArgs>& b, hipStream_t s);

@@ -82,6 +82,13 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
   const int item = 4 * lg + wave;
   if (item >= nitems) return;  // whole wave
   const int r = item / per_ref, rem = item - r * per_ref, by = rem / a.cols, bx = rem - by * a.cols;
+  if (a.kmap && a.level <= 1 && r > 0) {
+    // Searched only near a superblock that keeps reference r (lanes 0..8: the 3x3 neighbourhood).
+    const int shift = 3 - a.level;  // 8x8 blocks per superblock side: 8 at level 0, 4 at level 1
+    const int sx = (bx >> shift) + lane % 3 - 1, sy = (by >> shift) + lane / 3 - 1;
+    const bool keeps = lane < 9 && sx >= 0 && sy >= 0 && sx < a.sb_cols && sy < a.sb_rows && a.kmap[sy * a.sb_cols + sx] > r;
+    if (__ballot(keeps) == 0) return;  // whole wave
+  }
   const DPlane cur = a.cur, ref = a.ref[r];
   const BPlane rb = as_buffer(ref), cb = as_buffer(cur);
 

@@ -52,6 +52,9 @@ def lib():
         L.celv2hip_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), I, ctypes.POINTER(LL), I]
         L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.celv2hip_set_ref_limits.argtypes = [P, P, I]
+        L.celv2hip_fetch_sb_costs.argtypes = [P, P, ctypes.c_longlong]
+        L.celv2hip_fetch_sb_costs.restype = ctypes.c_longlong
         L.celv2hip_fetch_intra.argtypes = [P, P, ctypes.c_longlong]
         L.celv2hip_fetch_intra.restype = ctypes.c_longlong
         L.celv2hip_device_alloc.argtypes = [I, LL, ctypes.POINTER(P)]
@@ -269,6 +272,21 @@ class RawEngine:
         if got < 0:
             self._check(int(got), "fetch_intra")
         return out.reshape(self.sb_cols * self.sb_rows, 425, 13)
+
+    def set_ref_limits(self, k=None):
+        """Per-superblock reference limits (uint8, 1..7, row-major) for later launches; None clears."""
+        if k is None:
+            self._check(lib().celv2hip_set_ref_limits(self.h, None, 0), "set_ref_limits")
+            return
+        k = np.ascontiguousarray(k, dtype=np.uint8)
+        self._check(lib().celv2hip_set_ref_limits(self.h, k.ctypes.data, int(k.size)), "set_ref_limits")
+
+    def fetch_sb_costs(self):
+        out = np.zeros(self.sb_cols * self.sb_rows, np.int64)
+        got = lib().celv2hip_fetch_sb_costs(self.h, out.ctypes.data, out.size)
+        if got < 0:
+            self._check(int(got), "fetch_sb_costs")
+        return out
 
     def set_counting(self, on):
         self._check(lib().celv2hip_set_counting(self.h, 1 if on else 0), "set_counting")

@@ -152,6 +152,18 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
  * on != 0: device-resident mode; 0: normal. */
 int celv2hip_set_resident(celv2hip_engine* e, int on);
 
+/* Per-superblock reference limits for later launches (the pruned form): superblock s (row-major) is
+ * decided exactly as by a launch with only its nearest k[s] references (1..7; references beyond
+ * nrefs are absent anyway); the work for the pruned (superblock, reference) pairs is skipped. count =
+ * superblocks of the frame, or k NULL / count 0 to decide every superblock with all references. No
+ * counterpart in the reference's DecisionEngine: the hook of the learned pruner. */
+int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count);
+
+/* The last launch's per-superblock decision cost (the 64x64 square's partition DP cost, SATD*256 +
+ * lambda*bits units: J of the superblock) into out[0 .. superblocks); returns the count or a negative
+ * status. No counterpart in the reference's DecisionEngine (cost regret of the pruned form). */
+long long celv2hip_fetch_sb_costs(celv2hip_engine* e, int64_t* out, long long cap);
+
 /* on != 0: later launches count the non-repeated inter candidates they score (stats counts[4]);
  * off by default, as the count costs the inter kernel extra LDS atomics. */
 int celv2hip_set_counting(celv2hip_engine* e, int on);

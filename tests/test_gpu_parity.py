@@ -123,3 +123,31 @@ def test_batched_launch_matches_single_launches(gpu):
             assert diff is None, (f, sizes[i], diff)
     for e in batch + alone:
         e.close()
+
+
+def test_reference_limits_decide_as_fewer_references(gpu):
+    """celv2hip_set_ref_limits: superblock s decides exactly as a launch with its nearest k[s]
+    references (records and decision cost), for a random limit map and for a constant one."""
+    w, h = 352, 288
+    y8 = synth.clip_frames(synth.params(1, 21, w, h), 9)[1]
+    lam = gpu.lambda_q8(130)
+    f, refs = 8, list(range(7, 0, -1))
+    e = gpu.RawEngine(w, h)
+    maps, costs = {}, {}
+    for k in range(1, 8):
+        e.launch(gpu.RawEngine.luma(y8[f]), [gpu.RawEngine.luma(y8[r]) for r in refs[:k]], lam, keys=refs[:k], src_key=f)
+        maps[k], costs[k] = e.collect(), e.fetch_sb_costs()
+    nsb = e.sb_cols * e.sb_rows
+    rng = np.random.default_rng(5)
+    for kmap in (rng.integers(1, 8, nsb).astype(np.uint8), np.full(nsb, 3, np.uint8), np.full(nsb, 1, np.uint8)):
+        e.set_ref_limits(kmap)
+        e.launch(gpu.RawEngine.luma(y8[f]), [gpu.RawEngine.luma(y8[r]) for r in refs], lam, keys=refs, src_key=f)
+        got, got_cost = e.collect(), e.fetch_sb_costs()
+        want = np.stack([maps[int(kmap[s])][s] for s in range(nsb)])
+        want_cost = np.array([costs[int(kmap[s])][s] for s in range(nsb)])
+        assert compare_maps(got, want, e.sb_cols) is None, (kmap.tolist(), compare_maps(got, want, e.sb_cols))
+        assert np.array_equal(got_cost, want_cost), kmap.tolist()
+    e.set_ref_limits(None)
+    e.launch(gpu.RawEngine.luma(y8[f]), [gpu.RawEngine.luma(y8[r]) for r in refs], lam, keys=refs, src_key=f)
+    assert compare_maps(e.collect(), maps[7], e.sb_cols) is None
+    e.close()
