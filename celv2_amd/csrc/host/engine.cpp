@@ -17,12 +17,14 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../../include/celv2hip.h"
 #include "../kernels/launch.h"
+#include "pruner.h"
 
 namespace celv2hip {
 void upload_intra_tiles(hipStream_t s);
@@ -143,6 +145,12 @@ struct celv2hip_engine {
   uint8_t* kmap = nullptr;       // per-superblock reference limits on the device (when kmap_on)
   bool kmap_on = false;
   int64_t* sb_cost = nullptr;    // per-superblock decision cost of the last launch
+  // The learned pruner (celv2hip_set_pruner) and the per-superblock features it reads.
+  const celv2hip_pruner* pruner = nullptr;
+  float prune_threshold = 0.5f, prune_band = 0.f;
+  bool want_features = false, features_valid = false;
+  uint32_t* cost2[kMaxRefs] = {};  // level-2 best search cost per block and reference
+  CnnWork infer;                   // features (infer.x) and the network's activations, one frame
   void* rec_dev = nullptr;   // celv2_sb per superblock
   void* rec_host = nullptr;  // pinned
   std::vector<int> order;
@@ -185,6 +193,9 @@ struct celv2hip_engine {
         if (f) (void)hipFree(f);
     if (best) (void)hipFree(best);
     if (kmap) (void)hipFree(kmap);
+    for (uint32_t* c : cost2)
+      if (c) (void)hipFree(c);
+    cnn_free(infer);
     if (sb_cost) (void)hipFree(sb_cost);
     if (rec_dev) (void)hipFree(rec_dev);
     if (rec_host) (void)hipHostFree(rec_host);
@@ -404,6 +415,175 @@ long long celv2hip_fetch_sb_costs(celv2hip_engine* e, int64_t* out, long long ca
   return rc ? rc : n;
 }
 
+int celv2hip_pruner_param_count(void) { return kParams; }
+
+int celv2hip_pruner_create(int device, int max_positions, unsigned seed, celv2hip_pruner** out) {
+  if (!out || max_positions <= 0) return CELV2_ERR_ARG;
+  *out = nullptr;
+  return guarded([&] {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw Failure(CELV2_ERR_DEVICE, "pruner: no HIP device");
+    if (device < 0 || device >= count) throw Failure(CELV2_ERR_DEVICE, "pruner: no such HIP device");
+    check(hipSetDevice(device), "hipSetDevice");
+    auto p = std::make_unique<celv2hip_pruner>();
+    p->device = device;
+    check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipMalloc(&p->params, kParams * sizeof(float)), "hipMalloc params");
+    check(hipMalloc(&p->pb, kParams * sizeof(uint16_t)), "hipMalloc params");
+    check(hipMalloc(&p->m1, kParams * sizeof(float)), "hipMalloc adam");
+    check(hipMalloc(&p->m2, kParams * sizeof(float)), "hipMalloc adam");
+    check(hipMalloc(&p->grads, kParams * sizeof(float)), "hipMalloc grads");
+    cnn_alloc(p->work, static_cast<size_t>(max_positions));
+    // He-normal weights (fan-in = the im2col row), zero biases.
+    std::vector<float> w(kParams, 0.f);
+    std::mt19937 rng(seed);
+    auto fill = [&](int off, int fan_in, int count) {
+      std::normal_distribution<float> d(0.f, std::sqrt(2.f / static_cast<float>(fan_in)));
+      for (int k = 0; k < count; k++) w[off + k] = d(rng);
+    };
+    fill(kW1, 9 * kFeatures, 9 * kFeatures * kHidden);
+    fill(kW2, kK2, kK2 * kHidden);
+    fill(kW3, kHidden, kHidden * kOut);
+    check(hipMemcpy(p->params, w.data(), kParams * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy params");
+    check(hipMemset(p->m1, 0, kParams * sizeof(float)), "hipMemset");
+    check(hipMemset(p->m2, 0, kParams * sizeof(float)), "hipMemset");
+    cnn_params_to_bf16(p->params, p->pb, p->stream);
+    check(hipStreamSynchronize(p->stream), "pruner init");
+    *out = p.release();
+  });
+}
+
+int celv2hip_pruner_destroy(celv2hip_pruner* p) {
+  delete p;
+  return CELV2_OK;
+}
+
+int celv2hip_pruner_get_params(celv2hip_pruner* p, float* out, int cap) {
+  if (!p || !out || cap < kParams) return CELV2_ERR_ARG;
+  return guarded([&] {
+    check(hipSetDevice(p->device), "hipSetDevice");
+    check(hipStreamSynchronize(p->stream), "sync");
+    check(hipMemcpy(out, p->params, kParams * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy params");
+  });
+}
+
+int celv2hip_pruner_set_params(celv2hip_pruner* p, const float* in, int n) {
+  if (!p || !in || n != kParams) return CELV2_ERR_ARG;
+  return guarded([&] {
+    check(hipSetDevice(p->device), "hipSetDevice");
+    check(hipStreamSynchronize(p->stream), "sync");
+    check(hipMemcpy(p->params, in, kParams * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy params");
+    check(hipMemset(p->m1, 0, kParams * sizeof(float)), "hipMemset");
+    check(hipMemset(p->m2, 0, kParams * sizeof(float)), "hipMemset");
+    p->step = 0;
+    cnn_params_to_bf16(p->params, p->pb, p->stream);
+    check(hipStreamSynchronize(p->stream), "sync");
+  });
+}
+
+namespace {
+
+// Uploads b frames of h x w superblock features (and labels) into the pruner's workspace.
+void pruner_upload(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w) {
+  if (!p || !feats || b <= 0 || h <= 0 || w <= 0) throw Failure(CELV2_ERR_ARG, "pruner: bad arguments");
+  const size_t m = static_cast<size_t>(b) * h * w;
+  if (m > p->work.cap) throw Failure(CELV2_ERR_SIZE, "pruner: more positions than the workspace holds");
+  check(hipSetDevice(p->device), "hipSetDevice");
+  check(hipMemcpyAsync(p->work.x, feats, m * kFeatures * sizeof(float), hipMemcpyHostToDevice, p->stream), "hipMemcpy feats");
+  if (kstar) check(hipMemcpyAsync(p->work.kstar, kstar, m, hipMemcpyHostToDevice, p->stream), "hipMemcpy labels");
+}
+
+float pruner_loss(celv2hip_pruner* p, size_t m) {
+  const size_t parts = (m * kOut + 255) / 256;
+  std::vector<float> part(parts);
+  check(hipMemcpyAsync(part.data(), p->work.loss_part, parts * sizeof(float), hipMemcpyDeviceToHost, p->stream), "hipMemcpy loss");
+  check(hipStreamSynchronize(p->stream), "sync");
+  double l = 0;
+  for (float v : part) l += v;
+  return static_cast<float>(l / static_cast<double>(m * kOut));
+}
+
+}  // namespace
+
+int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits) {
+  if (!logits) return CELV2_ERR_ARG;
+  return guarded([&] {
+    pruner_upload(p, feats, nullptr, b, h, w);
+    cnn_forward(p->work, p->params, p->pb, b, h, w, p->stream);
+    const size_t m = static_cast<size_t>(b) * h * w;
+    check(hipMemcpyAsync(logits, p->work.z3, m * kOut * sizeof(float), hipMemcpyDeviceToHost, p->stream), "hipMemcpy logits");
+    check(hipStreamSynchronize(p->stream), "sync");
+  });
+}
+
+int celv2hip_pruner_gradients(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w,
+                              float* grads, float* loss) {
+  if (!kstar || !grads) return CELV2_ERR_ARG;
+  return guarded([&] {
+    pruner_upload(p, feats, kstar, b, h, w);
+    cnn_forward(p->work, p->params, p->pb, b, h, w, p->stream);
+    cnn_backward(p->work, p->pb, p->grads, b, h, w, p->stream);
+    check(hipMemcpyAsync(grads, p->grads, kParams * sizeof(float), hipMemcpyDeviceToHost, p->stream), "hipMemcpy grads");
+    const float l = pruner_loss(p, static_cast<size_t>(b) * h * w);
+    if (loss) *loss = l;
+  });
+}
+
+int celv2hip_pruner_train_step(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w,
+                               float lr, float weight_decay, float* loss) {
+  if (!kstar) return CELV2_ERR_ARG;
+  return guarded([&] {
+    pruner_upload(p, feats, kstar, b, h, w);
+    cnn_forward(p->work, p->params, p->pb, b, h, w, p->stream);
+    cnn_backward(p->work, p->pb, p->grads, b, h, w, p->stream);
+    cnn_adamw(p->params, p->grads, p->m1, p->m2, p->pb, ++p->step, lr, weight_decay, p->stream);
+    check(hipGetLastError(), "pruner kernels");
+    const float l = pruner_loss(p, static_cast<size_t>(b) * h * w);
+    if (loss) *loss = l;
+  });
+}
+
+int celv2hip_set_pruner(celv2hip_engine* e, const celv2hip_pruner* p, float threshold, float band) {
+  if (int rc = validate(e)) return rc;
+  if (p && p->device != e->device) {
+    g_error = "pruner and engine on different devices";
+    return CELV2_ERR_ARG;
+  }
+  e->pruner = p, e->prune_threshold = threshold, e->prune_band = band;
+  return CELV2_OK;
+}
+
+int celv2hip_set_features(celv2hip_engine* e, int on) {
+  if (!e) return CELV2_ERR_ARG;
+  e->want_features = on != 0;
+  return CELV2_OK;
+}
+
+long long celv2hip_fetch_features(celv2hip_engine* e, float* out, long long cap) {
+  if (int rc = validate(e)) return rc;
+  long long n = 0;
+  const int rc = guarded([&] {
+    if (!e->features_valid) throw Failure(CELV2_ERR_STATE, "no features: the last launch had none (set_features or a pruner, and references)");
+    n = static_cast<long long>(e->nsb()) * kFeatures;
+    if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "feature buffer too small");
+    e->sync_all();
+    check(hipMemcpy(out, e->infer.x, static_cast<size_t>(n) * sizeof(float), hipMemcpyDeviceToHost), "hipMemcpy features");
+  });
+  return rc ? rc : n;
+}
+
+long long celv2hip_fetch_ref_limits(celv2hip_engine* e, uint8_t* out, long long cap) {
+  if (int rc = validate(e)) return rc;
+  long long n = 0;
+  const int rc = guarded([&] {
+    n = static_cast<long long>(e->nsb());
+    if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "limit buffer too small");
+    e->sync_all();
+    check(hipMemcpy(out, e->kmap, static_cast<size_t>(n), hipMemcpyDeviceToHost), "hipMemcpy kmap");
+  });
+  return rc ? rc : n;
+}
+
 int celv2hip_set_counting(celv2hip_engine* e, int on) {
   if (!e) return CELV2_ERR_ARG;
   e->counting = on != 0;
@@ -504,6 +684,8 @@ void launch_batch(const LaunchIn* in, int n) {
   std::vector<std::pair<celv2hip_engine*, SrcSlot*>> intra_now;
   std::vector<SrcSlot*> cur(n);
   std::vector<std::vector<RefSlot*>> used(n);
+  std::vector<char> use_kmap(n);
+  for (int i = 0; i < n; i++) use_kmap[i] = in[i].e->kmap_on;
   for (int i = 0; i < n; i++) {
     const LaunchIn& x = in[i];
     celv2hip_engine* e = x.e;
@@ -564,8 +746,11 @@ void launch_batch(const LaunchIn* in, int n) {
       a.pcols = l + 1 < kLevels ? e->fcols[l + 1] : 0, a.prows = l + 1 < kLevels ? e->frows[l + 1] : 0;
       a.level0 = l == 0, a.phase_bytes = used[i][0]->p.phase_bytes;
       a.searches = e->counters + 16;
-      a.kmap = e->kmap_on ? e->kmap : nullptr, a.sb_cols = e->sb_cols, a.sb_rows = e->sb_rows, a.level = l;
+      a.kmap = use_kmap[i] ? e->kmap : nullptr, a.sb_cols = e->sb_cols, a.sb_rows = e->sb_rows, a.level = l;
+      const bool feats = l == 2 && (e->pruner || e->want_features);
       for (int r = 0; r < nr; r++) {
+        if (feats && !e->cost2[r]) check(hipMalloc(&e->cost2[r], static_cast<size_t>(e->fcols[2]) * e->frows[2] * sizeof(uint32_t)), "hipMalloc cost2");
+        a.cost_out[r] = feats ? e->cost2[r] : nullptr;
         if (!e->fields[r][l]) check(hipMalloc(&e->fields[r][l], static_cast<size_t>(e->fcols[l]) * e->frows[l] * sizeof(uint32_t)), "hipMalloc field");
         a.ref[r] = used[i][r]->p.lv[l];
         a.coarser[r] = l + 1 < kLevels ? e->fields[r][l + 1] : nullptr;
@@ -576,6 +761,30 @@ void launch_batch(const LaunchIn* in, int n) {
     }
     launch_me_level(b, S);
     check(hipGetLastError(), "me kernel");
+    if (l == 2) {
+      // The pruner: features from the coarse levels (searched in full), the network, then the
+      // per-superblock reference limits that levels 1 and 0, inter scores and finalize obey.
+      for (int i = 0; i < n; i++) {
+        celv2hip_engine* e = in[i].e;
+        e->features_valid = false;
+        if (!(e->pruner || e->want_features) || in[i].nrefs < 1) continue;
+        if (!e->infer.cap) cnn_alloc(e->infer, e->nsb());
+        FeatureArgs fa{};
+        fa.src2 = cur[i]->p.lv[2];
+        for (int r = 0; r < in[i].nrefs; r++) fa.cost2[r] = e->cost2[r];
+        fa.nrefs = in[i].nrefs, fa.cols2 = e->fcols[2], fa.rows2 = e->frows[2], fa.sb_cols = e->sb_cols, fa.sb_rows = e->sb_rows;
+        fa.feats = e->infer.x;
+        launch_features(fa, S);
+        e->features_valid = true;
+        if (e->pruner) {
+          cnn_forward(e->infer, e->pruner->params, e->pruner->pb, 1, e->sb_rows, e->sb_cols, S);
+          cnn_limits(e->infer.z3, static_cast<int>(e->nsb()), in[i].nrefs, e->prune_threshold, e->prune_band, e->kmap,
+                     e->counters + 3, S);
+          use_kmap[i] = true;
+        }
+        check(hipGetLastError(), "pruner kernels");
+      }
+    }
   }
   for (int i = 0; i < n; i++) check(hipEventRecord(in[i].e->ev[2], S), "hipEventRecord");
 
@@ -587,7 +796,7 @@ void launch_batch(const LaunchIn* in, int n) {
     ia.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
     ia.best = e->best;
     ia.counter = e->counters, ia.candidates = e->counting ? e->counters + 2 : nullptr;
-    ia.kmap = e->kmap_on ? e->kmap : nullptr;
+    ia.kmap = use_kmap[i] ? e->kmap : nullptr;
     for (int r = 0; r < in[i].nrefs; r++) ia.ref[r] = used[i][r]->p.lv[0], ia.field[r] = e->fields[r][0];
     return ia;
   };
@@ -598,7 +807,7 @@ void launch_batch(const LaunchIn* in, int n) {
     fa.nrefs = in[i].nrefs, fa.sb_cols = e->sb_cols, fa.lam = in[i].lam;
     fa.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
     fa.best = e->best, fa.intra = cur[i]->intra, fa.out = e->rec_dev, fa.counter = e->counters + 1;
-    fa.kmap = e->kmap_on ? e->kmap : nullptr, fa.sb_cost = e->sb_cost;
+    fa.kmap = use_kmap[i] ? e->kmap : nullptr, fa.sb_cost = e->sb_cost;
     for (int r = 0; r < in[i].nrefs; r++) fa.ref[r] = used[i][r]->p.lv[0];
     return fa;
   };
@@ -826,9 +1035,10 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
     check(hipMemcpy(e->counters_host, e->counters, 80 * 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "hipMemcpy counters");
     unsigned long long searches = 0;
     for (int i = 0; i < 64; i++) searches += e->counters_host[16 + 16 * i];
-    const long long v[5] = {n, static_cast<long long>(e->counters_host[0]), static_cast<long long>(searches),
-                            static_cast<long long>(e->counters_host[1]), static_cast<long long>(e->counters_host[2])};
-    for (int k = 0; k < 5 && k < cap_counts; k++) counts[k] = v[k];
+    const long long v[6] = {n, static_cast<long long>(e->counters_host[0]), static_cast<long long>(searches),
+                            static_cast<long long>(e->counters_host[1]), static_cast<long long>(e->counters_host[2]),
+                            static_cast<long long>(e->counters_host[3])};
+    for (int k = 0; k < 6 && k < cap_counts; k++) counts[k] = v[k];
   });
   return rc ? -rc : static_cast<int>(n);
 }

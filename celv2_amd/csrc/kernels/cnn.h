@@ -1,0 +1,53 @@
+// The learned pruner's network (cnn.hip): sizes, the flat FP32 parameter layout and the device
+// workspace of a forward / backward pass.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <stdexcept>
+
+namespace celv2hip {
+
+constexpr int kFeatures = 10;       // inputs per superblock (pruner.hip)
+constexpr int kHidden = 32;         // channels of the two 3x3 layers
+constexpr int kOut = 6;             // output j: needs more than j + 1 references
+constexpr int kK1 = 96;             // im2col row of layer 1: 9 * kFeatures = 90, padded
+constexpr int kK2 = 9 * kHidden;    // im2col row of layer 2
+// Parameters: W1[9 kFeatures][kHidden], b1, W2[kK2][kHidden], b2, W3[kHidden][kOut], b3 (row k of a
+// weight matrix is im2col column k: k = tap * channels + channel, tap = (dy + 1) * 3 + dx + 1).
+constexpr int kW1 = 0, kB1 = kW1 + 9 * kFeatures * kHidden, kW2 = kB1 + kHidden, kB2 = kW2 + kK2 * kHidden,
+              kW3 = kB2 + kHidden, kB3 = kW3 + kHidden * kOut, kParams = kB3 + kOut;
+
+struct CnnWork {
+  float* x = nullptr;  // [positions][kFeatures]
+  uint16_t* col1 = nullptr;
+  float* z1 = nullptr;
+  uint16_t* col2 = nullptr;
+  float* z2 = nullptr;
+  uint16_t* z2b = nullptr;
+  float* z3 = nullptr;  // logits [positions][kOut]
+  float* dz3 = nullptr;
+  uint16_t* dz3b = nullptr;
+  float* dz2 = nullptr;
+  uint16_t* dz2b = nullptr;
+  float* dcol2 = nullptr;
+  float* dz1 = nullptr;
+  uint16_t* dz1b = nullptr;
+  float* loss_part = nullptr;
+  uint8_t* kstar = nullptr;  // labels: the references a superblock needs (1..7)
+  size_t cap = 0;
+  size_t positions_capacity() const;
+};
+
+void cnn_alloc(CnnWork& w, size_t positions);
+void cnn_free(CnnWork& w);
+void cnn_params_to_bf16(const float* params, uint16_t* pb, hipStream_t s);
+void cnn_forward(const CnnWork& w, const float* params, const uint16_t* pb, int b, int h, int wd, hipStream_t s);
+void cnn_backward(const CnnWork& w, const uint16_t* pb, float* grads, int b, int h, int wd, hipStream_t s);
+void cnn_adamw(float* params, const float* grads, float* m1, float* m2, uint16_t* pb, long long step, float lr, float wd,
+               hipStream_t s);
+void cnn_limits(const float* logits, int m, int nrefs, float threshold, float band, uint8_t* kmap,
+                unsigned long long* fallbacks, hipStream_t s);
+
+}  // namespace celv2hip

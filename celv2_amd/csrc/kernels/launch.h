@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "batch.h"
+#include "cnn.h"
 #include "common.h"
 
 namespace celv2hip {
@@ -44,6 +45,7 @@ struct MeLevelArgs {
   // superblock of it keeps r (every field cell a kept superblock reads, and their centres' cells).
   const uint8_t* kmap;
   int sb_cols, sb_rows, level;
+  uint32_t* cost_out[kMaxRefs];  // the searched best cost (SAD + distance penalty) per block (may be null)
 };
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a);
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s);
@@ -91,6 +93,16 @@ struct FinalizeArgs {
 };
 void add_finalize(Batch<FinalizeArgs>& b, const FinalizeArgs& a);
 void launch_finalize(const Batch<FinalizeArgs>& b, hipStream_t s);
+
+// pruner.hip: the learned pruner's per-superblock inputs, from the source's level 2 and each
+// reference's level-2 search costs: feats[sb * kFeatures + f].
+struct FeatureArgs {
+  DPlane src2;                      // the source's level 2
+  const uint32_t* cost2[kMaxRefs];  // level-2 best cost per block, cols2 x rows2
+  int nrefs, cols2, rows2, sb_cols, sb_rows;
+  float* feats;
+};
+void launch_features(const FeatureArgs& a, hipStream_t s);
 
 // Static tables the kernels read (built once per process on the host, copied to the device).
 void upload_tables(hipStream_t s);

@@ -173,12 +173,15 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
     idx[k] = static_cast<uint32_t>((dy0 + k) * 17 + dxi);
   }
   const uint32_t kill5 = g3 == 2 ? 0xffffffffu : 0u;
+  uint32_t best_cost;
   const uint32_t k1 = wave_min_u32(act ? score_window(win1, corner_offset(rb, Y1, X1) & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
   const int i1 = static_cast<int>(k1 & 511);
   int mvx = p1x + i1 % 17 - kRange, mvy = p1y + i1 / 17 - kRange;
+  best_cost = k1 >> 9;
   if (two) {
     const uint32_t k2 = wave_min_u32(act ? score_window(win2, corner_offset(rb, Y2, X2) & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
     if ((k2 >> 9) < (k1 >> 9)) {
+      best_cost = k2 >> 9;
       const int i2 = static_cast<int>(k2 & 511);
       mvx = p2x + i2 % 17 - kRange, mvy = p2y + i2 / 17 - kRange;
     }
@@ -229,6 +232,7 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
     mvx = qcx, mvy = qcy;
   }
   if (lane == 0) {
+    if (a.cost_out[r]) a.cost_out[r][by * a.cols + bx] = best_cost;
     a.out[r][by * a.cols + bx] = pack_mv(mvx, mvy);
     if (a.searches) atomicAdd(a.searches + 16 * (item & 63), two ? 2ull : 1ull);
   }

@@ -53,6 +53,19 @@ def lib():
         L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_set_ref_limits.argtypes = [P, P, I]
+        L.celv2hip_pruner_create.argtypes = [I, I, ctypes.c_uint, ctypes.POINTER(P)]
+        L.celv2hip_pruner_destroy.argtypes = [P]
+        L.celv2hip_pruner_get_params.argtypes = [P, P, I]
+        L.celv2hip_pruner_set_params.argtypes = [P, P, I]
+        L.celv2hip_pruner_forward.argtypes = [P, P, I, I, I, P]
+        L.celv2hip_pruner_gradients.argtypes = [P, P, P, I, I, I, P, ctypes.POINTER(ctypes.c_float)]
+        L.celv2hip_pruner_train_step.argtypes = [P, P, P, I, I, I, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
+        L.celv2hip_set_pruner.argtypes = [P, P, ctypes.c_float, ctypes.c_float]
+        L.celv2hip_set_features.argtypes = [P, I]
+        L.celv2hip_fetch_features.argtypes = [P, P, ctypes.c_longlong]
+        L.celv2hip_fetch_features.restype = ctypes.c_longlong
+        L.celv2hip_fetch_ref_limits.argtypes = [P, P, ctypes.c_longlong]
+        L.celv2hip_fetch_ref_limits.restype = ctypes.c_longlong
         L.celv2hip_fetch_sb_costs.argtypes = [P, P, ctypes.c_longlong]
         L.celv2hip_fetch_sb_costs.restype = ctypes.c_longlong
         L.celv2hip_fetch_intra.argtypes = [P, P, ctypes.c_longlong]
@@ -288,6 +301,26 @@ class RawEngine:
             self._check(int(got), "fetch_sb_costs")
         return out
 
+    def set_pruner(self, pruner, threshold=0.5, band=0.0):
+        self._check(lib().celv2hip_set_pruner(self.h, pruner.h if pruner is not None else None, threshold, band), "set_pruner")
+
+    def set_features(self, on):
+        self._check(lib().celv2hip_set_features(self.h, 1 if on else 0), "set_features")
+
+    def fetch_features(self):
+        out = np.zeros((self.sb_rows, self.sb_cols, 10), np.float32)
+        got = lib().celv2hip_fetch_features(self.h, out.ctypes.data, out.size)
+        if got < 0:
+            self._check(int(got), "fetch_features")
+        return out
+
+    def fetch_ref_limits(self):
+        out = np.zeros(self.sb_cols * self.sb_rows, np.uint8)
+        got = lib().celv2hip_fetch_ref_limits(self.h, out.ctypes.data, out.size)
+        if got < 0:
+            self._check(int(got), "fetch_ref_limits")
+        return out
+
     def set_counting(self, on):
         self._check(lib().celv2hip_set_counting(self.h, 1 if on else 0), "set_counting")
 
@@ -297,13 +330,69 @@ class RawEngine:
     def stats(self):
         """(stage ms sums over the window, counts) - see celv2hip_stats."""
         ms = (ctypes.c_double * 7)()
-        cnt = (LL * 5)()
-        n = lib().celv2hip_stats(self.h, ms, 7, cnt, 5)
+        cnt = (LL * 6)()
+        n = lib().celv2hip_stats(self.h, ms, 7, cnt, 6)
         if n < 0:
             raise EngineError(last_error())
         keys = ["build", "me", "inter", "finalize", "copy", "total", "intra_prepare"]
-        ckeys = ["launches", "tile_evals", "me_centres", "compound_blocks", "inter_candidates"]
+        ckeys = ["launches", "tile_evals", "me_centres", "compound_blocks", "inter_candidates", "pruner_fallbacks"]
         return dict(zip(keys, list(ms))), dict(zip(ckeys, list(cnt)))
+
+
+class Pruner:
+    """The learned pruner's network (celv2hip_pruner_*): features [b, h, w, 10] float32."""
+
+    def __init__(self, max_positions=16384, seed=1, device=0):
+        p = P()
+        if lib().celv2hip_pruner_create(device, max_positions, seed, ctypes.byref(p)) != 0:
+            raise EngineError(last_error())
+        self.h = p.value
+        self.nparams = lib().celv2hip_pruner_param_count()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().celv2hip_pruner_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def params(self):
+        out = np.zeros(self.nparams, np.float32)
+        if lib().celv2hip_pruner_get_params(self.h, out.ctypes.data, out.size) != 0:
+            raise EngineError(last_error())
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if lib().celv2hip_pruner_set_params(self.h, w.ctypes.data, int(w.size)) != 0:
+            raise EngineError(last_error())
+
+    def forward(self, feats):
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        b, h, w, _ = feats.shape
+        out = np.zeros((b, h, w, 6), np.float32)
+        if lib().celv2hip_pruner_forward(self.h, feats.ctypes.data, b, h, w, out.ctypes.data) != 0:
+            raise EngineError(last_error())
+        return out
+
+    def gradients(self, feats, kstar):
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        kstar = np.ascontiguousarray(kstar, dtype=np.uint8)
+        b, h, w, _ = feats.shape
+        g = np.zeros(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        if lib().celv2hip_pruner_gradients(self.h, feats.ctypes.data, kstar.ctypes.data, b, h, w, g.ctypes.data, ctypes.byref(loss)) != 0:
+            raise EngineError(last_error())
+        return g, loss.value
+
+    def train_step(self, feats, kstar, lr, weight_decay):
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        kstar = np.ascontiguousarray(kstar, dtype=np.uint8)
+        b, h, w, _ = feats.shape
+        loss = ctypes.c_float()
+        if lib().celv2hip_pruner_train_step(self.h, feats.ctypes.data, kstar.ctypes.data, b, h, w, lr, weight_decay, ctypes.byref(loss)) != 0:
+            raise EngineError(last_error())
+        return loss.value
 
 
 class DeviceBuffer:

@@ -142,7 +142,8 @@ int celv2hip_sync(celv2hip_engine* e);
  * [5] launch start to end, [6] intra scores queued by prepare(). counts: [0] launches, [1] distinct
  * (MV, reference, superblock) tile evaluations, [2] motion-search centres searched, [3] compound
  * blocks scored, [4] non-repeated (block, MV, reference) inter candidates scored (while counting is on,
- * celv2hip_set_counting). Returns the number of launches (negative status on failure). */
+ * celv2hip_set_counting), [5] superblocks the pruner sent to its fallback. Returns the number of
+ * launches (negative status on failure). */
 int celv2hip_stats_reset(celv2hip_engine* e);
 int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts, int cap_counts);
 
@@ -163,6 +164,44 @@ int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count);
  * lambda*bits units: J of the superblock) into out[0 .. superblocks); returns the count or a negative
  * status. No counterpart in the reference's DecisionEngine (cost regret of the pruned form). */
 long long celv2hip_fetch_sb_costs(celv2hip_engine* e, int64_t* out, long long cap);
+
+/* ---- The learned pruner (the pruned form; no counterpart in the reference's DecisionEngine) ------
+ * A small network over a frame's superblock grid: inputs are 10 features per superblock from the
+ * source's level 2 and each reference's level-2 motion search; output j (0..5) is the logit of "the
+ * superblock needs more than its nearest j + 1 references". BF16 operands on the matrix cores, FP32
+ * accumulation and master weights, AdamW. Attached to an engine, every launch computes the features
+ * after motion-search level 2, runs the network and sets each superblock's reference limit (as
+ * celv2hip_set_ref_limits) before levels 1 and 0, inter scores and finalize. A pruner may be
+ * attached to several engines; it must not be trained while their launches run. */
+typedef struct celv2hip_pruner celv2hip_pruner;
+
+/* Number of FP32 parameters of the network. */
+int celv2hip_pruner_param_count(void);
+/* A pruner on `device` with a workspace for max_positions superblocks, He-initialised from seed. */
+int celv2hip_pruner_create(int device, int max_positions, unsigned seed, celv2hip_pruner** out);
+int celv2hip_pruner_destroy(celv2hip_pruner* p);
+int celv2hip_pruner_get_params(celv2hip_pruner* p, float* out, int cap);
+/* Loads parameters (n = param count) and resets the AdamW state. */
+int celv2hip_pruner_set_params(celv2hip_pruner* p, const float* in, int n);
+/* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 10 + i]: the 6 logits per superblock. */
+int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
+/* Mean binary cross-entropy against labels kstar (the references each superblock needs, 1..7) and its
+ * gradient with respect to every parameter, without updating them. */
+int celv2hip_pruner_gradients(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w,
+                              float* grads, float* loss);
+/* One AdamW step (beta 0.9 / 0.999, eps 1e-8) on that loss; *loss is the loss before the step. */
+int celv2hip_pruner_train_step(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w,
+                               float lr, float weight_decay, float* loss);
+/* Attaches p (NULL detaches). A superblock keeps 1 + the number of leading outputs whose probability
+ * is >= threshold references (at most nrefs); if any output lies within `band` of the threshold the
+ * superblock falls back to all references (counted in celv2hip_stats counts[5]). */
+int celv2hip_set_pruner(celv2hip_engine* e, const celv2hip_pruner* p, float threshold, float band);
+/* on != 0: launches compute the pruner's features even without a pruner (training data). */
+int celv2hip_set_features(celv2hip_engine* e, int on);
+/* The last launch's features (superblocks x 10 floats); returns the count or a negative status. */
+long long celv2hip_fetch_features(celv2hip_engine* e, float* out, long long cap);
+/* The reference limits the last pruned launch used (one byte per superblock). */
+long long celv2hip_fetch_ref_limits(celv2hip_engine* e, uint8_t* out, long long cap);
 
 /* on != 0: later launches count the non-repeated inter candidates they score (stats counts[4]);
  * off by default, as the count costs the inter kernel extra LDS atomics. */
