@@ -26,7 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from celv2_amd import engine, synth  # noqa: E402
+from celv2_amd import engine, pruner, synth  # noqa: E402
 from celv2_amd.records import SB_DTYPE, compare_maps  # noqa: E402
 
 METRIC = "1080p10 frames/s on 1 GPU: plain HIP vs learned-pruned at ≤0.5% BD-rate loss"
@@ -150,6 +150,72 @@ def bench_batched(NR, lam, steps=20, warmup=3, nf=16):
             "speedup": round(rates[best][0] / rates["plain"][0], 3), "decisions_equal_plain": same}
 
 
+def bench_learned(lum, decidable, NR, lam, steps, warmup, plain_fps):
+    """The learned-pruned form on the same workload: the trained network (celv2_amd/pruner) runs inside
+    every launch on the engine's coarse search and limits each superblock's references; frames/s over
+    the same steps, then, untimed, the decision-cost regret against the plain engine on every
+    decidable frame (sum of J_pruned - J_plain over sum of J_plain), the mean reference limit and the
+    fallback rate."""
+    w, cfg = pruner.load()
+    P = engine.Pruner(max_positions=(1920 // 64 + 1) * (1080 // 64 + 1), seed=1)
+    P.set_params(w)
+    E = engine.RawEngine(1920, 1080)
+    E.set_resident(True)
+    E.set_pruner(P, cfg["threshold"], cfg["band"])
+
+    def step(s):
+        i = decidable[s % len(decidable)]
+        nxt = decidable[(s + 1) % len(decidable)]
+        refs = [i - k for k in range(1, NR + 1)]
+        E.launch(lum[i], [lum[r] for r in refs], lam, keys=refs, src_key=i)
+        E.prepare(lum[nxt], nxt)
+
+    E.prepare(lum[decidable[0]], decidable[0])
+    for s in range(warmup):
+        step(s)
+    E.stats_reset()
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        step(s)
+    E.sync()
+    wall = time.perf_counter() - t0
+    ms, cnt = E.stats()
+    fps = steps / wall
+    n = max(1, cnt["launches"])
+    per = {k: round(v / n, 4) for k, v in ms.items() if k != "intra_prepare"}
+    per["intra"] = round(ms["intra_prepare"] / n, 4)
+    # Untimed: inter candidates counted, regret, limits and fallbacks on every decidable frame.
+    E.set_counting(True)
+    E.stats_reset()
+    Plain = engine.RawEngine(1920, 1080)
+    Plain.set_resident(True)
+    jp, jl, ks = [], [], []
+    for i in decidable:
+        refs = [i - k for k in range(1, NR + 1)]
+        for eng, out in ((Plain, jp), (E, jl)):
+            eng.launch(lum[i], [lum[r] for r in refs], lam, keys=refs)
+            eng.sync()
+            out.append(eng.fetch_sb_costs())
+        ks.append(E.fetch_ref_limits())
+    _, c2 = E.stats()
+    Plain.close()
+    E.close()
+    P.close()
+    jp, jl, ks = (np.stack(a).astype(np.float64) for a in (jp, jl, ks))
+    regret = float((jl - jp).sum() / jp.sum())
+    nd = len(decidable)
+    inter_l = c2["inter_candidates"] / nd
+    log(f"learned: {fps:.1f} frames/s ({fps / plain_fps:.2f}x plain), regret {regret * 100:.3f}% of J, mean limit {ks.mean():.2f}, "
+        f"fallbacks {c2['pruner_fallbacks'] / ks.size:.4f}; per frame ms {json.dumps(per)}")
+    return {"frames_per_s": round(fps, 2), "speedup_vs_plain": round(fps / plain_fps, 3),
+            "cost_regret": round(regret, 6), "cost_regret_unit": "sum(J_learned - J_plain) / sum(J_plain) over every decidable frame (engine J = SATD*256 + lambda*bits)",
+            "bd_rate": "pending (needs the full encoder and AVM)",
+            "mean_reference_limit": round(float(ks.mean()), 3), "fallback_rate": round(c2["pruner_fallbacks"] / ks.size, 5),
+            "threshold": cfg["threshold"], "band": cfg["band"], "stage_ms_per_frame": per,
+            "inter_candidates_per_frame": round(inter_l), "compound_blocks_per_frame": round(c2["compound_blocks"] / nd),
+            "network": "conv3x3 10-32, conv3x3 32-32, conv1x1 32-6, BF16 on MFMA, trained on 40 synthetic clips (celv2_amd/pruner/train.py)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +227,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-resolution batched form")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned form")
     ap.add_argument("--e2e-steps", type=int, default=12)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
@@ -276,6 +343,11 @@ def main():
         HE.close()
 
     batched = None if args.no_batched else bench_batched(NR, lam)
+    learned = None
+    if not args.no_learned and NR == 7:
+        learned = bench_learned(lum, decidable, NR, lam, args.steps, args.warmup, fps)
+        learned["scored_candidates_per_frame"] = round(sbs * 420 * 13 + learned["inter_candidates_per_frame"] + learned["compound_blocks_per_frame"])
+        learned["scored_candidates_per_s"] = round(learned["scored_candidates_per_frame"] * learned["frames_per_s"])
 
     cpu = None
     parity = None
@@ -327,7 +399,7 @@ def main():
         "all_kernels_tops": round(total_ops * fps / 1e12, 3),
         "engine_path_fps_host_buffers": e2e,
         "forms": {"plain": {"frames_per_s": round(fps, 2), "scored_candidates_per_s": round(cands * fps)},
-                  "batched": batched},
+                  "batched": batched, "learned": learned},
     }
     print(json.dumps(out), flush=True)
 

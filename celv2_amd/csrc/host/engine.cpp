@@ -509,7 +509,7 @@ int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h
   if (!logits) return CELV2_ERR_ARG;
   return guarded([&] {
     pruner_upload(p, feats, nullptr, b, h, w);
-    cnn_forward(p->work, p->params, p->pb, b, h, w, p->stream);
+    cnn_infer(p->work.x, b, h, w, p->params, p->pb, p->work.z3, kMaxRefs, 0.5f, 0.f, nullptr, nullptr, p->stream);
     const size_t m = static_cast<size_t>(b) * h * w;
     check(hipMemcpyAsync(logits, p->work.z3, m * kOut * sizeof(float), hipMemcpyDeviceToHost, p->stream), "hipMemcpy logits");
     check(hipStreamSynchronize(p->stream), "sync");
@@ -777,9 +777,8 @@ void launch_batch(const LaunchIn* in, int n) {
         launch_features(fa, S);
         e->features_valid = true;
         if (e->pruner) {
-          cnn_forward(e->infer, e->pruner->params, e->pruner->pb, 1, e->sb_rows, e->sb_cols, S);
-          cnn_limits(e->infer.z3, static_cast<int>(e->nsb()), in[i].nrefs, e->prune_threshold, e->prune_band, e->kmap,
-                     e->counters + 3, S);
+          cnn_infer(e->infer.x, 1, e->sb_rows, e->sb_cols, e->pruner->params, e->pruner->pb, nullptr, in[i].nrefs,
+                    e->prune_threshold, e->prune_band, e->kmap, e->counters + 3, S);
           use_kmap[i] = true;
         }
         check(hipGetLastError(), "pruner kernels");

@@ -49,5 +49,9 @@ void cnn_adamw(float* params, const float* grads, float* m1, float* m2, uint16_t
                hipStream_t s);
 void cnn_limits(const float* logits, int m, int nrefs, float threshold, float band, uint8_t* kmap,
                 unsigned long long* fallbacks, hipStream_t s);
+// Inference in one launch over x (b frames of h x wd superblocks): logits (if non-null, as cnn_forward
+// leaves them in w.z3, bit for bit) and reference limits into kmap (if non-null, as cnn_limits).
+void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
+               float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s);
 
 }  // namespace celv2hip

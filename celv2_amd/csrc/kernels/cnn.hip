@@ -68,6 +68,135 @@ __global__ void __launch_bounds__(256) k_gemm(Gemm g) {
   }
 }
 
+// Inference in one launch: a workgroup takes a 4 x 16 tile of one frame's grid. Layer 1 runs on the
+// tile plus a ring of 1 (6 x 18 positions) from the features staged as BF16 with a ring of 2, layer 2
+// and 3 on the tile, all through LDS. Each output element is the same chain of
+// v_mfma_f32_16x16x16bf16_1k over the same BF16 operands, K-chunk by K-chunk, as k_gemm over the
+// im2col buffers, so the logits are those of the layer-by-layer pass bit for bit. Then each
+// position's reference limit (as k_limits).
+constexpr int kTileH = 4, kTileW = 16;
+constexpr int kZ1H = kTileH + 2, kZ1W = kTileW + 2, kZ1 = kZ1H * kZ1W;  // 108 positions
+constexpr int kXH = kTileH + 4, kXW = kTileW + 4;
+
+__device__ __forceinline__ int limit_of(const float* z, int nrefs, float threshold, float band, bool& unsure) {
+  int k = 1;
+  bool open = true;
+  unsure = false;
+  for (int j = 0; j < kOut; j++) {
+    const float s = 1.f / (1.f + expf(-z[j]));
+    if (fabsf(s - threshold) < band) unsure = true;
+    if (open && s >= threshold) k = j + 2;
+    else open = false;
+  }
+  if (unsure) k = kMaxRefs;
+  return k < nrefs ? k : (nrefs < 1 ? 1 : nrefs);
+}
+
+struct Infer {
+  const float* x;  // [b][h][w][kFeatures]
+  int h, w;
+  const uint16_t* pb;
+  const float* params;
+  float* logits;  // [b][h][w][kOut] (may be null)
+  uint8_t* kmap;  // [b][h][w] (may be null)
+  unsigned long long* fallbacks;
+  int nrefs;
+  float threshold, band;
+};
+
+__global__ void __launch_bounds__(256) k_infer(Infer a) {
+  __shared__ uint16_t xs[kXH * kXW * kFeatures];
+  __shared__ uint16_t z1s[kZ1 * kHidden];
+  __shared__ uint16_t z2s[kTileH * kTileW * kHidden];
+  __shared__ float z3s[kTileH * kTileW * 8];
+  const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, i = lane & 15, q = lane >> 4;
+  const int tx0 = static_cast<int>(blockIdx.x) * kTileW, ty0 = static_cast<int>(blockIdx.y) * kTileH;
+  const size_t frame = static_cast<size_t>(blockIdx.z) * a.h * a.w;
+  for (int e = tid; e < kXH * kXW * kFeatures; e += 256) {
+    const int c = e % kFeatures, pos = e / kFeatures, yy = ty0 - 2 + pos / kXW, xx = tx0 - 2 + pos % kXW;
+    const bool in = yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+    xs[e] = bf16_bits(in ? a.x[(frame + static_cast<size_t>(yy) * a.w + xx) * kFeatures + c] : 0.f);
+  }
+  __syncthreads();
+  // Layer 1 on the 6 x 18 region (7 row tiles x 2 column tiles); 0 outside the grid (layer 2's padding).
+  for (int t = wave; t < 14; t += 4) {
+    const int tm = t >> 1, tn = t & 1, m = 16 * tm + i, col = 16 * tn + i;
+    const int ry = m / kZ1W, rx = m - ry * kZ1W;
+    f32x4v acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < 9 * kFeatures; k0 += 16) {
+      v4i16 av, bv;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int kk = k0 + 4 * q + j, tap = kk / kFeatures, c = kk - tap * kFeatures, dy = tap / 3 - 1, dx = tap % 3 - 1;
+        const bool ok = kk < 9 * kFeatures;
+        av[j] = static_cast<short>(ok && m < kZ1 ? xs[((ry + 1 + dy) * kXW + rx + 1 + dx) * kFeatures + c] : 0);
+        bv[j] = static_cast<short>(ok ? a.pb[kW1 + kk * kHidden + col] : 0);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int mm = 16 * tm + 4 * q + r;
+      if (mm >= kZ1) continue;
+      const int yy = ty0 - 1 + mm / kZ1W, xx = tx0 - 1 + mm % kZ1W;
+      const bool in = yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
+      z1s[mm * kHidden + col] = in ? bf16_bits(fmaxf(acc[r] + a.params[kB1 + col], 0.f)) : 0;
+    }
+  }
+  __syncthreads();
+  // Layer 2: row tile tm = tile row, its 16 rows = the tile's columns.
+  for (int t = wave; t < 2 * kTileH; t += 4) {
+    const int tm = t >> 1, tn = t & 1, col = 16 * tn + i;
+    f32x4v acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < 9 * kHidden; k0 += 16) {
+      v4i16 av, bv;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int kk = k0 + 4 * q + j, tap = kk / kHidden, c = kk - tap * kHidden, dy = tap / 3 - 1, dx = tap % 3 - 1;
+        av[j] = static_cast<short>(z1s[((tm + 1 + dy) * kZ1W + i + 1 + dx) * kHidden + c]);
+        bv[j] = static_cast<short>(a.pb[kW2 + kk * kHidden + col]);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) z2s[(tm * kTileW + 4 * q + r) * kHidden + col] = bf16_bits(fmaxf(acc[r] + a.params[kB2 + col], 0.f));
+  }
+  __syncthreads();
+  // Layer 3 (1x1, 6 outputs): one row tile per wave.
+  {
+    const int tm = wave;
+    f32x4v acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < kHidden; k0 += 16) {
+      v4i16 av, bv;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int kk = k0 + 4 * q + j;
+        av[j] = static_cast<short>(z2s[(tm * kTileW + i) * kHidden + kk]);
+        bv[j] = static_cast<short>(i < kOut ? a.pb[kW3 + kk * kOut + i] : 0);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+    if (i < kOut)
+#pragma unroll
+      for (int r = 0; r < 4; r++) z3s[(tm * kTileW + 4 * q + r) * 8 + i] = acc[r] + a.params[kB3 + i];
+  }
+  __syncthreads();
+  if (tid < kTileH * kTileW) {
+    const int yy = ty0 + tid / kTileW, xx = tx0 + tid % kTileW;
+    if (yy < a.h && xx < a.w) {
+      const size_t p = frame + static_cast<size_t>(yy) * a.w + xx;
+      const float* z = z3s + tid * 8;
+      if (a.logits)
+        for (int j = 0; j < kOut; j++) a.logits[p * kOut + j] = z[j];
+      if (a.kmap) {
+        bool unsure;
+        a.kmap[p] = static_cast<uint8_t>(limit_of(z, a.nrefs, a.threshold, a.band, unsure));
+        if (unsure && a.fallbacks) atomicAdd(a.fallbacks, 1ull);
+      }
+    }
+  }
+}
+
 // col[p][k] (k < kp) from x[p][c] (f32, c channels), 3x3 taps around p on each frame's grid.
 __global__ void k_im2col(const float* x, int b, int h, int w, int c, int kp, uint16_t* col) {
   const long long idx = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
@@ -179,17 +308,9 @@ __global__ void k_limits(const float* z, int m, int nrefs, float threshold, floa
                          unsigned long long* fallbacks) {
   const int p = static_cast<int>(blockIdx.x) * 256 + threadIdx.x;
   if (p >= m) return;
-  int k = 1;
-  bool open = true, unsure = false;
-  for (int j = 0; j < kOut; j++) {
-    const float s = 1.f / (1.f + expf(-z[static_cast<size_t>(p) * kOut + j]));
-    if (fabsf(s - threshold) < band) unsure = true;
-    if (open && s >= threshold) k = j + 2;
-    else open = false;
-  }
-  if (unsure) k = kMaxRefs;
+  bool unsure;
+  kmap[p] = static_cast<uint8_t>(limit_of(z + static_cast<size_t>(p) * kOut, nrefs, threshold, band, unsure));
   if (unsure && fallbacks) atomicAdd(fallbacks, 1ull);
-  kmap[p] = static_cast<uint8_t>(k < nrefs ? k : (nrefs < 1 ? 1 : nrefs));
 }
 
 unsigned grid(long long n) { return static_cast<unsigned>((n + 255) / 256); }
@@ -275,6 +396,13 @@ void cnn_adamw(float* params, const float* grads, float* m1, float* m2, uint16_t
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = 1.f - std::pow(b1, static_cast<float>(step)), c2 = 1.f - std::pow(b2, static_cast<float>(step));
   k_adamw<<<grid(kParams), 256, 0, s>>>(params, grads, m1, m2, kParams, lr, wd, b1, b2, 1e-8f, c1, c2, pb);
+}
+
+void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
+               float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s) {
+  if (b <= 0 || h <= 0 || wd <= 0) return;
+  const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band};
+  k_infer<<<dim3((wd + kTileW - 1) / kTileW, (h + kTileH - 1) / kTileH, b), 256, 0, s>>>(a);
 }
 
 void cnn_limits(const float* logits, int m, int nrefs, float threshold, float band, uint8_t* kmap,

@@ -6,8 +6,10 @@
 // superblock (its 64 cells and the ring, src/gpu/inter_common.h:225-243), and every candidate block's
 // 4x4 tiles lie on the superblock's 4x4 grid. So per distinct MV the kernel scores the whole 64x64
 // area once - 256 tile SATDs - and sums tiles up the block quadtree; each block's cost for any
-// candidate is then a lookup. A workgroup takes one (superblock, reference); its 4 waves take the
-// distinct MVs in turn. Within a block, candidate k's cost is (SATD << 7 | k) and the minimum of
+// candidate is then a lookup. A workgroup takes one (superblock, reference); each of its 4 waves
+// takes every fourth distinct MV on its own, with no workgroup barrier between MVs, the ref rows
+// through buffer loads and the cell -> (block, candidate) tables from LDS. Within a block, candidate
+// k's cost is (SATD << 7 | k) and the minimum of
 // those keys (LDS atomicMin, order-free) is the first strict minimum in candidate order, exactly the
 // reference's choice (a repeated MV costs the same as its first occurrence and never wins; the
 // lambda * (3 + r) term is the same for every candidate of a reference and cannot change the order).
@@ -38,6 +40,8 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
   __shared__ uint32_t best[kBlocks];
   __shared__ unsigned long long seen[kBlocks][2];  // MV classes among each block's candidates (counting only)
   __shared__ uint32_t sc[4][kBlocks];
+  __shared__ uint16_t cell_first[kRegion * kRegion + 1];
+  __shared__ uint16_t cell_tasks[kTasks];
   __shared__ int nuniq;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -47,7 +51,9 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
   const int bi = batch_item(batch, glg);
   const InterArgs& a = batch.item[bi];
   const int lg = glg - batch.first[bi];
-  const int r = lg / (a.nrows * a.sb_cols), rem = lg - r * a.nrows * a.sb_cols;
+  // Superblock-major (the references of a superblock adjacent): the XCD-major numbering then gives
+  // every XCD whole superblocks, however the reference limits thin out the far references.
+  const int r = lg % a.nrefs, rem = lg / a.nrefs;
   const int sby = a.row0 + rem / a.sb_cols, sbx = rem % a.sb_cols;
   const int ox = 64 * sbx, oy = 64 * sby;
   if (a.kmap && r >= a.kmap[sby * a.sb_cols + sbx]) return;  // reference pruned here: whole workgroup
@@ -58,23 +64,21 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
     mv[tid] = field[cy * a.cols + cx];
   }
   for (int b = tid; b < kBlocks; b += 256) best[b] = 0xffffffffu, seen[b][0] = seen[b][1] = 0;
+  // The cell -> (block, candidate) tables, read once per MV below: from LDS.
+  for (int i = tid; i < kRegion * kRegion + 1; i += 256) cell_first[i] = g_cell_first[i];
+  for (int i = tid; i < kTasks; i += 256) cell_tasks[i] = g_cell_tasks[i];
   __syncthreads();
   // Distinct MVs, numbered in cell order (wave 0); each cell's class is its first cell with that MV.
   const int c0 = lane, c1 = lane + 64;
   int rep0 = c0, rep1 = c1;
   if (wave == 0) {
+    // Each cell's first cell with the same MV: cell c's MV broadcast from the lane that holds it.
     const uint32_t m0 = mv[c0], m1 = c1 < 100 ? mv[c1] : 0u;
-    for (int c = 0; c < c0; c++)
-      if (mv[c] == m0) {
-        rep0 = c;
-        break;
-      }
-    if (c1 < 100)
-      for (int c = 0; c < c1; c++)
-        if (mv[c] == m1) {
-          rep1 = c;
-          break;
-        }
+    for (int c = 0; c < 99; c++) {
+      const uint32_t mc = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c < 64 ? m0 : m1), c & 63));
+      if (c < c0 && rep0 == c0 && m0 == mc) rep0 = c;
+      if (c < c1 && rep1 == c1 && m1 == mc) rep1 = c;
+    }
     const uint64_t b0 = __ballot(rep0 == c0), b1 = __ballot(c1 < 100 && rep1 == c1);
     const uint64_t below = (1ull << lane) - 1ull;
     if (rep0 == c0) {
@@ -110,71 +114,85 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
     cpair[ty] = tile_pair(ca, cb);
   }
   const DPlane ref = a.ref[r];
+  const BPlane rb = as_buffer(ref);
   const int U = nuniq;
   uint32_t* const mysc = sc[wave];
-  for (int u0 = 0; u0 < U; u0 += 4) {
-    const int u = u0 + wave;
-    if (u < U) {
-      const uint32_t m = mv[urep[u]];
-      const int mx = mv_x(m), my = mv_y(m);
-      const uint8_t* plane = ref.p + a.phase_bytes * ((my & 3) * 4 + (mx & 3));
-      const int X = clamp_origin(ox + (mx >> 2), ref.w, 64), Y = clamp_origin(oy + (my >> 2), ref.h, 64);
-      uint32_t pw[8][2];
+  // The 8 rows of the lane's square under MV m: 12 aligned bytes each (row i at soffset i * stride,
+  // a multiple of 4), and the byte shift of the square's first sample.
+  auto load_rows = [&](uint32_t m, uint32_t (&d)[8][3], int& sh) {
+    const int mx = mv_x(m), my = mv_y(m);
+    const int X = clamp_origin(ox + (mx >> 2), ref.w, 64), Y = clamp_origin(oy + (my >> 2), ref.h, 64);
+    const int off = static_cast<int>(a.phase_bytes) * ((my & 3) * 4 + (mx & 3)) + corner_offset(rb, Y + 8 * sqy, X + 8 * sqx);
+    sh = off & 3;
 #pragma unroll
-      for (int i = 0; i < 8; i++) load8u(plane + static_cast<ptrdiff_t>(Y + 8 * sqy + i) * ref.stride + X + 8 * sqx, pw[i][0], pw[i][1]);
-      // Tiles (ty, 0) and (ty, 1) of the square in one packed pass per tile row.
-      uint32_t t[2][2];
+    for (int i = 0; i < 8; i++) bload12(rb, off & ~3, i * rb.stride, d[i][0], d[i][1], d[i][2]);
+  };
+  // Each wave takes the distinct MVs u = wave, wave + 4, ... on its own (its own LDS scores; the
+  // block minima are order-free atomics).
+  for (int u = wave; u < U; u += 4) {
+    uint32_t dn[8][3];
+    int shn = 0;
+    load_rows(mv[urep[u]], dn, shn);
+    uint32_t pw[8][2];
 #pragma unroll
-      for (int ty = 0; ty < 2; ty++) {
-        const uint32_t pa[4] = {pw[4 * ty][0], pw[4 * ty + 1][0], pw[4 * ty + 2][0], pw[4 * ty + 3][0]};
-        const uint32_t pb[4] = {pw[4 * ty][1], pw[4 * ty + 1][1], pw[4 * ty + 2][1], pw[4 * ty + 3][1]};
-        const uint32_t two = satd4x4_pair(cpair[ty], pa, pb);
-        t[ty][0] = two & 0xffffu, t[ty][1] = two >> 16;
-      }
-      // Level 3 (8x8 and its halves): this lane's square.
-      const int b3 = (21 + 8 * sqy + sqx) * 5;
-      const uint32_t W = t[0][0] + t[0][1] + t[1][0] + t[1][1];
-      mysc[b3] = W, mysc[b3 + 1] = t[0][0] + t[0][1], mysc[b3 + 2] = t[1][0] + t[1][1];
-      mysc[b3 + 3] = t[0][0] + t[1][0], mysc[b3 + 4] = t[0][1] + t[1][1];
-      // Level 2 (16x16): lanes (2a..2a+1, 2b..2b+1).
-      const uint32_t hp = W + dpp<kDppXor1>(W), vp = W + dpp<kDppRor8>(W), quad = hp + dpp<kDppRor8>(hp);
-      const int b2 = (5 + 4 * (sqy >> 1) + (sqx >> 1)) * 5;
-      if (!(sqx & 1) && !(sqy & 1)) mysc[b2] = quad, mysc[b2 + 1] = hp, mysc[b2 + 3] = vp;
-      if (!(sqx & 1) && (sqy & 1)) mysc[b2 + 2] = hp;
-      if ((sqx & 1) && !(sqy & 1)) mysc[b2 + 4] = vp;
-      // Level 1 (32x32).
-      const uint32_t hp2 = quad + dpp<kDppXor2>(quad), vp2 = xor_add(quad, 16), quad2 = xor_add(hp2, 16);
-      const int b1 = (1 + 2 * (sqy >> 2) + (sqx >> 2)) * 5;
-      if (!(sqx & 3) && !(sqy & 3)) mysc[b1] = quad2, mysc[b1 + 1] = hp2, mysc[b1 + 3] = vp2;
-      if (!(sqx & 3) && (sqy & 3) == 2) mysc[b1 + 2] = hp2;
-      if ((sqx & 3) == 2 && !(sqy & 3)) mysc[b1 + 4] = vp2;
-      // Level 0 (64x64).
-      const uint32_t hp1 = quad2 + dpp<kDppHalfMirror>(quad2), vp1 = xor_add(quad2, 32), quad1 = xor_add(hp1, 32);
-      if (lane == 0) mysc[0] = quad1, mysc[1] = hp1, mysc[3] = vp1;
-      if (lane == 32) mysc[2] = hp1;
-      if (lane == 4) mysc[4] = vp1;
+    for (int i = 0; i < 8; i++) pw[i][0] = __builtin_amdgcn_alignbyte(dn[i][1], dn[i][0], shn), pw[i][1] = __builtin_amdgcn_alignbyte(dn[i][2], dn[i][1], shn);
+    // Tiles (ty, 0) and (ty, 1) of the square in one packed pass per tile row.
+    uint32_t t[2][2];
+#pragma unroll
+    for (int ty = 0; ty < 2; ty++) {
+      const uint32_t pa[4] = {pw[4 * ty][0], pw[4 * ty + 1][0], pw[4 * ty + 2][0], pw[4 * ty + 3][0]};
+      const uint32_t pb[4] = {pw[4 * ty][1], pw[4 * ty + 1][1], pw[4 * ty + 2][1], pw[4 * ty + 3][1]};
+      const uint32_t two = satd4x4_pair(cpair[ty], pa, pb);
+      t[ty][0] = two & 0xffffu, t[ty][1] = two >> 16;
     }
-    __syncthreads();
-    if (u < U) {
-      // Every (block, candidate) whose cell has this MV.
-      const uint64_t in0 = __ballot(cuid[lane] == u), in1 = __ballot(lane + 64 < 100 && cuid[min(lane + 64, 99)] == u);
-      for (int half = 0; half < 2; half++) {
-        uint64_t bits = half ? in1 : in0;
-        while (bits) {
-          const int c = __ffsll(static_cast<long long>(bits)) - 1 + 64 * half;
-          bits &= bits - 1;
-          const int f0 = g_cell_first[c], f1 = g_cell_first[c + 1];
-          for (int i = f0 + lane; i < f1; i += 64) {
-            const uint32_t e = g_cell_tasks[i];
-            const int b = static_cast<int>(e >> 7);
-            atomicMin(&best[b], (mysc[b] << 7) | (e & 127u));
-            if (a.candidates) atomicOr(&seen[b][u >> 6], 1ull << (u & 63));
-          }
+    // Level 3 (8x8 and its halves): this lane's square.
+    const int b3 = (21 + 8 * sqy + sqx) * 5;
+    const uint32_t W = t[0][0] + t[0][1] + t[1][0] + t[1][1];
+    mysc[b3] = W, mysc[b3 + 1] = t[0][0] + t[0][1], mysc[b3 + 2] = t[1][0] + t[1][1];
+    mysc[b3 + 3] = t[0][0] + t[1][0], mysc[b3 + 4] = t[0][1] + t[1][1];
+    // Level 2 (16x16): lanes (2a..2a+1, 2b..2b+1).
+    const uint32_t hp = W + dpp<kDppXor1>(W), vp = W + dpp<kDppRor8>(W), quad = hp + dpp<kDppRor8>(hp);
+    const int b2 = (5 + 4 * (sqy >> 1) + (sqx >> 1)) * 5;
+    if (!(sqx & 1) && !(sqy & 1)) mysc[b2] = quad, mysc[b2 + 1] = hp, mysc[b2 + 3] = vp;
+    if (!(sqx & 1) && (sqy & 1)) mysc[b2 + 2] = hp;
+    if ((sqx & 1) && !(sqy & 1)) mysc[b2 + 4] = vp;
+    // Level 1 (32x32).
+    const uint32_t hp2 = quad + dpp<kDppXor2>(quad), vp2 = xor_add(quad, 16), quad2 = xor_add(hp2, 16);
+    const int b1 = (1 + 2 * (sqy >> 2) + (sqx >> 2)) * 5;
+    if (!(sqx & 3) && !(sqy & 3)) mysc[b1] = quad2, mysc[b1 + 1] = hp2, mysc[b1 + 3] = vp2;
+    if (!(sqx & 3) && (sqy & 3) == 2) mysc[b1 + 2] = hp2;
+    if ((sqx & 3) == 2 && !(sqy & 3)) mysc[b1 + 4] = vp2;
+    // Level 0 (64x64).
+    const uint32_t hp1 = quad2 + dpp<kDppHalfMirror>(quad2), vp1 = xor_add(quad2, 32), quad1 = xor_add(hp1, 32);
+    if (lane == 0) mysc[0] = quad1, mysc[1] = hp1, mysc[3] = vp1;
+    if (lane == 32) mysc[2] = hp1;
+    if (lane == 4) mysc[4] = vp1;
+    // The wave's scores are in LDS before any lane reads another lane's.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Every (block, candidate) whose cell has this MV.
+    const uint64_t in0 = __ballot(cuid[lane] == u), in1 = __ballot(lane + 64 < 100 && cuid[min(lane + 64, 99)] == u);
+    for (int half = 0; half < 2; half++) {
+      uint64_t bits = half ? in1 : in0;
+      while (bits) {
+        const int c = __ffsll(static_cast<long long>(bits)) - 1 + 64 * half;
+        bits &= bits - 1;
+        const int f0 = cell_first[c], f1 = cell_first[c + 1];
+        for (int i = f0 + lane; i < f1; i += 64) {
+          const uint32_t e = cell_tasks[i];
+          const int b = static_cast<int>(e >> 7);
+          atomicMin(&best[b], (mysc[b] << 7) | (e & 127u));
+          if (a.candidates) atomicOr(&seen[b][u >> 6], 1ull << (u & 63));
         }
       }
     }
-    __syncthreads();
+    // Every lane's reads of the scores come before the next MV's writes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+  __syncthreads();
   uint2* out = a.best + ((static_cast<size_t>(sby) * a.sb_cols + sbx) * kMaxRefs + r) * kBlocks;
   for (int b = tid; b < kBlocks; b += 256) {
     const uint32_t key = best[b];

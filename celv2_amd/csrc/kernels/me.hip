@@ -68,47 +68,68 @@ __device__ __forceinline__ uint32_t score_window(const uint32_t* win, int sh, in
 constexpr uint32_t kRefDx = (1u << 0) | (0u << 2) | (1u << 4) | (2u << 6) | (0u << 8) | (2u << 10) | (0u << 12) | (1u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u << 8) | (1u << 10) | (2u << 12) | (2u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 
-__global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
-  __shared__ uint32_t wins[4][2][kWinLds];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addressing
-  const int total = batch.total();
-  const int glg = xcd_logical(blockIdx.x, total);
-  if (glg >= total) return;
-  const int bi = batch_item(batch, glg);
-  const MeLevelArgs& a = batch.item[bi];
-  const int lg = glg - batch.first[bi];
-  const int per_ref = a.rows * a.cols, nitems = a.nrefs * per_ref;
-  const int item = 4 * lg + wave;
-  if (item >= nitems) return;  // whole wave
-  const int r = item / per_ref, rem = item - r * per_ref, by = rem / a.cols, bx = rem - by * a.cols;
-  if (a.kmap && a.level <= 1 && r > 0) {
-    // Searched only near a superblock that keeps reference r (lanes 0..8: the 3x3 neighbourhood).
+// The loads that open a block's search, issued together: lanes 0..8
+// the reference limits of the 3x3 superblocks around it (0 outside the frame or without limits), lanes
+// 0..15 the block's 16 words, lanes 0..4 the coarser MVs under and beside it, and at level 0 the row
+// of the block's 4x4 tile that the lane feeds to the refinement's products.
+struct Opening {
+  uint32_t km = 0, mine = 0, cm = 0, crow = 0;
+};
+
+struct BlockPos {
+  int r, bx, by;
+};
+__device__ __forceinline__ BlockPos block_pos(const MeLevelArgs& a, int item) {
+  const int per_ref = a.rows * a.cols;
+  const int r = item / per_ref, rem = item - r * per_ref, by = rem / a.cols;
+  return {r, rem - by * a.cols, by};
+}
+__device__ __forceinline__ bool limited(const MeLevelArgs& a, int r) { return a.kmap && a.level <= 1 && r > 0; }
+
+__device__ __forceinline__ Opening open_block(const MeLevelArgs& a, int item, int lane) {
+  const BlockPos q = block_pos(a, item);
+  Opening o;
+  if (limited(a, q.r)) {
     const int shift = 3 - a.level;  // 8x8 blocks per superblock side: 8 at level 0, 4 at level 1
-    const int sx = (bx >> shift) + lane % 3 - 1, sy = (by >> shift) + lane / 3 - 1;
-    const bool keeps = lane < 9 && sx >= 0 && sy >= 0 && sx < a.sb_cols && sy < a.sb_rows && a.kmap[sy * a.sb_cols + sx] > r;
-    if (__ballot(keeps) == 0) return;  // whole wave
+    const int sx = (q.bx >> shift) + lane % 3 - 1, sy = (q.by >> shift) + lane / 3 - 1;
+    if (lane < 9 && sx >= 0 && sy >= 0 && sx < a.sb_cols && sy < a.sb_rows) o.km = a.kmap[sy * a.sb_cols + sx];
   }
+  const BPlane cb = as_buffer(a.cur);
+  if (lane < 16) o.mine = bload4(cb, corner_offset(cb, 8 * q.by + (lane >> 1), 8 * q.bx + 4 * (lane & 1)), 0);
+  if (a.coarser[q.r] && lane < 5) {
+    const int ox = lane == 1 ? -1 : lane == 2 ? 1 : 0, oy = lane == 3 ? -1 : lane == 4 ? 1 : 0;
+    const int qx = clampi(q.bx / 2 + ox, 0, a.pcols - 1), qy = clampi(q.by / 2 + oy, 0, a.prows - 1);
+    o.cm = a.coarser[q.r][qy * a.pcols + qx];
+  }
+  if (a.level0) {
+    const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;
+    o.crow = bload4(cb, corner_offset(cb, 8 * q.by + 4 * ty + g, 8 * q.bx + 4 * tx), 0);
+  }
+  return o;
+}
+
+// Block `item` (reference-major: r = item / blocks per reference) of one launch item, by one wave;
+// win1 / win2 are the wave's two LDS windows.
+__device__ __forceinline__ void search_block(const MeLevelArgs& a, int item, int lane, uint32_t* win1, uint32_t* win2) {
+  const BlockPos q = block_pos(a, item);
+  const int r = q.r, bx = q.bx, by = q.by;
+  const Opening o = open_block(a, item, lane);
+  // Searched only near a superblock that keeps reference r.
+  if (limited(a, r) && __ballot(o.km > static_cast<uint32_t>(r)) == 0) return;  // whole wave
   const DPlane cur = a.cur, ref = a.ref[r];
   const BPlane rb = as_buffer(ref), cb = as_buffer(cur);
 
   // The block, every lane holding every word (scalar registers after readlane).
-  const uint32_t mine = lane < 16 ? bload4(cb, corner_offset(cb, 8 * by + (lane >> 1), 8 * bx + 4 * (lane & 1)), 0) : 0u;
   uint32_t c[16];
 #pragma unroll
-  for (int i = 0; i < 16; i++) c[i] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(mine), i));
+  for (int i = 0; i < 16; i++) c[i] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(o.mine), i));
 
   int p1x = 0, p1y = 0, p2x = 0, p2y = 0;
   bool two = false;
   if (a.coarser[r]) {
     // Centres k = 0..4: the coarser MV under the block and its left, right, above and below
     // neighbours (clamped into the coarser field), doubled; k = 5: zero.
-    uint32_t cm = 0;
-    if (lane < 5) {
-      const int ox = lane == 1 ? -1 : lane == 2 ? 1 : 0, oy = lane == 3 ? -1 : lane == 4 ? 1 : 0;
-      const int qx = clampi(bx / 2 + ox, 0, a.pcols - 1), qy = clampi(by / 2 + oy, 0, a.prows - 1);
-      cm = a.coarser[r][qy * a.pcols + qx];
-    }
+    const uint32_t cm = o.cm;
     int cx[6], cy[6];
 #pragma unroll
     for (int k = 0; k < 5; k++) {
@@ -153,8 +174,6 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
     koff[i] = row * rb.stride + 4 * d;
     kdst[i] = k < kWin * kWinDwords ? row * kWinStride + d : -1;
   }
-  uint32_t* win1 = wins[wave][0];
-  uint32_t* win2 = wins[wave][1];
   const int X1 = clamp_origin(8 * bx + p1x - kRange, ref.w, kWin), Y1 = clamp_origin(8 * by + p1y - kRange, ref.h, kWin);
   const int X2 = clamp_origin(8 * bx + p2x - kRange, ref.w, kWin), Y2 = clamp_origin(8 * by + p2y - kRange, ref.h, kWin);
   stage_window(rb, X1, Y1, koff, kdst, win1);
@@ -194,8 +213,7 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
     const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;
     const half4 hh = hadamard_operand(lane, false), nhh = hadamard_operand(lane, true);
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-    const uint32_t crow = bload4(cb, corner_offset(cb, 8 * by + 4 * ty + g, 8 * bx + 4 * tx), 0);
-    const f32x4 ccur = mfma16(hh, row_halves(crow), zero);
+    const f32x4 ccur = mfma16(hh, row_halves(o.crow), zero);
     int odx[3], ody[3];
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) {
@@ -238,11 +256,29 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
   }
 }
 
+// One workgroup per unit: 4 blocks of one reference, one per wave.
+__global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
+  __shared__ uint32_t wins[4][2][kWinLds];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addressing
+  const int total = batch.total();
+  const int g = xcd_logical(blockIdx.x, static_cast<int>(gridDim.x));
+  if (g >= total) return;  // whole workgroup
+  // Units are (group of 4 consecutive blocks of a reference, reference), the reference cycling
+  // fastest: the XCD-major numbering then gives every XCD all references of its blocks, however the
+  // reference limits thin out the far ones, and a workgroup's 4 waves search neighbouring blocks.
+  const int bi = batch_item(batch, g);
+  const MeLevelArgs& a = batch.item[bi];
+  const int u = g - batch.first[bi], r = u % a.nrefs, block = 4 * (u / a.nrefs) + wave, per_ref = a.rows * a.cols;
+  if (block >= per_ref) return;  // whole wave
+  search_block(a, r * per_ref + block, lane, wins[wave][0], wins[wave][1]);
+}
+
 }  // namespace
 
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a) {
-  const int nitems = a.nrefs * a.rows * a.cols;
-  if (nitems > 0) b.add(a, (nitems + 3) / 4);
+  const int units = a.nrefs * ((a.rows * a.cols + 3) / 4);
+  if (units > 0) b.add(a, units);
 }
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s) {
   if (b.total() <= 0) return;

@@ -1,0 +1,15 @@
+"""The learned pruner's trained parameters and operating point (weights.f32, pruner.json), written by
+celv2_amd.pruner.train."""
+import json
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def load(directory=HERE):
+    """(parameters float32, config dict) of a trained pruner."""
+    w = np.fromfile(os.path.join(directory, "weights.f32"), dtype="<f4")
+    with open(os.path.join(directory, "pruner.json")) as f:
+        return w, json.load(f)

@@ -179,13 +179,14 @@ int orc_fields(int w, int h, const uint8_t* cur, const uint8_t* ref, int* mvx, i
 
 // One search_cell (+ refine at level 0) on raw planes: cur/ref are w x h (stride w); ref_phases
 // non-null: 16 planes of w*h for the refinement. pmx/pmy: coarser field (NULL at the top).
+// Returns the integer search's winning cost (SAD + distance penalty).
 int orc_search_cell(const uint8_t* cur, const uint8_t* ref, const uint8_t* ref_phases, int w, int h, int bx, int by,
                     const int* pmx, const int* pmy, int pcols, int prows, int level0, int* mv) {
   const Plane8 pc{cur, w, w, h, 0};
   const Plane8 pr{ref_phases ? ref_phases : ref, w, w, h, ref_phases ? static_cast<size_t>(w) * h : 0};
-  search_cell(pc, pr, bx, by, pmx, pmy, pcols, prows, &mv[0], &mv[1], level0 != 0);
+  const int cost = search_cell(pc, pr, bx, by, pmx, pmy, pcols, prows, &mv[0], &mv[1], level0 != 0);
   if (level0 && ref_phases) refine_qpel(pc, pr, bx, by, &mv[0], &mv[1]);
-  return 0;
+  return cost;
 }
 
 // SATDs on raw planes (w x h, stride w; ref with phases if ref_phases).

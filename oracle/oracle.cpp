@@ -121,8 +121,8 @@ void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx,
 // inter_common.h:327-358 — six candidate centres (coarser MV x2 under the block, its left, right,
 // above and below neighbours, then zero), each distinct one ranked by SAD; the best and the
 // runner-up are both searched and the runner-up's result kept only if strictly cheaper.
-void search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
-                 int prows, int* mvx, int* mvy, bool /*level0*/) {
+int search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
+                int prows, int* mvx, int* mvy, bool /*level0*/) {
   int c1x = 0, c1y = 0, c2x = 0, c2y = 0;
   bool have2 = false;
   if (pmx) {
@@ -148,8 +148,13 @@ void search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int
   const int cost1 = full_search(cur, ref, bx, by, c1x, c1y, kRange, mvx, mvy);
   if (have2) {
     int mx, my;
-    if (full_search(cur, ref, bx, by, c2x, c2y, kRange, &mx, &my) < cost1) *mvx = mx, *mvy = my;
+    const int cost2 = full_search(cur, ref, bx, by, c2x, c2y, kRange, &mx, &my);
+    if (cost2 < cost1) {
+      *mvx = mx, *mvy = my;
+      return cost2;
+    }
   }
+  return cost1;
 }
 
 // intra_common.h:47-56 — source neighbours of the block at (x, y); 128 outside the frame's top/left.

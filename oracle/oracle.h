@@ -70,8 +70,9 @@ int full_search(const Plane8& cur, const Plane8& ref, int bx, int by, int px, in
                 int* mvy);                                                           // :273-284
 int satd_8x8_q(const Plane8& cur, int x, int y, const Plane8& ref, int qmx, int qmy); // :287-298
 void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx, int* mvy);  // :306-319
-void search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
-                 int prows, int* mvx, int* mvy, bool level0);                         // :327-358
+// Returns the integer search's winning cost (SAD + distance penalty from its centre).
+int search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
+                int prows, int* mvx, int* mvy, bool level0);                          // :327-358
 int edge_above(const Plane8& s, int x, int y, int k);                                 // intra_common.h:47-51
 int edge_left(const Plane8& s, int x, int y, int k);                                  // :52-56
 int directional(const Plane8& s, int x, int y, int w, int h, int angle, int i, int j); // :61-85

@@ -8,6 +8,7 @@ import pytest
 from celv2_amd import synth
 from celv2_amd.records import compare_maps
 import cnn_ref as cr
+import oracle_lib as ol
 
 pytestmark = pytest.mark.gpu
 
@@ -27,6 +28,18 @@ def batch():
 def test_forward_matches_cpu_reference(gpu, batch):
     x, _ = batch
     p = gpu.Pruner(max_positions=1024, seed=5)
+    got = p.forward(x).reshape(-1, 6)
+    want, _ = cr.forward(p.params(), x)
+    assert rel(got, want) < 2e-3, rel(got, want)
+    p.close()
+
+
+def test_forward_on_a_1080p_grid(gpu):
+    """The one-launch inference over several 4 x 16 tiles of a 17 x 30 grid (two frames), ragged at
+    the bottom and right."""
+    rng = np.random.default_rng(12)
+    x = rng.uniform(0, 1, (2, 17, 30, cr.F)).astype(np.float32)
+    p = gpu.Pruner(max_positions=2 * 17 * 30, seed=13)
     got = p.forward(x).reshape(-1, 6)
     want, _ = cr.forward(p.params(), x)
     assert rel(got, want) < 2e-3, rel(got, want)
@@ -86,3 +99,65 @@ def test_engine_uses_the_network_limits(gpu):
     assert np.array_equal(e.fetch_sb_costs(), pruned_cost)
     e.close()
     p.close()
+
+
+def cpu_features(src2, costs, sb_cols, sb_rows):
+    """The features from the level-2 source and each reference's level-2 block costs [r][rows2, cols2]."""
+    n = len(costs)
+    rows2, cols2 = costs[0].shape
+    h2, w2 = src2.shape
+    out = np.zeros((sb_rows, sb_cols, cr.F))
+    for sy in range(sb_rows):
+        for sx in range(sb_cols):
+            f = out[sy, sx]
+            blocks = [(min(2 * sy + (q >> 1), rows2 - 1), min(2 * sx + (q & 1), cols2 - 1)) for q in range(4)]
+            env = np.array([[c[b] for b in blocks] for c in costs], np.float64)
+            e = np.minimum.accumulate(env, axis=0).sum(1)
+            l = np.log2(1 + e)
+            f[0] = l[-1] / 16
+            for k in range(1, n):
+                f[k] = min(4.0, 8 * (l[k - 1] - l[k]))
+            ys = np.minimum(16 * sy + np.arange(16), h2 - 1)
+            xs = np.minimum(16 * sx + np.arange(16), w2 - 1)
+            v = src2[np.ix_(ys, xs)].astype(np.float64)
+            right = src2[np.ix_(ys, np.minimum(xs + 1, w2 - 1))].astype(np.float64)
+            down = src2[np.ix_(np.minimum(ys + 1, h2 - 1), xs)].astype(np.float64)
+            mean = v.mean()
+            f[7] = mean / 255
+            f[8] = np.log2(1 + max(0.0, (v * v).mean() - mean * mean)) / 16
+            f[9] = np.log2(1 + 16 * (np.abs(right - v) + np.abs(down - v)).sum() / 512) / 16
+    return out
+
+
+@pytest.mark.parametrize("w,h,nrefs", [(352, 288, 7), (202, 122, 3)])
+def test_features_match_the_coarse_search(gpu, w, h, nrefs):
+    """Features against the oracle's level-2 search costs (on the engine's own level-2 planes and
+    level-3 fields, both checked against the oracle elsewhere) and the level-2 source."""
+    O = ol.oracle()
+    y8 = synth.clip_frames(synth.params(1, 41, w, h), nrefs + 1)[1]
+    refs = list(range(nrefs - 1, -1, -1))
+    e = gpu.RawEngine(w, h)
+    e.set_features(True)
+    e.launch(gpu.RawEngine.luma(y8[nrefs]), [gpu.RawEngine.luma(y8[r]) for r in refs], gpu.lambda_q8(130), keys=refs)
+    e.collect()
+    got = e.fetch_features()
+    src2 = np.ascontiguousarray(e.fetch_plane(-1, 2))
+    h2, w2 = src2.shape
+    costs = []
+    for r in range(nrefs):
+        ref2 = np.ascontiguousarray(e.fetch_plane(r, 2))
+        pmx, pmy = (np.ascontiguousarray(a, np.int32) for a in e.fetch_field(r, 3))
+        mx2, _ = e.fetch_field(r, 2)
+        rows2, cols2 = mx2.shape
+        c = np.zeros((rows2, cols2), np.int64)
+        mv = np.zeros(2, np.int32)
+        for by in range(rows2):
+            for bx in range(cols2):
+                c[by, bx] = O.orc_search_cell(src2.ctypes.data, ref2.ctypes.data, None, w2, h2, bx, by, pmx.ctypes.data, pmy.ctypes.data,
+                                              pmx.shape[1], pmx.shape[0], 0, mv.ctypes.data)
+        costs.append(c)
+    want = cpu_features(src2, costs, e.sb_cols, e.sb_rows)
+    e.close()
+    assert np.all(got[..., nrefs:7] == 0)
+    err = np.abs(got - want).max(axis=(0, 1))
+    assert err.max() < 1e-3, err.tolist()

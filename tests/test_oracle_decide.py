@@ -162,8 +162,9 @@ def test_two_centres_are_searched():
     pmx = np.full(8 * 4, 13, np.int32)
     pmy = np.zeros(8 * 4, np.int32)
     mv = np.zeros(2, np.int32)
-    O.orc_search_cell(ptr(cur), ptr(ref), None, w, h, bx, by, ptr(pmx), ptr(pmy), 8, 4, 0, ptr(mv))
+    cost = O.orc_search_cell(ptr(cur), ptr(ref), None, w, h, bx, by, ptr(pmx), ptr(pmy), 8, 4, 0, ptr(mv))
     assert list(mv) == [20, 0]
+    assert cost == 4 * (26 - 20)  # an exact match 6 pels from the one centre (2 x 13, 0)
 
 
 def shifted_texture(w, h, sx, sy, seed=1):
