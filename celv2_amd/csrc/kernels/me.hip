@@ -90,9 +90,17 @@ __device__ __forceinline__ Opening open_block(const MeLevelArgs& a, int item, in
   const BlockPos q = block_pos(a, item);
   Opening o;
   if (limited(a, q.r)) {
-    const int shift = 3 - a.level;  // 8x8 blocks per superblock side: 8 at level 0, 4 at level 1
-    const int sx = (q.bx >> shift) + lane % 3 - 1, sy = (q.by >> shift) + lane / 3 - 1;
-    if (lane < 9 && sx >= 0 && sy >= 0 && sx < a.sb_cols && sy < a.sb_rows) o.km = a.kmap[sy * a.sb_cols + sx];
+    // A superblock's decisions read the level-0 field on it and a ring of 1 block (its inter
+    // candidates), and those blocks' searches read the level-1 field on them and 1 more block each
+    // way (their coarser centres): a ring of 2 at level 1. Lane 3 (dy + 1) + dx + 1 asks the
+    // neighbour (dx, dy) whether this block lies in its ring.
+    const int shift = 3 - a.level, n = 1 << shift;  // blocks per superblock side: 8 at level 0, 4 at level 1
+    const int reach = a.level == 0 ? 1 : 2;
+    const int dx = lane % 3 - 1, dy = lane / 3 - 1;
+    const int sx = (q.bx >> shift) + dx, sy = (q.by >> shift) + dy, lx = q.bx & (n - 1), ly = q.by & (n - 1);
+    const bool in_x = dx == 0 || (dx < 0 ? lx < reach : lx >= n - reach);
+    const bool in_y = dy == 0 || (dy < 0 ? ly < reach : ly >= n - reach);
+    if (lane < 9 && in_x && in_y && sx >= 0 && sy >= 0 && sx < a.sb_cols && sy < a.sb_rows) o.km = a.kmap[sy * a.sb_cols + sx];
   }
   const BPlane cb = as_buffer(a.cur);
   if (lane < 16) o.mine = bload4(cb, corner_offset(cb, 8 * q.by + (lane >> 1), 8 * q.bx + 4 * (lane & 1)), 0);
