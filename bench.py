@@ -150,18 +150,20 @@ def bench_batched(NR, lam, steps=20, warmup=3, nf=16):
             "speedup": round(rates[best][0] / rates["plain"][0], 3), "decisions_equal_plain": same}
 
 
-def bench_learned(lum, decidable, NR, lam, steps, warmup, plain_fps):
+def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
     """The learned-pruned form on the same workload: the trained network (celv2_amd/pruner) runs inside
     every launch on the engine's coarse search and limits each superblock's references; frames/s over
     the same steps, then, untimed, the decision-cost regret against the plain engine on every
     decidable frame (sum of J_pruned - J_plain over sum of J_plain), the mean reference limit and the
     fallback rate."""
+    lam = engine.lambda_q8(qindex)
     w, cfg = pruner.load()
+    threshold, band = pruner.operating_point(cfg, qindex)
     P = engine.Pruner(max_positions=(1920 // 64 + 1) * (1080 // 64 + 1), seed=1)
     P.set_params(w)
     E = engine.RawEngine(1920, 1080)
     E.set_resident(True)
-    E.set_pruner(P, cfg["threshold"], cfg["band"])
+    E.set_pruner(P, threshold, band)
 
     def step(s):
         i = decidable[s % len(decidable)]
@@ -211,9 +213,9 @@ def bench_learned(lum, decidable, NR, lam, steps, warmup, plain_fps):
             "cost_regret": round(regret, 6), "cost_regret_unit": "sum(J_learned - J_plain) / sum(J_plain) over every decidable frame (engine J = SATD*256 + lambda*bits)",
             "bd_rate": "pending (needs the full encoder and AVM)",
             "mean_reference_limit": round(float(ks.mean()), 3), "fallback_rate": round(c2["pruner_fallbacks"] / ks.size, 5),
-            "threshold": cfg["threshold"], "band": cfg["band"], "stage_ms_per_frame": per,
+            "threshold": threshold, "band": band, "stage_ms_per_frame": per,
             "inter_candidates_per_frame": round(inter_l), "compound_blocks_per_frame": round(c2["compound_blocks"] / nd),
-            "network": "conv3x3 10-32, conv3x3 32-32, conv1x1 32-6, BF16 on MFMA, trained on 40 synthetic clips (celv2_amd/pruner/train.py)"}
+            "network": f"conv3x3 11-32, conv3x3 32-32, conv1x1 32-6, BF16 on MFMA, trained on {cfg['train_superblocks']} superblocks of synthetic clips (celv2_amd/pruner/train.py)"}
 
 
 def main():
@@ -345,7 +347,7 @@ def main():
     batched = None if args.no_batched else bench_batched(NR, lam)
     learned = None
     if not args.no_learned and NR == 7:
-        learned = bench_learned(lum, decidable, NR, lam, args.steps, args.warmup, fps)
+        learned = bench_learned(lum, decidable, NR, args.qindex, args.steps, args.warmup, fps)
         learned["scored_candidates_per_frame"] = round(sbs * 420 * 13 + learned["inter_candidates_per_frame"] + learned["compound_blocks_per_frame"])
         learned["scored_candidates_per_s"] = round(learned["scored_candidates_per_frame"] * learned["frames_per_s"])
 

@@ -773,6 +773,7 @@ void launch_batch(const LaunchIn* in, int n) {
         fa.src2 = cur[i]->p.lv[2];
         for (int r = 0; r < in[i].nrefs; r++) fa.cost2[r] = e->cost2[r];
         fa.nrefs = in[i].nrefs, fa.cols2 = e->fcols[2], fa.rows2 = e->frows[2], fa.sb_cols = e->sb_cols, fa.sb_rows = e->sb_rows;
+        fa.lambda_q8 = in[i].lam;
         fa.feats = e->infer.x;
         launch_features(fa, S);
         e->features_valid = true;

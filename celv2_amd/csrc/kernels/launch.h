@@ -100,6 +100,7 @@ struct FeatureArgs {
   DPlane src2;                      // the source's level 2
   const uint32_t* cost2[kMaxRefs];  // level-2 best cost per block, cols2 x rows2
   int nrefs, cols2, rows2, sb_cols, sb_rows;
+  long long lambda_q8;  // the frame's rate multiplier
   float* feats;
 };
 void launch_features(const FeatureArgs& a, hipStream_t s);

@@ -7,6 +7,7 @@
 //   f[7]     the 16x16 level-2 source mean / 255
 //   f[8]     log2(1 + the source variance) / 16
 //   f[9]     log2(1 + 16 x the mean absolute horizontal + vertical difference) / 16
+//   f[10]    log2(1 + lambda_q8) / 32 (the rate multiplier: how much a reference's extra bits weigh)
 // The gains are differences taken before rounding, so the network sees them at full relative
 // precision in BF16. Positions beyond the frame read the clamped plane, as every kernel does.
 #include "common.h"
@@ -68,6 +69,7 @@ __global__ void __launch_bounds__(256) k_features(FeatureArgs a) {
   f[7] = mean / 255.f;
   f[8] = __log2f(1.f + var) / 16.f;
   f[9] = __log2f(1.f + 16.f * grad / 512.f) / 16.f;
+  f[10] = __log2f(1.f + static_cast<float>(a.lambda_q8)) / 32.f;
   if (lane == 0) {
     float* out = a.feats + static_cast<size_t>(sb) * kFeatures;
 #pragma unroll

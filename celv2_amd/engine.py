@@ -20,6 +20,7 @@ _LIB = None
 P = ctypes.c_void_p
 I = ctypes.c_int
 LL = ctypes.c_longlong
+FEATURES = 11  # the learned pruner's inputs per superblock (include/celv2hip.h)
 
 
 class Luma(ctypes.Structure):
@@ -308,7 +309,7 @@ class RawEngine:
         self._check(lib().celv2hip_set_features(self.h, 1 if on else 0), "set_features")
 
     def fetch_features(self):
-        out = np.zeros((self.sb_rows, self.sb_cols, 10), np.float32)
+        out = np.zeros((self.sb_rows, self.sb_cols, FEATURES), np.float32)
         got = lib().celv2hip_fetch_features(self.h, out.ctypes.data, out.size)
         if got < 0:
             self._check(int(got), "fetch_features")

@@ -13,3 +13,10 @@ def load(directory=HERE):
     w = np.fromfile(os.path.join(directory, "weights.f32"), dtype="<f4")
     with open(os.path.join(directory, "pruner.json")) as f:
         return w, json.load(f)
+
+
+def operating_point(cfg, qindex):
+    """(threshold, band) chosen for the trained qindex nearest to `qindex`."""
+    ops = cfg["operating_points"]
+    q = min(ops, key=lambda k: abs(int(k) - qindex))
+    return ops[q]["threshold"], ops[q]["band"]

@@ -1,26 +1,32 @@
 """Trains the learned pruner on decision logs of the plain engine.
 
-For every decidable frame of seeded synthetic clips (workload 1, 1920x1080 10-bit, 7 references =
-the previous source frames), the engine decides the frame with its nearest k = 1..7 references and
-records each superblock's decision cost J_k (celv2hip_fetch_sb_costs) and, at k = 7, the pruner's
-features (celv2hip_fetch_features). Each superblock's decision depends only on its own limit, so the
-total cost of any per-superblock allocation k(s) follows from these logs exactly.
+Data: seeded synthetic 1920x1080 10-bit clips of 32 frames. Most are natural content (workload 1)
+with the grain sigma, object speed and pan drawn per clip; some are flat-shaded cel content
+(workload 4 at 1080p). Each clip is decided at one qindex from {100, 130, 160, 190}. For every
+decidable frame (frame i >= 7 against the 7 source frames before it, nearest first) the engine
+decides the frame with its nearest k = 1..7 references and records each superblock's decision cost
+J_k (celv2hip_fetch_sb_costs) and, at k = 7, the pruner's features (celv2hip_fetch_features). Each
+superblock's decision depends only on its own limit, so the total cost of any per-superblock
+allocation k(s) follows from these logs exactly.
 
-Labels are the allocation that keeps the fewest references for a given total regret,
-sum(J_k(s) - J_7) / sum(J_7): with a multiplier mu, k*(s) = argmin_k k + mu (J_k(s) - J_7(s)), mu
-set (by bisection) so the training set's label regret equals the target. The network (BF16 on the
-matrix cores, celv2hip_pruner_*) learns P(k* > j + 1) with AdamW on the GPU; on held-out clips the
-operating point (threshold, fallback band) is the one with the fewest references per superblock whose
-regret stays within the target, then measured again through the engine itself. Writes weights.f32
-(the FP32 parameters) and pruner.json.
+Labels: per qindex, the allocation that keeps the fewest references for a given total regret
+sum(J_k(s) - J_7) / sum(J_7): with a multiplier mu, k*(s) = argmin_k k + mu (J_k(s) - J_7(s)), mu set
+by bisection so the training clips' label regret at that qindex equals the target. The network
+(BF16 on the matrix cores, celv2hip_pruner_*) learns P(k* > j + 1) with AdamW on the GPU.
 
-    python -m celv2_amd.pruner.train [--clips 40] [--val-clips 8] [--steps 2000] [--out DIR]
+Operating points: on held-out clips, per qindex, the (threshold, fallback band) with the fewest
+references per superblock whose regret stays within the target, measured again through the engine
+itself (the network run inside the launch). Writes weights.f32 (the FP32 parameters) and
+pruner.json.
+
+    python -m celv2_amd.pruner.train [--clips 120] [--val-clips 16] [--steps 6000] [--out DIR]
 """
 import argparse
 import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -29,21 +35,34 @@ sys.path.insert(0, ROOT)
 
 from celv2_amd import engine, synth  # noqa: E402
 
-W, H, NR, QINDEX = 1920, 1080, 7, 130
+W, H, NR, FRAMES = 1920, 1080, 7, 32
+QINDICES = (100, 130, 160, 190)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def clip_logs(E, seed, frames, lam):
-    """(features [n, sb_rows, sb_cols, 10], costs [n, sbs, 7]) of a clip's decidable frames. Reference
+def clip_spec(seed):
+    """The clip of a seed: (synth params, qindex, description). Every fifth clip is cel content."""
+    rng = np.random.default_rng(seed)
+    q = QINDICES[seed % len(QINDICES)]
+    if seed % 5 == 4:
+        return synth.params(4, seed, W, H), q, "cel"
+    sigma = int(rng.integers(0, 13))
+    p = synth.params(1, seed, W, H, sigma_x16=16 * sigma, vmax8=int(rng.integers(24, 193)),
+                     pan_x8=int(rng.integers(-32, 33)), pan_y8=int(rng.integers(-16, 17)))
+    return p, q, f"natural sigma {sigma}"
+
+
+def clip_logs(E, seed, y16):
+    """(features [n, sb_rows, sb_cols, F], costs [n, sbs, 7]) of a clip's decidable frames. Reference
     keys are unique across clips (seed * 1000 + frame), so the engine's cached planes never mix clips."""
-    y16, _ = synth.clip_frames(synth.params(1, seed, W, H), frames)
+    lam = engine.lambda_q8(clip_spec(seed)[1])
     dev = [engine.DeviceBuffer(f) for f in y16]
     lum = [engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]
     feats, costs = [], []
-    for f in range(NR, frames):
+    for f in range(NR, len(y16)):
         refs = [f - k for k in range(1, NR + 1)]
         keys = [seed * 1000 + r for r in refs]
         J = []
@@ -54,35 +73,34 @@ def clip_logs(E, seed, frames, lam):
             J.append(E.fetch_sb_costs())
         feats.append(E.fetch_features())
         costs.append(np.stack(J, axis=1))
+    E.set_features(False)
     for b in dev:
         b.free()
     return np.stack(feats), np.stack(costs)
 
 
-def engine_check(E, P, cfg, seeds, frames, lam, j7):
-    """The operating point measured through the engine itself (the network run inside the launch):
-    (total cost regret against j7 [n, sbs], mean reference limit, fallback rate)."""
-    E.set_features(False)
-    E.set_pruner(P, cfg["threshold"], cfg["band"])
-    E.stats_reset()
-    got, ks = [], []
-    for seed in seeds:
-        y16, _ = synth.clip_frames(synth.params(1, seed, W, H), frames)
-        dev = [engine.DeviceBuffer(f) for f in y16]
-        lum = [engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]
-        for f in range(NR, frames):
-            refs = [f - k for k in range(1, NR + 1)]
-            E.launch(lum[f], [lum[r] for r in refs], lam, keys=[seed * 1000 + r for r in refs])
-            E.sync()
-            got.append(E.fetch_sb_costs())
-            ks.append(E.fetch_ref_limits())
-        for b in dev:
-            b.free()
-    fallbacks = E.stats()[1]["pruner_fallbacks"]
-    E.set_pruner(None)
-    got, ks = np.stack(got).astype(np.float64), np.stack(ks)
-    ref = j7.astype(np.float64)
-    return float((got - ref).sum() / ref.sum()), float(ks.mean()), fallbacks / ks.size
+def gather(E, seeds, keep=None):
+    """Decision logs of the clips of `seeds`, clips generated on host threads ahead of the GPU; keep
+    (a dict) receives each clip's frames."""
+    out = {}
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        futs = {s: ex.submit(lambda s=s: synth.clip_frames(clip_spec(s)[0], FRAMES, threads=4)[0]) for s in seeds}
+        for i, s in enumerate(seeds):
+            y16 = futs.pop(s).result()
+            out[s] = clip_logs(E, s, y16)
+            if keep is not None:
+                keep[s] = y16
+            if i % 10 == 0:
+                log(f"  clip {i + 1}/{len(seeds)} (seed {s}, {clip_spec(s)[2]}, qindex {clip_spec(s)[1]})")
+    return out
+
+
+def regret(costs, k):
+    """Total cost regret (fraction) and mean references when superblock s keeps k[s]."""
+    c = costs.reshape(-1, NR).astype(np.float64)
+    kk = k.reshape(-1).astype(np.int64)
+    chosen = c[np.arange(c.shape[0]), kk - 1]
+    return float((chosen - c[:, NR - 1]).sum() / c[:, NR - 1].sum()), float(kk.mean())
 
 
 def allocation(costs, mu):
@@ -106,15 +124,8 @@ def optimal_labels(costs, target):
     return allocation(costs, hi), hi
 
 
-def regret(costs, k):
-    """Total cost regret (fraction) and mean references when superblock s keeps k[s]."""
-    c = costs.reshape(-1, NR).astype(np.float64)
-    kk = k.reshape(-1).astype(np.int64)
-    chosen = c[np.arange(c.shape[0]), kk - 1]
-    return float((chosen - c[:, NR - 1]).sum() / c[:, NR - 1].sum()), float(kk.mean())
-
-
 def limits(logits, threshold, band):
+    """The engine's limit rule (celv2hip_set_pruner) on logits [positions, 6], all 7 references present."""
     s = 1.0 / (1.0 + np.exp(-logits.astype(np.float64)))
     k = np.ones(s.shape[0], np.int64)
     open_ = np.ones(s.shape[0], bool)
@@ -126,67 +137,132 @@ def limits(logits, threshold, band):
     return k, float(unsure.mean())
 
 
+def operating_point(logits, costs, target):
+    """The (threshold, band) with the fewest references whose regret on these logs is within target."""
+    best = None
+    for threshold in np.arange(0.05, 0.96, 0.025):
+        for band in (0.0, 0.025, 0.05, 0.1):
+            k, fb = limits(logits, threshold, band)
+            r, mk = regret(costs, k)
+            if r <= target and (best is None or mk < best["mean_refs"]):
+                best = {"threshold": round(float(threshold), 4), "band": band, "regret": r, "mean_refs": mk, "fallback_rate": fb}
+    return best or {"threshold": 1.0, "band": 0.0, "regret": 0.0, "mean_refs": float(NR), "fallback_rate": 0.0}
+
+
+def engine_check(E, P, op, seeds, clips):
+    """An operating point through the engine itself on clips (seed -> y16 frames): (regret against the
+    plain engine's J_7, mean limit, fallback rate)."""
+    E.set_pruner(P, op["threshold"], op["band"])
+    E.stats_reset()
+    num = den = 0.0
+    ks = []
+    for s in seeds:
+        _, q, _ = clip_spec(s)
+        lam = engine.lambda_q8(q)
+        dev = [engine.DeviceBuffer(f) for f in clips[s]]
+        lum = [engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]
+        for f in range(NR, len(dev)):
+            refs = [f - k for k in range(1, NR + 1)]
+            keys = [s * 1000 + r for r in refs]
+            E.set_pruner(None)
+            E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
+            E.sync()
+            j7 = E.fetch_sb_costs().astype(np.float64)
+            E.set_pruner(P, op["threshold"], op["band"])
+            E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
+            E.sync()
+            num += (E.fetch_sb_costs() - j7).sum()
+            den += j7.sum()
+            ks.append(E.fetch_ref_limits())
+        for b in dev:
+            b.free()
+    fallbacks = E.stats()[1]["pruner_fallbacks"]
+    E.set_pruner(None)
+    ks = np.concatenate(ks)
+    return float(num / den), float(ks.mean()), fallbacks / ks.size
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--clips", type=int, default=20)
-    ap.add_argument("--val-clips", type=int, default=4)
-    ap.add_argument("--frames", type=int, default=16)
+    ap.add_argument("--clips", type=int, default=120)
+    ap.add_argument("--val-clips", type=int, default=16)
     ap.add_argument("--label-target", type=float, default=0.004, help="the total regret of the training labels")
-    ap.add_argument("--target", type=float, default=0.004, help="largest total cost regret the operating point may have")
-    ap.add_argument("--steps", type=int, default=1500)
+    ap.add_argument("--target", type=float, default=0.004, help="largest total cost regret an operating point may have")
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=os.path.dirname(os.path.abspath(__file__)))
-    ap.add_argument("--logs", default="", help="also write the decision logs (features, costs) to this .npz")
+    ap.add_argument("--logs", default="", help="also write the held-out clips' logs (features, costs) to this .npz")
     args = ap.parse_args()
-    lam = engine.lambda_q8(QINDEX)
+    train_seeds = list(range(1000, 1000 + args.clips))
+    val_seeds = list(range(5000, 5000 + args.val_clips))
     E = engine.RawEngine(W, H)
     E.set_resident(True)
     t0 = time.perf_counter()
-    data = {}
-    for name, seeds in (("train", range(1000, 1000 + args.clips)), ("val", range(2000, 2000 + args.val_clips))):
-        fs, cs = zip(*[clip_logs(E, s, args.frames, lam) for s in seeds])
-        data[name] = (np.concatenate(fs), np.concatenate(cs))
-        log(f"{name}: {data[name][0].shape[0]} frames of decision logs ({time.perf_counter() - t0:.1f}s)")
-    xt, ct = data["train"]
-    xv, cv = data["val"]
-    if args.logs:
-        np.savez_compressed(args.logs, xt=xt, ct=ct, xv=xv, cv=cv)
-    kt, mu = optimal_labels(ct, args.label_target)
-    kv = allocation(cv, mu)
-    log(f"labels: mu {mu:.3e}, train regret {regret(ct, kt)}, k* histogram {np.bincount(kt.ravel(), minlength=8)[1:].tolist()}")
+    tl = gather(E, train_seeds)
+    log(f"train logs: {len(train_seeds)} clips in {time.perf_counter() - t0:.0f}s")
+    val_clips = {}
+    vl = gather(E, val_seeds, val_clips)
+    log(f"validation logs: {len(val_seeds)} clips in {time.perf_counter() - t0:.0f}s")
+
+    # Labels per qindex.
+    feats, labels, mus = [], [], {}
+    for q in QINDICES:
+        ss = [s for s in train_seeds if clip_spec(s)[1] == q]
+        if not ss:
+            continue
+        c = np.concatenate([tl[s][1] for s in ss])
+        k, mu = optimal_labels(c, args.label_target)
+        mus[q] = mu
+        feats.append(np.concatenate([tl[s][0] for s in ss]))
+        labels.append(k)
+        log(f"qindex {q}: {c.shape[0]} frames, mu {mu:.3e}, label regret {regret(c, k)}, k* histogram {np.bincount(k.ravel(), minlength=8)[1:].tolist()}")
+    xt, kt = np.concatenate(feats), np.concatenate(labels)
     rows, cols = xt.shape[1], xt.shape[2]
     P = engine.Pruner(max_positions=args.batch * rows * cols, seed=args.seed)
     rng = np.random.default_rng(args.seed)
     hist = []
     for step in range(args.steps):
         idx = rng.choice(xt.shape[0], args.batch, replace=False)
-        lr = args.lr * (0.1 + 0.9 * 0.5 * (1 + np.cos(np.pi * step / args.steps)))
-        loss = P.train_step(xt[idx], kt[idx].reshape(args.batch, -1), lr, 1e-4)
-        hist.append(loss)
-        if step % 250 == 0 or step == args.steps - 1:
-            log(f"step {step}: loss {np.mean(hist[-50:]):.4f}")
-    logits = np.concatenate([P.forward(xv[i:i + args.batch]) for i in range(0, xv.shape[0], args.batch)]).reshape(-1, 6)
-    best = None
-    for threshold in np.arange(0.10, 0.91, 0.05):
-        for band in (0.0, 0.05, 0.1, 0.15):
-            k, fb = limits(logits, threshold, band)
-            r, mk = regret(cv, k)
-            if r <= args.target and (best is None or mk < best["val_mean_refs"]):
-                best = {"threshold": round(float(threshold), 3), "band": band, "val_regret": r, "val_mean_refs": mk,
-                        "val_fallback_rate": fb}
-    nearest = {k: regret(cv, np.full(cv.shape[:2], k))[0] for k in range(1, NR + 1)}
-    label_regret, label_mean = regret(cv, kv)
-    cfg = dict(best or {"threshold": 1.0, "band": 0.0}, label_target=args.label_target, label_mu=mu, target=args.target, steps=args.steps,
-               train_frames=int(xt.shape[0]), val_frames=int(xv.shape[0]), final_loss=float(np.mean(hist[-50:])),
-               first_loss=float(np.mean(hist[:50])), val_label_regret=label_regret, val_label_mean_refs=label_mean,
-               val_nearest_k_regret={str(k): v for k, v in nearest.items()}, qindex=QINDEX, workload="1 (1920x1080 10-bit)",
-               train_seeds=[1000, 1000 + args.clips - 1], val_seeds=[2000, 2000 + args.val_clips - 1])
-    if best is not None:
-        r, mk, fb = engine_check(E, P, best, range(2000, 2000 + args.val_clips), args.frames, lam, cv[..., NR - 1])
-        cfg.update(engine_val_regret=r, engine_val_mean_refs=mk, engine_val_fallback_rate=fb)
+        lr = args.lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * step / args.steps)))
+        hist.append(P.train_step(xt[idx], kt[idx].reshape(args.batch, -1), lr, 1e-4))
+        if step % 500 == 0 or step == args.steps - 1:
+            log(f"step {step}: loss {np.mean(hist[-100:]):.4f}")
+
+    # Operating points per qindex on the held-out clips, then through the engine.
+    ops, report = {}, {}
+    for q in QINDICES:
+        ss = [s for s in val_seeds if clip_spec(s)[1] == q]
+        if not ss:
+            continue
+        xv = np.concatenate([vl[s][0] for s in ss])
+        cv = np.concatenate([vl[s][1] for s in ss])
+        logits = np.concatenate([P.forward(xv[i:i + args.batch]) for i in range(0, xv.shape[0], args.batch)]).reshape(-1, 6)
+        op = operating_point(logits, cv, args.target)
+        r, mk, fb = engine_check(E, P, op, ss, val_clips)
+        kv = allocation(cv, mus[q]) if q in mus else None
+        nearest = {k: regret(cv, np.full(cv.shape[:2], k))[0] for k in range(1, NR + 1)}
+        # The rule without the network: the nearest k references everywhere, k the fewest within target.
+        k_plain = min(k for k in range(1, NR + 1) if nearest[k] <= args.target)
+        ops[str(q)] = {"threshold": op["threshold"], "band": op["band"]}
+        report[str(q)] = {"val_frames": int(cv.shape[0]), "regret": r, "mean_refs": mk, "fallback_rate": fb,
+                          "simulated_regret": op["regret"], "simulated_mean_refs": op["mean_refs"],
+                          "label_regret": regret(cv, kv)[0] if kv is not None else None,
+                          "label_mean_refs": regret(cv, kv)[1] if kv is not None else None,
+                          "nearest_k_regret": {str(k): v for k, v in nearest.items()},
+                          "nearest_k_within_target": k_plain}
+        log(f"qindex {q}: {json.dumps(ops[str(q)])} engine regret {r:.5f}, mean limit {mk:.3f} (nearest-k within target: {k_plain})")
+        if args.logs and q == 130:
+            np.savez_compressed(args.logs, x=xv.astype(np.float16), c=cv.astype(np.int64))
     E.close()
+    cfg = {"operating_points": ops, "validation": report, "target": args.target, "label_target": args.label_target,
+           "label_mu": {str(q): m for q, m in mus.items()}, "steps": args.steps, "batch": args.batch, "lr": args.lr,
+           "train_frames": int(xt.shape[0]), "train_superblocks": int(xt.shape[0] * rows * cols),
+           "first_loss": float(np.mean(hist[:100])), "final_loss": float(np.mean(hist[-100:])),
+           "train_seeds": [train_seeds[0], train_seeds[-1]], "val_seeds": [val_seeds[0], val_seeds[-1]],
+           "clips": "1920x1080 10-bit, 32 frames; every fifth seed cel (workload 4), the rest natural (workload 1) "
+                    "with grain sigma 0-12, speed and pan drawn per seed; qindex = QINDICES[seed % 4]"}
     os.makedirs(args.out, exist_ok=True)
     P.params().astype("<f4").tofile(os.path.join(args.out, "weights.f32"))
     with open(os.path.join(args.out, "pruner.json"), "w") as f:

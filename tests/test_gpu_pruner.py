@@ -101,8 +101,9 @@ def test_engine_uses_the_network_limits(gpu):
     p.close()
 
 
-def cpu_features(src2, costs, sb_cols, sb_rows):
-    """The features from the level-2 source and each reference's level-2 block costs [r][rows2, cols2]."""
+def cpu_features(src2, costs, sb_cols, sb_rows, lam):
+    """The features from the level-2 source, each reference's level-2 block costs [r][rows2, cols2]
+    and the frame's lambda."""
     n = len(costs)
     rows2, cols2 = costs[0].shape
     h2, w2 = src2.shape
@@ -126,6 +127,7 @@ def cpu_features(src2, costs, sb_cols, sb_rows):
             f[7] = mean / 255
             f[8] = np.log2(1 + max(0.0, (v * v).mean() - mean * mean)) / 16
             f[9] = np.log2(1 + 16 * (np.abs(right - v) + np.abs(down - v)).sum() / 512) / 16
+            f[10] = np.log2(1 + lam) / 32
     return out
 
 
@@ -138,7 +140,8 @@ def test_features_match_the_coarse_search(gpu, w, h, nrefs):
     refs = list(range(nrefs - 1, -1, -1))
     e = gpu.RawEngine(w, h)
     e.set_features(True)
-    e.launch(gpu.RawEngine.luma(y8[nrefs]), [gpu.RawEngine.luma(y8[r]) for r in refs], gpu.lambda_q8(130), keys=refs)
+    lam = gpu.lambda_q8(130)
+    e.launch(gpu.RawEngine.luma(y8[nrefs]), [gpu.RawEngine.luma(y8[r]) for r in refs], lam, keys=refs)
     e.collect()
     got = e.fetch_features()
     src2 = np.ascontiguousarray(e.fetch_plane(-1, 2))
@@ -156,7 +159,7 @@ def test_features_match_the_coarse_search(gpu, w, h, nrefs):
                 c[by, bx] = O.orc_search_cell(src2.ctypes.data, ref2.ctypes.data, None, w2, h2, bx, by, pmx.ctypes.data, pmy.ctypes.data,
                                               pmx.shape[1], pmx.shape[0], 0, mv.ctypes.data)
         costs.append(c)
-    want = cpu_features(src2, costs, e.sb_cols, e.sb_rows)
+    want = cpu_features(src2, costs, e.sb_cols, e.sb_rows, lam)
     e.close()
     assert np.all(got[..., nrefs:7] == 0)
     err = np.abs(got - want).max(axis=(0, 1))
