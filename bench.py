@@ -158,12 +158,14 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
     fallback rate."""
     lam = engine.lambda_q8(qindex)
     w, cfg = pruner.load()
-    threshold, band = pruner.operating_point(cfg, qindex)
+    op = pruner.operating_point(cfg, qindex)
+    threshold, band = op if op else (None, None)
     P = engine.Pruner(max_positions=(1920 // 64 + 1) * (1080 // 64 + 1), seed=1)
     P.set_params(w)
     E = engine.RawEngine(1920, 1080)
     E.set_resident(True)
-    E.set_pruner(P, threshold, band)
+    if op:
+        E.set_pruner(P, threshold, band)
 
     def step(s):
         i = decidable[s % len(decidable)]
@@ -218,6 +220,157 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
             "network": f"conv3x3 11-32, conv3x3 32-32, conv1x1 32-6, BF16 on MFMA, trained on {cfg['train_superblocks']} superblocks of synthetic clips (celv2_amd/pruner/train.py)"}
 
 
+def _timed(step, engines, prepare, steps, warmup):
+    """Seconds for `steps` calls of step(s) after `warmup` untimed ones (every engine synchronized on
+    both sides); prepare() first."""
+    prepare()
+    for s in range(warmup):
+        step(s)
+    for e in engines:
+        e.sync()
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        step(s)
+    for e in engines:
+        e.sync()
+    return time.perf_counter() - t0
+
+
+def _regret(plain, learned, launch, frames):
+    """Cost regret of the learned engines against the plain ones over frames (launch(engine, i) runs
+    one frame), the mean reference limit and the fallback rate."""
+    jp = jl = 0.0
+    ks = []
+    for e in learned:
+        e.stats_reset()
+    for i in frames:
+        for ep, el in zip(plain, learned):
+            launch(ep, i)
+            launch(el, i)
+            ep.sync()
+            el.sync()
+            jp += ep.fetch_sb_costs().astype(np.float64).sum()
+            jl += el.fetch_sb_costs().astype(np.float64).sum()
+            ks.append(el.fetch_ref_limits())
+    fb = sum(e.stats()[1]["pruner_fallbacks"] for e in learned)
+    ks = np.concatenate(ks)
+    return jl / jp - 1, float(ks.mean()), fb / ks.size
+
+
+def bench_qindex_sweep(lum, decidable, NR, steps, warmup):
+    """The plain and learned forms of the 1080p workload at qindex 100, 130, 160 and 190 (the learned
+    form at each qindex's operating point): frames/s of each over the same steps, and the learned
+    form's cost regret, mean reference limit and fallback rate over every decidable frame."""
+    w, cfg = pruner.load()
+    P = engine.Pruner(max_positions=(1920 // 64 + 1) * (1080 // 64 + 1), seed=1)
+    P.set_params(w)
+    Ep, El = engine.RawEngine(1920, 1080), engine.RawEngine(1920, 1080)
+    for e in (Ep, El):
+        e.set_resident(True)
+    out = {}
+    for q in (100, 130, 160, 190):
+        lam = engine.lambda_q8(q)
+        op = pruner.operating_point(cfg, q)
+        El.set_pruner(P, *op) if op else El.set_pruner(None)
+
+        def launch(e, i):
+            refs = [i - k for k in range(1, NR + 1)]
+            e.launch(lum[i], [lum[r] for r in refs], lam, keys=refs)
+
+        def stepper(e):
+            def step(s):
+                i, nxt = decidable[s % len(decidable)], decidable[(s + 1) % len(decidable)]
+                refs = [i - k for k in range(1, NR + 1)]
+                e.launch(lum[i], [lum[r] for r in refs], lam, keys=refs, src_key=i)
+                e.prepare(lum[nxt], nxt)
+            return step
+
+        first = lambda e: (lambda: e.prepare(lum[decidable[0]], decidable[0]))
+        tp = _timed(stepper(Ep), [Ep], first(Ep), steps, warmup)
+        tl = _timed(stepper(El), [El], first(El), steps, warmup)
+        r, mk, fb = _regret([Ep], [El], launch, decidable)
+        out[str(q)] = {"operating_point": op, "plain_frames_per_s": round(steps / tp, 2), "learned_frames_per_s": round(steps / tl, 2),
+                       "speedup": round(tp / tl, 3), "cost_regret": round(r, 6), "mean_reference_limit": round(mk, 3),
+                       "fallback_rate": round(fb, 5)}
+        log(f"qindex {q}: plain {steps / tp:.1f} frames/s, learned {steps / tl:.1f} ({tp / tl:.2f}x), regret {r * 100:.3f}%, mean limit {mk:.2f}")
+    for e in (Ep, El):
+        e.close()
+    P.close()
+    return out
+
+
+def bench_cel(NR, qindex, steps=20, warmup=3, nf=48):
+    """Two 3840x2160 10-bit streams of flat-shaded cel content (workload 4: Voronoi cells of a
+    16-colour palette, outlines, cuts at frames 16 and 32, motion held every other frame, grain
+    sigma 1), 48 frames each, 7 references. Timed: the plain engines launched one by one, then the
+    learned engines with both streams in one shared launch (celv2hip_launch_batch); the learned form's
+    cost regret against plain over every decidable frame of both streams."""
+    t = time.perf_counter()
+    W4, H4 = 3840, 2160
+    streams = [[engine.DeviceBuffer(f) for f in synth.clip_frames(synth.params(4, seed, W4, H4), nf)[0]] for seed in (1, 2)]
+    lums = [[engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev] for dev in streams]
+    log(f"cel workload: 2 streams x {nf} frames 3840x2160 in {time.perf_counter() - t:.1f}s")
+    w, cfg = pruner.load()
+    P = engine.Pruner(max_positions=(W4 // 64 + 1) * (H4 // 64 + 1), seed=1)
+    P.set_params(w)
+    lam = engine.lambda_q8(qindex)
+    plain = [engine.RawEngine(W4, H4) for _ in streams]
+    learned = [engine.RawEngine(W4, H4) for _ in streams]
+    for e in plain + learned:
+        e.set_resident(True)
+    op = pruner.operating_point(cfg, qindex)
+    for e in learned:
+        if op:
+            e.set_pruner(P, *op)
+    decidable = list(range(NR, nf))
+
+    def frame(s):
+        return decidable[s % len(decidable)], decidable[(s + 1) % len(decidable)]
+
+    def prep(engines):
+        return lambda: [e.prepare(lum[decidable[0]], decidable[0]) for e, lum in zip(engines, lums)]
+
+    def step_plain(s):
+        i, nxt = frame(s)
+        refs = [i - k for k in range(1, NR + 1)]
+        for e, lum in zip(plain, lums):
+            e.launch(lum[i], [lum[r] for r in refs], lam, keys=refs, src_key=i)
+            e.prepare(lum[nxt], nxt)
+
+    def step_learned(s):
+        i, nxt = frame(s)
+        refs = [i - k for k in range(1, NR + 1)]
+        engine.RawEngine.launch_batch(learned, [lum[i] for lum in lums], [[lum[r] for r in refs] for lum in lums],
+                                      [lam] * 2, keys=[refs] * 2, src_keys=[i] * 2)
+        for e, lum in zip(learned, lums):
+            e.prepare(lum[nxt], nxt)
+
+    tp = _timed(step_plain, plain, prep(plain), steps, warmup)
+    tl = _timed(step_learned, learned, prep(learned), steps, warmup)
+
+    def launch(e, i):
+        lum = lums[(plain + learned).index(e) % 2]
+        refs = [i - k for k in range(1, NR + 1)]
+        e.launch(lum[i], [lum[r] for r in refs], lam, keys=refs)
+
+    r, mk, fb = _regret(plain, learned, launch, decidable)
+    for e in plain + learned:
+        e.close()
+    P.close()
+    for dev in streams:
+        for b in dev:
+            b.free()
+    sbs = 2 * (W4 // 64) * ((H4 + 63) // 64)
+    log(f"cel workload: plain {2 * steps / tp:.1f} frames/s, learned + batched {2 * steps / tl:.1f} ({tp / tl:.2f}x), "
+        f"regret {r * 100:.3f}%, mean limit {mk:.2f}")
+    return {"workload": "2 streams of 3840x2160 10-bit cel content (workload 4), 48 frames, 7 references, qindex "
+                        f"{qindex}; learned form: both streams in one launch", "steps": steps,
+            "plain_frames_per_s": round(2 * steps / tp, 2), "learned_batched_frames_per_s": round(2 * steps / tl, 2),
+            "superblocks_per_step": sbs, "speedup": round(tp / tl, 3), "cost_regret": round(r, 6),
+            "mean_reference_limit": round(mk, 3), "fallback_rate": round(fb, 5),
+            "bd_rate": "pending (needs the full encoder and AVM)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -230,6 +383,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the mixed-resolution batched form")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned form")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the qindex sweep of the learned form")
+    ap.add_argument("--no-cel", action="store_true", help="skip the 4K cel workload (learned + batched)")
     ap.add_argument("--e2e-steps", type=int, default=12)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
@@ -350,6 +505,9 @@ def main():
         learned = bench_learned(lum, decidable, NR, args.qindex, args.steps, args.warmup, fps)
         learned["scored_candidates_per_frame"] = round(sbs * 420 * 13 + learned["inter_candidates_per_frame"] + learned["compound_blocks_per_frame"])
         learned["scored_candidates_per_s"] = round(learned["scored_candidates_per_frame"] * learned["frames_per_s"])
+        if not args.no_sweep:
+            learned["qindex_sweep"] = bench_qindex_sweep(lum, decidable, NR, args.steps, args.warmup)
+    cel = None if args.no_cel or args.no_learned or NR != 7 else bench_cel(NR, args.qindex)
 
     cpu = None
     parity = None
@@ -401,7 +559,7 @@ def main():
         "all_kernels_tops": round(total_ops * fps / 1e12, 3),
         "engine_path_fps_host_buffers": e2e,
         "forms": {"plain": {"frames_per_s": round(fps, 2), "scored_candidates_per_s": round(cands * fps)},
-                  "batched": batched, "learned": learned},
+                  "batched": batched, "learned": learned, "learned_batched_cel_4k": cel},
     }
     print(json.dumps(out), flush=True)
 

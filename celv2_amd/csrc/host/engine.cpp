@@ -149,6 +149,7 @@ struct celv2hip_engine {
   const celv2hip_pruner* pruner = nullptr;
   float prune_threshold = 0.5f, prune_band = 0.f;
   bool want_features = false, features_valid = false;
+  bool limits_last = false;  // the last launch obeyed per-superblock reference limits (kmap)
   uint32_t* cost2[kMaxRefs] = {};  // level-2 best search cost per block and reference
   CnnWork infer;                   // features (infer.x) and the network's activations, one frame
   void* rec_dev = nullptr;   // celv2_sb per superblock
@@ -578,8 +579,10 @@ long long celv2hip_fetch_ref_limits(celv2hip_engine* e, uint8_t* out, long long 
   const int rc = guarded([&] {
     n = static_cast<long long>(e->nsb());
     if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "limit buffer too small");
+    if (!e->launched) throw Failure(CELV2_ERR_STATE, "no launch");
     e->sync_all();
-    check(hipMemcpy(out, e->kmap, static_cast<size_t>(n), hipMemcpyDeviceToHost), "hipMemcpy kmap");
+    if (e->limits_last) check(hipMemcpy(out, e->kmap, static_cast<size_t>(n), hipMemcpyDeviceToHost), "hipMemcpy kmap");
+    else std::fill(out, out + n, static_cast<uint8_t>(e->nrefs_last));  // every superblock kept them all
   });
   return rc ? rc : n;
 }
@@ -883,6 +886,7 @@ void launch_batch(const LaunchIn* in, int n) {
     check(hipEventRecord(e->ev[5], S), "hipEventRecord");
     e->release(*cur[i], S);
     e->nrefs_last = in[i].nrefs;
+    e->limits_last = use_kmap[i] != 0;
     e->last_refs = used[i];
     e->launched = true;
   }

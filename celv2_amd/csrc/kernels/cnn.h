@@ -9,10 +9,10 @@
 
 namespace celv2hip {
 
-constexpr int kFeatures = 11;       // inputs per superblock (pruner.hip)
+constexpr int kFeatures = 14;       // inputs per superblock (pruner.hip)
 constexpr int kHidden = 32;         // channels of the two 3x3 layers
 constexpr int kOut = 6;             // output j: needs more than j + 1 references
-constexpr int kK1 = 112;            // im2col row of layer 1: 9 * kFeatures = 99, padded
+constexpr int kK1 = 128;            // im2col row of layer 1: 9 * kFeatures = 126, padded
 constexpr int kK2 = 9 * kHidden;    // im2col row of layer 2
 // Parameters: W1[9 kFeatures][kHidden], b1, W2[kK2][kHidden], b2, W3[kHidden][kOut], b3 (row k of a
 // weight matrix is im2col column k: k = tap * channels + channel, tap = (dy + 1) * 3 + dx + 1).

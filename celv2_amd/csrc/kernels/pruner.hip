@@ -8,6 +8,8 @@
 //   f[8]     log2(1 + the source variance) / 16
 //   f[9]     log2(1 + 16 x the mean absolute horizontal + vertical difference) / 16
 //   f[10]    log2(1 + lambda_q8) / 32 (the rate multiplier: how much a reference's extra bits weigh)
+//   f[11..13] the frame's means of f[0], f[8] and f[1] + ... + f[6] (the content's kind: flat-shaded
+//             or natural, and how much further references gain across the frame)
 // The gains are differences taken before rounding, so the network sees them at full relative
 // precision in BF16. Positions beyond the frame read the clamped plane, as every kernel does.
 #include "common.h"
@@ -77,11 +79,38 @@ __global__ void __launch_bounds__(256) k_features(FeatureArgs a) {
   }
 }
 
+// The frame context f[11..13]: one workgroup sums the three per-superblock quantities in a fixed
+// order (a strided pass, then a tree), so the means are the same on every run.
+__global__ void __launch_bounds__(256) k_frame_context(float* feats, int n) {
+  __shared__ float part[3][256];
+  const int tid = static_cast<int>(threadIdx.x);
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+  for (int sb = tid; sb < n; sb += 256) {
+    const float* f = feats + static_cast<size_t>(sb) * kFeatures;
+    s0 += f[0], s1 += f[8], s2 += f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
+  }
+  part[0][tid] = s0, part[1][tid] = s1, part[2][tid] = s2;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if (tid < w)
+      for (int c = 0; c < 3; c++) part[c][tid] += part[c][tid + w];
+    __syncthreads();
+  }
+  const float inv = 1.f / static_cast<float>(n);
+  for (int sb = tid; sb < n; sb += 256) {
+    float* f = feats + static_cast<size_t>(sb) * kFeatures;
+    f[11] = part[0][0] * inv, f[12] = part[1][0] * inv, f[13] = part[2][0] * inv;
+  }
+}
+
 }  // namespace
 
 void launch_features(const FeatureArgs& a, hipStream_t s) {
   const int n = a.sb_cols * a.sb_rows;
-  if (n > 0) k_features<<<(n + 3) / 4, 256, 0, s>>>(a);
+  if (n > 0) {
+    k_features<<<(n + 3) / 4, 256, 0, s>>>(a);
+    k_frame_context<<<1, 256, 0, s>>>(a.feats, n);
+  }
 }
 
 }  // namespace celv2hip

@@ -20,7 +20,7 @@ _LIB = None
 P = ctypes.c_void_p
 I = ctypes.c_int
 LL = ctypes.c_longlong
-FEATURES = 11  # the learned pruner's inputs per superblock (include/celv2hip.h)
+FEATURES = 14  # the learned pruner's inputs per superblock (include/celv2hip.h)
 
 
 class Luma(ctypes.Structure):

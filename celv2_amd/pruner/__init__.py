@@ -16,7 +16,8 @@ def load(directory=HERE):
 
 
 def operating_point(cfg, qindex):
-    """(threshold, band) chosen for the trained qindex nearest to `qindex`."""
+    """(threshold, band) chosen for the trained qindex nearest to `qindex`, or None where no threshold
+    keeps every held-out clip within the budget short of keeping all references (run plain there)."""
     ops = cfg["operating_points"]
     q = min(ops, key=lambda k: abs(int(k) - qindex))
-    return ops[q]["threshold"], ops[q]["band"]
+    return (ops[q]["threshold"], ops[q]["band"]) if ops[q] else None

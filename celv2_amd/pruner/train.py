@@ -9,15 +9,16 @@ J_k (celv2hip_fetch_sb_costs) and, at k = 7, the pruner's features (celv2hip_fet
 superblock's decision depends only on its own limit, so the total cost of any per-superblock
 allocation k(s) follows from these logs exactly.
 
-Labels: per qindex, the allocation that keeps the fewest references for a given total regret
-sum(J_k(s) - J_7) / sum(J_7): with a multiplier mu, k*(s) = argmin_k k + mu (J_k(s) - J_7(s)), mu set
-by bisection so the training clips' label regret at that qindex equals the target. The network
-(BF16 on the matrix cores, celv2hip_pruner_*) learns P(k* > j + 1) with AdamW on the GPU.
+The budget is per sequence (a BD-rate is measured on one sequence): a clip's cost regret is
+sum(J_k(s) - J_7) / sum(J_7) over its superblocks. Labels: per clip, the allocation that keeps the
+fewest references within the target regret: with a multiplier mu, k*(s) = argmin_k k + mu (J_k(s) -
+J_7(s)), mu set by bisection so the clip's label regret equals the target. The network (BF16 on the
+matrix cores, celv2hip_pruner_*) learns P(k* > j + 1) with AdamW on the GPU.
 
 Operating points: on held-out clips, per qindex, the (threshold, fallback band) with the fewest
-references per superblock whose regret stays within the target, measured again through the engine
-itself (the network run inside the launch). Writes weights.f32 (the FP32 parameters) and
-pruner.json.
+references per superblock whose regret on every held-out clip of that qindex stays within the
+target, measured again through the engine itself (the network run inside the launch). Writes
+weights.f32 (the FP32 parameters) and pruner.json.
 
     python -m celv2_amd.pruner.train [--clips 120] [--val-clips 16] [--steps 6000] [--out DIR]
 """
@@ -138,25 +139,30 @@ def limits(logits, threshold, band):
 
 
 def operating_point(logits, costs, target):
-    """The (threshold, band) with the fewest references whose regret on these logs is within target."""
+    """The (threshold, band) with the fewest references whose regret on every clip is within target;
+    logits and costs are lists over clips."""
     best = None
-    for threshold in np.arange(0.05, 0.96, 0.025):
+    for threshold in np.concatenate([[0.005, 0.01, 0.02, 0.03, 0.04], np.arange(0.05, 0.96, 0.025)]):
         for band in (0.0, 0.025, 0.05, 0.1):
-            k, fb = limits(logits, threshold, band)
-            r, mk = regret(costs, k)
-            if r <= target and (best is None or mk < best["mean_refs"]):
-                best = {"threshold": round(float(threshold), 4), "band": band, "regret": r, "mean_refs": mk, "fallback_rate": fb}
+            ks = [limits(z, threshold, band) for z in logits]
+            rs = [regret(c, k)[0] for c, (k, _) in zip(costs, ks)]
+            n = sum(k.size for k, _ in ks)
+            mk = sum(float(k.sum()) for k, _ in ks) / n
+            fb = sum(f * k.size for k, f in ks) / n
+            if max(rs) <= target and (best is None or mk < best["mean_refs"]):
+                best = {"threshold": round(float(threshold), 4), "band": band, "regret": max(rs), "mean_refs": mk, "fallback_rate": fb}
     return best or {"threshold": 1.0, "band": 0.0, "regret": 0.0, "mean_refs": float(NR), "fallback_rate": 0.0}
 
 
 def engine_check(E, P, op, seeds, clips):
-    """An operating point through the engine itself on clips (seed -> y16 frames): (regret against the
-    plain engine's J_7, mean limit, fallback rate)."""
+    """An operating point through the engine itself on clips (seed -> y16 frames): (the largest clip
+    regret against the plain engine's J_7, mean limit, fallback rate)."""
     E.set_pruner(P, op["threshold"], op["band"])
     E.stats_reset()
-    num = den = 0.0
+    worst = 0.0
     ks = []
     for s in seeds:
+        num = den = 0.0
         _, q, _ = clip_spec(s)
         lam = engine.lambda_q8(q)
         dev = [engine.DeviceBuffer(f) for f in clips[s]]
@@ -176,10 +182,11 @@ def engine_check(E, P, op, seeds, clips):
             ks.append(E.fetch_ref_limits())
         for b in dev:
             b.free()
+        worst = max(worst, float(num / den))
     fallbacks = E.stats()[1]["pruner_fallbacks"]
     E.set_pruner(None)
     ks = np.concatenate(ks)
-    return float(num / den), float(ks.mean()), fallbacks / ks.size
+    return worst, float(ks.mean()), fallbacks / ks.size
 
 
 def main():
@@ -206,19 +213,14 @@ def main():
     vl = gather(E, val_seeds, val_clips)
     log(f"validation logs: {len(val_seeds)} clips in {time.perf_counter() - t0:.0f}s")
 
-    # Labels per qindex.
-    feats, labels, mus = [], [], {}
+    # Labels per clip.
+    labels = {s: optimal_labels(tl[s][1], args.label_target)[0] for s in train_seeds}
     for q in QINDICES:
         ss = [s for s in train_seeds if clip_spec(s)[1] == q]
-        if not ss:
-            continue
-        c = np.concatenate([tl[s][1] for s in ss])
-        k, mu = optimal_labels(c, args.label_target)
-        mus[q] = mu
-        feats.append(np.concatenate([tl[s][0] for s in ss]))
-        labels.append(k)
-        log(f"qindex {q}: {c.shape[0]} frames, mu {mu:.3e}, label regret {regret(c, k)}, k* histogram {np.bincount(k.ravel(), minlength=8)[1:].tolist()}")
-    xt, kt = np.concatenate(feats), np.concatenate(labels)
+        k = np.concatenate([labels[s] for s in ss]) if ss else np.zeros(0, np.uint8)
+        log(f"qindex {q}: {len(ss)} clips, label mean refs {k.mean():.3f}, k* histogram {np.bincount(k.ravel(), minlength=8)[1:].tolist()}")
+    xt = np.concatenate([tl[s][0] for s in train_seeds])
+    kt = np.concatenate([labels[s] for s in train_seeds])
     rows, cols = xt.shape[1], xt.shape[2]
     P = engine.Pruner(max_positions=args.batch * rows * cols, seed=args.seed)
     rng = np.random.default_rng(args.seed)
@@ -236,28 +238,32 @@ def main():
         ss = [s for s in val_seeds if clip_spec(s)[1] == q]
         if not ss:
             continue
-        xv = np.concatenate([vl[s][0] for s in ss])
-        cv = np.concatenate([vl[s][1] for s in ss])
-        logits = np.concatenate([P.forward(xv[i:i + args.batch]) for i in range(0, xv.shape[0], args.batch)]).reshape(-1, 6)
-        op = operating_point(logits, cv, args.target)
+        logits = [np.concatenate([P.forward(vl[s][0][i:i + args.batch]) for i in range(0, vl[s][0].shape[0], args.batch)]).reshape(-1, 6)
+                  for s in ss]
+        costs = [vl[s][1] for s in ss]
+        op = operating_point(logits, costs, args.target)
         r, mk, fb = engine_check(E, P, op, ss, val_clips)
-        kv = allocation(cv, mus[q]) if q in mus else None
-        nearest = {k: regret(cv, np.full(cv.shape[:2], k))[0] for k in range(1, NR + 1)}
-        # The rule without the network: the nearest k references everywhere, k the fewest within target.
+        best = [optimal_labels(c, args.target)[0] for c in costs]
+        # The rule without the network: the nearest k references everywhere, k the fewest within
+        # target on every clip.
+        nearest = {k: max(regret(c, np.full(c.shape[:2], k))[0] for c in costs) for k in range(1, NR + 1)}
         k_plain = min(k for k in range(1, NR + 1) if nearest[k] <= args.target)
-        ops[str(q)] = {"threshold": op["threshold"], "band": op["band"]}
-        report[str(q)] = {"val_frames": int(cv.shape[0]), "regret": r, "mean_refs": mk, "fallback_rate": fb,
-                          "simulated_regret": op["regret"], "simulated_mean_refs": op["mean_refs"],
-                          "label_regret": regret(cv, kv)[0] if kv is not None else None,
-                          "label_mean_refs": regret(cv, kv)[1] if kv is not None else None,
-                          "nearest_k_regret": {str(k): v for k, v in nearest.items()},
+        # An operating point that keeps every reference everywhere is no pruning: the plain engine.
+        ops[str(q)] = {"threshold": op["threshold"], "band": op["band"]} if mk < NR - 0.05 else None
+        report[str(q)] = {"val_clips": len(ss), "val_frames": int(sum(c.shape[0] for c in costs)),
+                          "worst_clip_regret": r, "mean_refs": mk, "fallback_rate": fb,
+                          "simulated_worst_clip_regret": op["regret"], "simulated_mean_refs": op["mean_refs"],
+                          "best_allocation_mean_refs": float(np.mean(np.concatenate([b.ravel() for b in best]))),
+                          "nearest_k_worst_clip_regret": {str(k): v for k, v in nearest.items()},
                           "nearest_k_within_target": k_plain}
-        log(f"qindex {q}: {json.dumps(ops[str(q)])} engine regret {r:.5f}, mean limit {mk:.3f} (nearest-k within target: {k_plain})")
+        log(f"qindex {q}: {json.dumps(ops[str(q)])} worst clip regret {r:.5f}, mean limit {mk:.3f} "
+            f"(best allocation {report[str(q)]['best_allocation_mean_refs']:.3f}, nearest-k within target: {k_plain})")
         if args.logs and q == 130:
-            np.savez_compressed(args.logs, x=xv.astype(np.float16), c=cv.astype(np.int64))
+            np.savez_compressed(args.logs, x=np.concatenate([vl[s][0] for s in ss]).astype(np.float16),
+                                c=np.concatenate(costs).astype(np.int64))
     E.close()
     cfg = {"operating_points": ops, "validation": report, "target": args.target, "label_target": args.label_target,
-           "label_mu": {str(q): m for q, m in mus.items()}, "steps": args.steps, "batch": args.batch, "lr": args.lr,
+           "budget": "per clip: sum(J_learned - J_7) / sum(J_7) over the clip's superblocks", "steps": args.steps, "batch": args.batch, "lr": args.lr,
            "train_frames": int(xt.shape[0]), "train_superblocks": int(xt.shape[0] * rows * cols),
            "first_loss": float(np.mean(hist[:100])), "final_loss": float(np.mean(hist[-100:])),
            "train_seeds": [train_seeds[0], train_seeds[-1]], "val_seeds": [val_seeds[0], val_seeds[-1]],

@@ -166,16 +166,16 @@ int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count);
 long long celv2hip_fetch_sb_costs(celv2hip_engine* e, int64_t* out, long long cap);
 
 /* ---- The learned pruner (the pruned form; no counterpart in the reference's DecisionEngine) ------
- * A small network over a frame's superblock grid: inputs are CELV2HIP_PRUNER_FEATURES (11) features
- * per superblock from the source's level 2, each reference's level-2 motion search and the frame's
- * lambda; output j (0..5) is the logit of "the
+ * A small network over a frame's superblock grid: inputs are CELV2HIP_PRUNER_FEATURES (14) features
+ * per superblock from the source's level 2, each reference's level-2 motion search, the frame's
+ * lambda and frame-wide means; output j (0..5) is the logit of "the
  * superblock needs more than its nearest j + 1 references". BF16 operands on the matrix cores, FP32
  * accumulation and master weights, AdamW. Attached to an engine, every launch computes the features
  * after motion-search level 2, runs the network and sets each superblock's reference limit (as
  * celv2hip_set_ref_limits) before levels 1 and 0, inter scores and finalize. A pruner may be
  * attached to several engines; it must not be trained while their launches run. */
 typedef struct celv2hip_pruner celv2hip_pruner;
-#define CELV2HIP_PRUNER_FEATURES 11
+#define CELV2HIP_PRUNER_FEATURES 14
 
 /* Number of FP32 parameters of the network. */
 int celv2hip_pruner_param_count(void);
@@ -185,7 +185,7 @@ int celv2hip_pruner_destroy(celv2hip_pruner* p);
 int celv2hip_pruner_get_params(celv2hip_pruner* p, float* out, int cap);
 /* Loads parameters (n = param count) and resets the AdamW state. */
 int celv2hip_pruner_set_params(celv2hip_pruner* p, const float* in, int n);
-/* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 11 + i]: the 6 logits per superblock. */
+/* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 14 + i]: the 6 logits per superblock. */
 int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
 /* Mean binary cross-entropy against labels kstar (the references each superblock needs, 1..7) and its
  * gradient with respect to every parameter, without updating them. */
@@ -200,9 +200,10 @@ int celv2hip_pruner_train_step(celv2hip_pruner* p, const float* feats, const uin
 int celv2hip_set_pruner(celv2hip_engine* e, const celv2hip_pruner* p, float threshold, float band);
 /* on != 0: launches compute the pruner's features even without a pruner (training data). */
 int celv2hip_set_features(celv2hip_engine* e, int on);
-/* The last launch's features (superblocks x 11 floats); returns the count or a negative status. */
+/* The last launch's features (superblocks x 14 floats); returns the count or a negative status. */
 long long celv2hip_fetch_features(celv2hip_engine* e, float* out, long long cap);
-/* The reference limits the last pruned launch used (one byte per superblock). */
+/* The reference limits the last launch used (one byte per superblock; its reference count for every
+ * superblock when it had no limits). */
 long long celv2hip_fetch_ref_limits(celv2hip_engine* e, uint8_t* out, long long cap);
 
 /* on != 0: later launches count the non-repeated inter candidates they score (stats counts[4]);

@@ -4,7 +4,7 @@ rounding=True) BF16 rounding of every GEMM operand exactly where the GPU rounds 
 outputs and gradients differ from it only by FP32 accumulation order."""
 import numpy as np
 
-F, HID, OUT = 11, 32, 6
+F, HID, OUT = 14, 32, 6
 W1 = 0
 B1 = W1 + 9 * F * HID
 W2 = B1 + HID
