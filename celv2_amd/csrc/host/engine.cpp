@@ -131,6 +131,11 @@ struct celv2hip_engine {
   hipStream_t stream = nullptr;  // launches
   hipStream_t side = nullptr;    // prepare(): the next source's planes and intra scores
   hipEvent_t order_ev = nullptr;  // orders this engine's stream around batched launches
+  // Recorded before the last launch's level-0 motion search: prepare() starts the next source's
+  // intra scores there, beside the long latency-bound launches (level 0, inter, finalize) rather than
+  // beside the short coarse levels, which wait for free CU slots while intra holds them.
+  hipEvent_t deep_ev = nullptr;
+  bool deep_pending = false;
   std::string name;
   int row_streams = 1;
   bool resident = false;
@@ -210,6 +215,7 @@ struct celv2hip_engine {
     if (stream) (void)hipStreamDestroy(stream);
     if (side) (void)hipStreamDestroy(side);
     if (order_ev) (void)hipEventDestroy(order_ev);
+    if (deep_ev) (void)hipEventDestroy(deep_ev);
   }
 
   void sync_all() {
@@ -330,6 +336,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking), "hipStreamCreate");
     check(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&e->deep_ev, hipEventDisableTiming), "hipEventCreate");
     for (SrcSlot& s : e->srcs) {
       check(hipEventCreateWithFlags(&s.side_done, hipEventDisableTiming), "hipEventCreate");
       check(hipEventCreateWithFlags(&s.main_done, hipEventDisableTiming), "hipEventCreate");
@@ -647,6 +654,7 @@ int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key) 
     s.key = -1;
     e->claim(s, e->side);
     run_builds({e->build_job(*next, s.p, s.up, false, e->side)}, e->side);
+    if (e->deep_pending) check(hipStreamWaitEvent(e->side, e->deep_ev, 0), "hipStreamWaitEvent");
     run_intra({{e, &s}}, e->side);
     e->release(s, e->side);
     s.key = key;
@@ -762,6 +770,11 @@ void launch_batch(const LaunchIn* in, int n) {
       add_me_level(b, a);
       if (b.n == kMaxItems) launch_me_level(b, S), b = Batch<MeLevelArgs>();
     }
+    if (l == 0)
+      for (int i = 0; i < n; i++) {
+        check(hipEventRecord(in[i].e->deep_ev, S), "hipEventRecord");
+        in[i].e->deep_pending = true;
+      }
     launch_me_level(b, S);
     check(hipGetLastError(), "me kernel");
     if (l == 2) {
