@@ -41,11 +41,12 @@ struct MeLevelArgs {
   long long phase_bytes;
   unsigned long long* searches;  // centres searched, spread over 64 slots 16 words apart (may be null)
   // Per-superblock reference limits (may be null): superblock s uses references 0 .. kmap[s] - 1.
-  // At levels 0 and 1 a block is searched for reference r only when a superblock within one
-  // superblock of it keeps r (every field cell a kept superblock reads, and their centres' cells).
+  // At levels 0 and 1 a block is searched for reference r only when a superblock that keeps r reads
+  // it: its own, or a neighbour whose ring it lies in (1 block at level 0, 2 at level 1).
   const uint8_t* kmap;
   int sb_cols, sb_rows, level;
   uint32_t* cost_out[kMaxRefs];  // the searched best cost (SAD + distance penalty) per block (may be null)
+  float inv_nrefs, inv_cols;     // 1 / nrefs, 1 / cols (set by add_me_level)
 };
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a);
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s);

@@ -79,15 +79,16 @@ struct Opening {
 struct BlockPos {
   int r, bx, by;
 };
-__device__ __forceinline__ BlockPos block_pos(const MeLevelArgs& a, int item) {
-  const int per_ref = a.rows * a.cols;
-  const int r = item / per_ref, rem = item - r * per_ref, by = rem / a.cols;
-  return {r, rem - by * a.cols, by};
+// n / d for 0 <= n < 2^24 from a float estimate with 1 / d, corrected once (the estimate is within 1).
+__device__ __forceinline__ int div_small(int n, int d, float inv) {
+  int q = static_cast<int>(static_cast<float>(n) * inv);
+  if (q * d > n) q--;
+  else if ((q + 1) * d <= n) q++;
+  return q;
 }
 __device__ __forceinline__ bool limited(const MeLevelArgs& a, int r) { return a.kmap && a.level <= 1 && r > 0; }
 
-__device__ __forceinline__ Opening open_block(const MeLevelArgs& a, int item, int lane) {
-  const BlockPos q = block_pos(a, item);
+__device__ __forceinline__ Opening open_block(const MeLevelArgs& a, const BlockPos& q, int lane) {
   Opening o;
   if (limited(a, q.r)) {
     // A superblock's decisions read the level-0 field on it and a ring of 1 block (its inter
@@ -116,12 +117,11 @@ __device__ __forceinline__ Opening open_block(const MeLevelArgs& a, int item, in
   return o;
 }
 
-// Block `item` (reference-major: r = item / blocks per reference) of one launch item, by one wave;
-// win1 / win2 are the wave's two LDS windows.
-__device__ __forceinline__ void search_block(const MeLevelArgs& a, int item, int lane, uint32_t* win1, uint32_t* win2) {
-  const BlockPos q = block_pos(a, item);
+// Block (q.bx, q.by) against reference q.r of one launch item, by one wave; win1 / win2 are the
+// wave's two LDS windows.
+__device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPos& q, int lane, uint32_t* win1, uint32_t* win2) {
   const int r = q.r, bx = q.bx, by = q.by;
-  const Opening o = open_block(a, item, lane);
+  const Opening o = open_block(a, q, lane);
   // Searched only near a superblock that keeps reference r.
   if (limited(a, r) && __ballot(o.km > static_cast<uint32_t>(r)) == 0) return;  // whole wave
   const DPlane cur = a.cur, ref = a.ref[r];
@@ -231,7 +231,9 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, int item, int
     int qcx = 4 * mvx, qcy = 4 * mvy;
 #pragma unroll
     for (int step = 2; step >= 1; step--) {
-      uint32_t best = 0xffffffffu;
+      // The three products' rows first, then the three products back to back, then their sums
+      // (every partial sum an integer below 2^24: any order is exact).
+      uint32_t prow[3];
 #pragma unroll
       for (int cc = 0; cc < 3; cc++) {
         const int qx = qcx + odx[cc] * step, qy = qcy + ody[cc] * step;
@@ -240,16 +242,27 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, int item, int
         const int off = static_cast<int>(a.phase_bytes) * ((qy & 3) * 4 + (qx & 3)) + corner_offset(rb, Y, X);
         uint32_t d0, d1;
         bload8(rb, off & ~3, 0, d0, d1);
-        const uint32_t prow = __builtin_amdgcn_alignbyte(d1, d0, off & 3);
-        float s = abs_sum4(mfma16(nhh, row_halves(prow), ccur));
-        s += as_f32(dpp<kDppXor1>(as_u32f(s)));  // the position's 4 tiles (lanes 4o' .. 4o'+3)
-        s += as_f32(dpp<kDppXor2>(as_u32f(s)));
-        s = sum_rows16(s, s);                       // its 4 rows of coefficients
-        s = sum_halves32(s, s);
-        const int o = 4 * cc + (j >> 2);
-        const uint32_t cost = static_cast<uint32_t>(s) + (o ? kQpelPenalty : 0);
-        if (o < 9) best = min(best, (cost << 4) | static_cast<uint32_t>(o));
+        prow[cc] = __builtin_amdgcn_alignbyte(d1, d0, off & 3);
       }
+      f32x4 m[3];
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) m[cc] = mfma16(nhh, row_halves(prow[cc]), ccur);
+      float sv[3];
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) {
+        float v = abs_sum4(m[cc]);
+        v += as_f32(dpp<kDppXor1>(as_u32f(v)));  // the position's 4 tiles (lanes 4o' .. 4o'+3)
+        sv[cc] = v + as_f32(dpp<kDppXor2>(as_u32f(v)));
+      }
+      // Its 4 rows of coefficients: products 0 and 1 in one pass (rows 0 and 2 of the result hold
+      // product 0's totals, rows 1 and 3 product 1's), product 2 in another.
+      const float s01 = sum_halves32(sum_rows16(sv[0], sv[1]), sum_rows16(sv[0], sv[1]));
+      const float s2 = sum_halves32(sum_rows16(sv[2], sv[2]), sum_rows16(sv[2], sv[2]));
+      const int o01 = 4 * (g & 1) + (j >> 2), o2 = 8 + (j >> 2);
+      const uint32_t c01 = static_cast<uint32_t>(s01) + (o01 ? kQpelPenalty : 0);
+      const uint32_t c2 = static_cast<uint32_t>(s2) + kQpelPenalty;
+      uint32_t best = (c01 << 4) | static_cast<uint32_t>(o01);
+      if (o2 < 9) best = min(best, (c2 << 4) | static_cast<uint32_t>(o2));
       const uint32_t key = wave_min_u32(best);
       const int ob = static_cast<int>(key & 15);
       qcx += (static_cast<int>((kRefDx >> (2 * ob)) & 3u) - 1) * step;
@@ -260,7 +273,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, int item, int
   if (lane == 0) {
     if (a.cost_out[r]) a.cost_out[r][by * a.cols + bx] = best_cost;
     a.out[r][by * a.cols + bx] = pack_mv(mvx, mvy);
-    if (a.searches) atomicAdd(a.searches + 16 * (item & 63), two ? 2ull : 1ull);
+    if (a.searches) atomicAdd(a.searches + 16 * ((bx + 7 * by + r) & 63), two ? 2ull : 1ull);
   }
 }
 
@@ -277,16 +290,21 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
   // reference limits thin out the far ones, and a workgroup's 4 waves search neighbouring blocks.
   const int bi = batch_item(batch, g);
   const MeLevelArgs& a = batch.item[bi];
-  const int u = g - batch.first[bi], r = u % a.nrefs, block = 4 * (u / a.nrefs) + wave, per_ref = a.rows * a.cols;
-  if (block >= per_ref) return;  // whole wave
-  search_block(a, r * per_ref + block, lane, wins[wave][0], wins[wave][1]);
+  const int u = g - batch.first[bi], grp = div_small(u, a.nrefs, a.inv_nrefs), r = u - grp * a.nrefs;
+  const int block = 4 * grp + wave;
+  if (block >= a.rows * a.cols) return;  // whole wave
+  const int by = div_small(block, a.cols, a.inv_cols);
+  search_block(a, BlockPos{r, block - by * a.cols, by}, lane, wins[wave][0], wins[wave][1]);
 }
 
 }  // namespace
 
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a) {
   const int units = a.nrefs * ((a.rows * a.cols + 3) / 4);
-  if (units > 0) b.add(a, units);
+  if (units <= 0) return;
+  MeLevelArgs x = a;
+  x.inv_nrefs = 1.f / static_cast<float>(a.nrefs), x.inv_cols = 1.f / static_cast<float>(a.cols);
+  b.add(x, units);
 }
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s) {
   if (b.total() <= 0) return;
