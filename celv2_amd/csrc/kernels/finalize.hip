@@ -53,6 +53,8 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
   __shared__ uint32_t bmv[kBlocks], bmv2[kBlocks];  // the inter choice's MVs (quarter pel, packed)
   __shared__ int64_t sq_cost[kSquares];
   __shared__ uint8_t sq_choice[kSquares];
+  __shared__ int16_t sq_nt[kSquares], sq_nl[kSquares];      // the chosen subtree's tree entries and leaves
+  __shared__ int16_t sq_toff[kSquares], sq_loff[kSquares];  // its first tree entry and leaf (-1: not in the tree)
   __shared__ uint8_t tree[CELV2_SB_CAP];
   __shared__ uint16_t leaf[CELV2_SB_CAP];
   __shared__ int counts[2];
@@ -98,7 +100,10 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
     // Four steps at a time: every load of the four is in flight before the first product.
     constexpr int kWaves = kThreads / 64, kBatch = 4;
     static_assert((kAllTiles / 16) % (kWaves * kBatch) == 0, "compound steps per batch");
-    for (int step0 = wave; step0 < kAllTiles / 16; step0 += kWaves * kBatch) {
+    constexpr int kRounds = kAllTiles / 16 / (kWaves * kBatch);
+#pragma unroll
+    for (int it = 0; it < kRounds; it++) {
+      const int step0 = wave + it * kWaves * kBatch;
       uint32_t cur[kBatch], avg[kBatch];
       int blk[kBatch];
       bool on[kBatch];
@@ -200,35 +205,53 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
         const int child = level_first(level + 1) + (2 * qy) * (2 * nn) + 2 * qx;
         const int64_t cs = part + sq_cost[child] + sq_cost[child + 1] + sq_cost[child + 2 * nn] + sq_cost[child + 2 * nn + 1];
         if (cs < best) best = cs, choice = 3;
+        if (choice == 3) {
+          sq_nt[sq] = static_cast<int16_t>(1 + sq_nt[child] + sq_nt[child + 1] + sq_nt[child + 2 * nn] + sq_nt[child + 2 * nn + 1]);
+          sq_nl[sq] = static_cast<int16_t>(sq_nl[child] + sq_nl[child + 1] + sq_nl[child + 2 * nn] + sq_nl[child + 2 * nn + 1]);
+        }
       }
+      if (choice != 3) sq_nt[sq] = static_cast<int16_t>(choice == 0 ? 1 : 3), sq_nl[sq] = static_cast<int16_t>(choice == 0 ? 1 : 2);
       sq_cost[sq] = best, sq_choice[sq] = static_cast<uint8_t>(choice);
     }
     __syncthreads();
   }
 
-  // Preorder walk of the chosen tree (one thread): tree entries and the leaves' blocks.
-  if (tid == 0) {
-    if (a.sb_cost) a.sb_cost[sb] = sq_cost[0];
-    int st_level[16], st_qx[16], st_qy[16], sp = 1, nt = 0, nl = 0;
-    st_level[0] = 0, st_qx[0] = 0, st_qy[0] = 0;
-    while (sp > 0) {
-      sp--;
-      const int level = st_level[sp], qx = st_qx[sp], qy = st_qy[sp];
-      const int sq = level_first(level) + (qy << level) + qx, b = 5 * sq, ch = sq_choice[sq];
-      if (ch == 0) {
-        tree[nt++] = 0, leaf[nl++] = static_cast<uint16_t>(b);
-      } else if (ch == 1) {
-        tree[nt++] = 1, tree[nt++] = 0, tree[nt++] = 0;
-        leaf[nl++] = static_cast<uint16_t>(b + 1), leaf[nl++] = static_cast<uint16_t>(b + 2);
-      } else if (ch == 2) {
-        tree[nt++] = 2, tree[nt++] = 0, tree[nt++] = 0;
-        leaf[nl++] = static_cast<uint16_t>(b + 3), leaf[nl++] = static_cast<uint16_t>(b + 4);
-      } else {
-        tree[nt++] = 3;
-        for (int q = 3; q >= 0; q--) st_level[sp] = level + 1, st_qx[sp] = 2 * qx + (q & 1), st_qy[sp] = 2 * qy + (q >> 1), sp++;
+  // The chosen tree in preorder, built in parallel: each square's first tree entry and leaf from its
+  // parent's (a split's children follow it in quadrant order 0..3, each after the previous child's
+  // whole subtree), then every square in the tree writes its own entries.
+  for (int i = tid; i < kSquares; i += kThreads) sq_toff[i] = static_cast<int16_t>(i == 0 ? 0 : -1), sq_loff[i] = 0;
+  __syncthreads();
+  for (int level = 0; level < 3; level++) {
+    const int cnt = 1 << (2 * level);
+    if (tid < cnt) {
+      const int sq = level_first(level) + tid;
+      if (sq_toff[sq] >= 0 && sq_choice[sq] == 3) {
+        const int nn = 1 << level, qx = tid % nn, qy = tid / nn;
+        const int child = level_first(level + 1) + (2 * qy) * (2 * nn) + 2 * qx;
+        const int kids[4] = {child, child + 1, child + 2 * nn, child + 2 * nn + 1};
+        int t = sq_toff[sq] + 1, l = sq_loff[sq];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          sq_toff[kids[q]] = static_cast<int16_t>(t), sq_loff[kids[q]] = static_cast<int16_t>(l);
+          t += sq_nt[kids[q]], l += sq_nl[kids[q]];
+        }
       }
     }
-    counts[0] = nt, counts[1] = nl;
+    __syncthreads();
+  }
+  for (int sq = tid; sq < kSquares; sq += kThreads) {
+    const int t = sq_toff[sq], l = sq_loff[sq], ch = sq_choice[sq], b = 5 * sq;
+    if (t < 0) continue;
+    tree[t] = static_cast<uint8_t>(ch);
+    if (ch == 0) leaf[l] = static_cast<uint16_t>(b);
+    if (ch == 1 || ch == 2) {
+      tree[t + 1] = 0, tree[t + 2] = 0;
+      leaf[l] = static_cast<uint16_t>(b + 2 * ch - 1), leaf[l + 1] = static_cast<uint16_t>(b + 2 * ch);
+    }
+  }
+  if (tid == 0) {
+    if (a.sb_cost) a.sb_cost[sb] = sq_cost[0];
+    counts[0] = sq_nt[0], counts[1] = sq_nl[0];
   }
   __syncthreads();
   const int nt = counts[0], nl = counts[1];
@@ -253,10 +276,12 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
     }
     out->blocks[i] = d;
   }
-  if (tid == 0 && a.counter) {
-    unsigned long long k = 0;
-    for (int b = 0; b < kBlocks; b++) k += (rs[b] >= 0 && pair_allowed(ra[b], rs[b], n)) ? 1 : 0;
-    atomicAdd(a.counter, k);
+  if (a.counter) {
+    // Compound blocks scored: each thread's blocks, summed over its wave.
+    int k = 0;
+    for (int b = tid; b < kBlocks; b += kThreads) k += (rs[b] >= 0 && pair_allowed(ra[b], rs[b], n)) ? 1 : 0;
+    for (int m = 32; m >= 1; m >>= 1) k += __shfl_xor(k, m, 64);
+    if ((tid & 63) == 0 && k) atomicAdd(a.counter, static_cast<unsigned long long>(k));
   }
 }
 
