@@ -68,12 +68,13 @@ __device__ __forceinline__ uint32_t score_window(const uint32_t* win, int sh, in
 constexpr uint32_t kRefDx = (1u << 0) | (0u << 2) | (1u << 4) | (2u << 6) | (0u << 8) | (2u << 10) | (0u << 12) | (1u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u << 8) | (1u << 10) | (2u << 12) | (2u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 
-// The loads that open a block's search, issued together: lanes 0..8
-// the reference limits of the 3x3 superblocks around it (0 outside the frame or without limits), lanes
-// 0..15 the block's 16 words, lanes 0..4 the coarser MVs under and beside it, and at level 0 the row
-// of the block's 4x4 tile that the lane feeds to the refinement's products.
+// The loads that open a block's search, issued together: lanes 0..8 the reference limits of the 3x3
+// superblocks around it (0 outside the frame or without limits), lanes 0..15 the block's 16 words
+// (row i's two words in lanes 2i, 2i + 1; every later use of the block's samples takes them from
+// here by lane shuffles, not from memory again: the search is bound by the vector-memory data path),
+// lanes 0..4 the coarser MVs under and beside it.
 struct Opening {
-  uint32_t km = 0, mine = 0, cm = 0, crow = 0;
+  uint32_t km = 0, mine = 0, cm = 0;
 };
 
 struct BlockPos {
@@ -110,10 +111,6 @@ __device__ __forceinline__ Opening open_block(const MeLevelArgs& a, const BlockP
     const int qx = clampi(q.bx / 2 + ox, 0, a.pcols - 1), qy = clampi(q.by / 2 + oy, 0, a.prows - 1);
     o.cm = a.coarser[q.r][qy * a.pcols + qx];
   }
-  if (a.level0) {
-    const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;
-    o.crow = bload4(cb, corner_offset(cb, 8 * q.by + 4 * ty + g, 8 * q.bx + 4 * tx), 0);
-  }
   return o;
 }
 
@@ -124,8 +121,8 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   const Opening o = open_block(a, q, lane);
   // Searched only near a superblock that keeps reference r.
   if (limited(a, r) && __ballot(o.km > static_cast<uint32_t>(r)) == 0) return;  // whole wave
-  const DPlane cur = a.cur, ref = a.ref[r];
-  const BPlane rb = as_buffer(ref), cb = as_buffer(cur);
+  const DPlane ref = a.ref[r];
+  const BPlane rb = as_buffer(ref);
 
   // The block, every lane holding every word (scalar registers after readlane).
   uint32_t c[16];
@@ -155,8 +152,8 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     uint32_t d0, d1, d2;
     bload12(rb, off & ~3, 0, d0, d1, d2);
     const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, off & 3), w1 = __builtin_amdgcn_alignbyte(d2, d1, off & 3);
-    uint32_t c0, c1;
-    bload8(cb, corner_offset(cb, 8 * by + row, 8 * bx), 0, c0, c1);
+    const uint32_t c0 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row, 64));
+    const uint32_t c1 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row + 1, 64));
     const uint32_t s = oct_sum(sad4(c1, w1, sad4(c0, w0, 0)));
     // me_cell's ranking (src/gpu/inter_common.h:331-350): each distinct centre once, best and runner-up.
     int s1 = 0x7fffffff, s2 = 0x7fffffff;
@@ -221,7 +218,9 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;
     const half4 hh = hadamard_operand(lane, false), nhh = hadamard_operand(lane, true);
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-    const f32x4 ccur = mfma16(hh, row_halves(o.crow), zero);
+    // Row g of the block's tile (tx, ty): word 2 (4 ty + g) + tx of the block.
+    const uint32_t crow = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * (4 * ty + g) + tx, 64));
+    const f32x4 ccur = mfma16(hh, row_halves(crow), zero);
     int odx[3], ody[3];
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) {
