@@ -224,7 +224,8 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     int odx[3], ody[3];
 #pragma unroll
     for (int cc = 0; cc < 3; cc++) {
-      const int o = 4 * cc + (j >> 2);
+      // Padding columns (positions 9..11) read position 8's rows: the same cache lines, results unused.
+      const int o = min(4 * cc + (j >> 2), 8);
       odx[cc] = static_cast<int>((kRefDx >> (2 * o)) & 3u) - 1, ody[cc] = static_cast<int>((kRefDy >> (2 * o)) & 3u) - 1;
     }
     int qcx = 4 * mvx, qcy = 4 * mvy;
