@@ -56,6 +56,23 @@ def test_seven_refs_with_key_reuse(gpu):
         check(got, want, e.sb_cols)
 
 
+@pytest.mark.parametrize("w,h,workload", [(3840, 128, 4), (128, 2160, 1), (3840, 128, 1)])
+def test_4k_strips_with_seven_refs(gpu, w, h, workload):
+    """The 4K frame's width (60 superblock columns) and height (34 rows) as strips, cel and natural
+    content, 7 references, against the oracle's threaded CPU engine."""
+    _, y16, y8 = clip(workload, 21, 8, w, h)
+    lam = gpu.lambda_q8(130)
+    refs = [y8[i] for i in range(6, -1, -1)]
+    e = gpu.HipEngine(w, h)
+    e.launch(y8[7], refs, lam=lam, keys=list(range(6, -1, -1)))
+    got, ready = e.collect()
+    assert ready == e.sb_rows
+    cpu = ol.CpuEngine(w, h, threads=8)
+    cpu.launch(y8[7], refs, lam)
+    want, _ = cpu.collect()
+    check(got, want, e.sb_cols)
+
+
 def test_ten_bit_rounded_on_device(gpu):
     w, h = 352, 288
     _, y16, y8 = clip(0, 9, 2, w, h)
