@@ -26,7 +26,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from celv2_amd import engine, pruner, synth  # noqa: E402
+from celv2_amd import engine, gop, pruner, synth  # noqa: E402
 from celv2_amd.records import SB_DTYPE, compare_maps  # noqa: E402
 
 METRIC = "1080p10 frames/s on 1 GPU: plain HIP vs learned-pruned at ≤0.5% BD-rate loss"
@@ -371,6 +371,83 @@ def bench_cel(NR, qindex, steps=20, warmup=3, nf=48):
             "bd_rate": "pending (needs the full encoder and AVM)"}
 
 
+def bench_gop(lum, qindex, passes=3):
+    """The 32-frame clip coded as the frame loop orders it (celv2_amd/gop.py: keyframe, then 16-frame
+    mini-GOPs with a hidden anchor and pyramid, 7 references, per-frame qindex deltas bq 16, aq -16,
+    level step 8): every coded frame decided against its ranked references at its own qindex, the
+    next coded frame prepared behind each launch. One untimed pass, then `passes` timed ones (keys
+    fresh per pass, so every pass builds its references as a new sequence would). The learned form
+    takes the operating point of each frame's qindex. Then, untimed, the learned form's cost regret
+    against plain over every coded frame."""
+    nf = len(lum)
+    cfg = gop.defaults(7, qindex, 1920, 1080, long_gop=True)
+    entries = gop.coded(gop.plan(cfg, nf))
+    w, pcfg = pruner.load()
+    P = engine.Pruner(max_positions=(1920 // 64 + 1) * (1080 // 64 + 1), seed=1)
+    P.set_params(w)
+    ops = {e.qindex: pruner.operating_point(pcfg, e.qindex) for e in entries}
+
+    def run(E, base, learned):
+        for j, f in enumerate(entries):
+            if learned:
+                op = ops[f.qindex] if f.refs else None
+                E.set_pruner(P, *op) if op else E.set_pruner(None)
+            E.launch(lum[f.display], [lum[r] for r in f.refs], engine.lambda_q8(f.qindex),
+                     keys=[base + r for r in f.refs], src_key=base + f.display)
+            if j + 1 < len(entries):
+                E.prepare(lum[entries[j + 1].display], base + entries[j + 1].display)
+
+    rates = {}
+    for name, learned in (("plain", False), ("learned", True)):
+        E = engine.RawEngine(1920, 1080)
+        E.set_resident(True)
+        run(E, 0, learned)
+        E.sync()
+        t0 = time.perf_counter()
+        for k in range(passes):
+            run(E, 1000 * (k + 1), learned)
+        E.sync()
+        rates[name] = passes * len(entries) / (time.perf_counter() - t0)
+        E.close()
+    # Untimed: regret, limits and fallbacks over every coded inter frame.
+    Ep, El = engine.RawEngine(1920, 1080), engine.RawEngine(1920, 1080)
+    for e in (Ep, El):
+        e.set_resident(True)
+    jp = jl = 0.0
+    ks, fb = [], 0
+    El.stats_reset()
+    for f in entries:
+        if not f.refs:
+            continue
+        op = ops[f.qindex]
+        El.set_pruner(P, *op) if op else El.set_pruner(None)
+        for e in (Ep, El):
+            e.launch(lum[f.display], [lum[r] for r in f.refs], engine.lambda_q8(f.qindex), keys=list(f.refs))
+            e.sync()
+        jp += Ep.fetch_sb_costs().astype(np.float64).sum()
+        jl += El.fetch_sb_costs().astype(np.float64).sum()
+        ks.append(El.fetch_ref_limits())
+    fb = El.stats()[1]["pruner_fallbacks"]
+    ks = np.concatenate(ks)
+    for e in (Ep, El):
+        e.close()
+    P.close()
+    refs_mean = float(np.mean([len(f.refs) for f in entries]))
+    log(f"gop: {len(entries)} coded frames (mean {refs_mean:.2f} references): plain {rates['plain']:.1f} frames/s, "
+        f"learned {rates['learned']:.1f} ({rates['learned'] / rates['plain']:.2f}x), regret {(jl / jp - 1) * 100:.3f}%, "
+        f"mean limit {ks.mean():.2f}")
+    return {"workload": f"the same {nf}-frame 1080p 10-bit clip in the frame loop's order: keyframe, 16-frame mini-GOPs "
+                        "(hidden anchor, 5 pyramid frames, 10 B frames), up to 7 ranked references, qindex "
+                        f"{qindex} with per-frame deltas (anchor -16, pyramid levels 0/+8/+16, B frames +16)",
+            "coded_frames": len(entries), "mean_references": round(refs_mean, 3), "passes": passes,
+            "qindex_by_type": {t: sorted({f.qindex for f in entries if f.type == t}) for t in "KAMBP" if any(f.type == t for f in entries)},
+            "plain_frames_per_s": round(rates["plain"], 2), "learned_frames_per_s": round(rates["learned"], 2),
+            "speedup": round(rates["learned"] / rates["plain"], 3), "cost_regret": round(jl / jp - 1, 6),
+            "mean_reference_limit": round(float(ks.mean()), 3), "fallback_rate": round(fb / ks.size, 5),
+            "operating_points": {str(q): op for q, op in sorted(ops.items())},
+            "bd_rate": "pending (needs the full encoder and AVM)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,6 +462,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-pruned form")
     ap.add_argument("--no-sweep", action="store_true", help="skip the qindex sweep of the learned form")
     ap.add_argument("--no-cel", action="store_true", help="skip the 4K cel workload (learned + batched)")
+    ap.add_argument("--no-gop", action="store_true", help="skip the clip in the frame loop's coding order")
     ap.add_argument("--e2e-steps", type=int, default=12)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
@@ -508,6 +586,7 @@ def main():
         if not args.no_sweep:
             learned["qindex_sweep"] = bench_qindex_sweep(lum, decidable, NR, args.steps, args.warmup)
     cel = None if args.no_cel or args.no_learned or NR != 7 else bench_cel(NR, args.qindex)
+    gop_form = None if args.no_gop or NR != 7 else bench_gop(lum, args.qindex)
 
     cpu = None
     parity = None
@@ -559,7 +638,7 @@ def main():
         "all_kernels_tops": round(total_ops * fps / 1e12, 3),
         "engine_path_fps_host_buffers": e2e,
         "forms": {"plain": {"frames_per_s": round(fps, 2), "scored_candidates_per_s": round(cands * fps)},
-                  "batched": batched, "learned": learned, "learned_batched_cel_4k": cel},
+                  "batched": batched, "learned": learned, "learned_batched_cel_4k": cel, "gop_order": gop_form},
     }
     print(json.dumps(out), flush=True)
 

@@ -1,0 +1,29 @@
+"""The HIP engine driven over a coding schedule (keyframe, hidden anchor, pyramid frames, B frames,
+a lone frame) the way Pipeline::code launches it (src/pipeline.cpp:124-146): each coded frame's
+ranked references with their display indices as keys (reused across frames), lambda of the frame's
+own qindex, the next coded frame prepared behind each launch. Every decision map must equal the
+oracle's CPU engine given the same references and lambda."""
+import pytest
+
+from celv2_amd import gop, synth
+from celv2_amd.records import compare_maps
+import oracle_lib as ol
+
+pytestmark = pytest.mark.gpu
+
+
+def test_minigop_schedule_matches_oracle(gpu):
+    w, h, frames = 352, 288, 18
+    cfg = gop.defaults(7, 130, w, h, long_gop=True)  # 16-frame mini-GOP, then a lone frame
+    entries = gop.plan(cfg, frames)
+    assert [e.type for e in gop.coded(entries)][:2] == ["K", "A"] and gop.coded(entries)[-1].type == "P"
+    _, y8 = synth.clip_frames(synth.params(1, 17, w, h), frames)
+    e = gpu.RawEngine(w, h)
+    maps = gop.drive(e, y8, entries)
+    cpu = ol.CpuEngine(w, h, threads=8)
+    for f in gop.coded(entries):
+        print(f"frame {f.display} {f.type} q{f.qindex} refs {f.refs}", flush=True)
+        cpu.launch(y8[f.display], [y8[r] for r in f.refs], gpu.lambda_q8(f.qindex))
+        want, _ = cpu.collect()
+        diff = compare_maps(maps[f.display], want, e.sb_cols)
+        assert diff is None, (f.display, f.type, diff)
