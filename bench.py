@@ -371,7 +371,7 @@ def bench_cel(NR, qindex, steps=20, warmup=3, nf=48):
             "bd_rate": "pending (needs the full encoder and AVM)"}
 
 
-def bench_gop(lum, qindex, passes=3):
+def bench_gop(lum, qindex, passes=10):
     """The 32-frame clip coded as the frame loop orders it (celv2_amd/gop.py: keyframe, then 16-frame
     mini-GOPs with a hidden anchor and pyramid, 7 references, per-frame qindex deltas bq 16, aq -16,
     level step 8): every coded frame decided against its ranked references at its own qindex, the
