@@ -40,7 +40,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}"]
 
 KERNELS = ["planes", "me", "intra", "inter", "finalize", "pruner", "cnn"]
-HOST = ["engine", "hip_engine", "row_gate", "engine_capi", "gop"]
+HOST = ["engine", "hip_engine", "row_gate", "engine_capi", "gop", "y4m"]
 
 
 def _run(cmd):
