@@ -11,7 +11,9 @@
 // shapes of 256 tiles each), a block's tiles contiguous, so every 16-tile "step" has one shape and
 // a block's tiles fill an aligned group of 2..16 columns of the product. Lane (j, g) = (lane & 15,
 // lane >> 4) holds row g of tile j: it predicts that row's four samples in each mode. A superblock is
-// split over 4 workgroups of 36 steps, at block boundaries; each wave takes every 4th step.
+// split over 4 workgroups of 36 steps, at block boundaries; each wave takes every 4th step. Per step
+// the 13 modes' sums are reduced two at a time (row swaps, then DPP within rows), and one lane per
+// block and mode adds the total to the block's LDS accumulator.
 //
 // Edges. The source around the superblock (rows and columns -1 .. 95) is staged in LDS twice. The
 // copies differ only in row -1 and column -1, so that every edge sample the reference reads through
@@ -21,6 +23,7 @@
 // at offsets from a static per-(shape, mode, row, column) table built from dir_pred's geometry.
 #include <algorithm>
 #include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -129,15 +132,9 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) a[jj] = static_cast<int>((a4 >> (8 * jj)) & 255);
 
-    auto score = [&](int m, uint32_t p01, uint32_t p23) {
-      float s = abs_sum4(mfma16(nhh, as_half4(p01, p23), ccur));
-      // Sum over the block's tiles (an aligned group of sh.seg columns of this row).
-      if (sh.seg >= 2) s += as_f32(dpp<kDppXor1>(as_u32f(s)));
-      if (sh.seg >= 4) s += as_f32(dpp<kDppXor2>(as_u32f(s)));
-      if (sh.seg >= 8) s += as_f32(dpp<kDppHalfMirror>(as_u32f(s)));
-      if (sh.seg >= 16) s += as_f32(dpp<kDppMirror>(as_u32f(s)));
-      if ((j16 & (sh.seg - 1)) == 0) atomicAdd(&acc[lp * kModes + m], s);
-    };
+    // Each mode's |coefficient| sums, a quarter tile per lane; summed per block below.
+    float part[kModes];
+    auto score = [&](int m, uint32_t p01, uint32_t p23) { part[m] = abs_sum4(mfma16(nhh, as_half4(p01, p23), ccur)); };
 
     // 0 DC, 1 V, 2 H (intra_common.h:89-96).
     {
@@ -197,6 +194,29 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
         v[jj] = static_cast<uint32_t>(e0 + (((e1 - e0) * f + 16) >> 5) + 0x6400);
       }
       score(7 + mm, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+    }
+    // Block sums, two modes at a time: a tile's 16 coefficients sit in lanes j, j + 16, j + 32, j + 48,
+    // so the row sums leave mode m's tile totals in row 0 and mode m + 1's in row 1; then the block's
+    // SEG columns (an aligned group) are summed within the row, and one lane per block adds it.
+    auto flush = [&](auto seg_c) {
+      constexpr int SEG = decltype(seg_c)::value;
+#pragma unroll
+      for (int m = 0; m < kModes; m += 2) {
+        float t = sum_rows16(part[m], m + 1 < kModes ? part[m + 1] : 0.f);
+        t = sum_halves32(t, t);
+        if constexpr (SEG >= 2) t += as_f32(dpp<kDppXor1>(as_u32f(t)));
+        if constexpr (SEG >= 4) t += as_f32(dpp<kDppXor2>(as_u32f(t)));
+        if constexpr (SEG >= 8) t += as_f32(dpp<kDppHalfMirror>(as_u32f(t)));
+        if constexpr (SEG >= 16) t += as_f32(dpp<kDppMirror>(as_u32f(t)));
+        const int mode = m + g;
+        if (g < 2 && (j16 & (SEG - 1)) == 0 && mode < kModes) atomicAdd(&acc[lp * kModes + mode], t);
+      }
+    };
+    switch (sh.seg) {
+      case 16: flush(std::integral_constant<int, 16>{}); break;
+      case 8: flush(std::integral_constant<int, 8>{}); break;
+      case 4: flush(std::integral_constant<int, 4>{}); break;
+      default: flush(std::integral_constant<int, 2>{}); break;
     }
   }
   __syncthreads();
