@@ -46,6 +46,7 @@ constexpr int kGroupSteps = kSteps / kGroups;          // 36
 constexpr int kMaxGroupBlocks = 272;
 constexpr int kDirModes = 6;                           // modes 7..12
 constexpr int kDirEntries = kDirModes * 2688;          // 2688 = samples of the nine shapes
+constexpr int kMultiBlocks = 20;                       // 32x32, 32x16, 16x32: the first 20 positions
 
 struct ShapeInfo {
   int w, h, lw, lh;     // size and log2 size
@@ -57,13 +58,13 @@ struct ShapeInfo {
 
 __constant__ ShapeInfo g_shape[kShapes];
 __constant__ int g_group_pos[kGroups + 1];  // block positions (shape-sorted order) of each group
-__device__ uint32_t g_itile[kIntraTiles];   // x/4 | y/4 << 4 | tx << 8 | ty << 11 | pos << 14
+__device__ uint32_t g_itile[kIntraTiles];   // x/4 | y/4 << 4 | tx << 8 | ty << 11 | pos << 14 | block << 23
 __device__ uint16_t g_pos_block[420];       // block index of each position
 __device__ __align__(16) uint32_t g_dir[kDirEntries];  // uint16 offset | f << 16 | step << 21
 
 __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
   __shared__ __align__(16) uint8_t reg[2 * kRBytes];  // above copy, left copy
-  __shared__ float acc[kMaxGroupBlocks * kModes];
+  __shared__ float macc[kMultiBlocks * kModes];  // 32x32, 32x16, 16x32 blocks (their tiles span steps)
   __shared__ uint32_t dcv[kMaxGroupBlocks];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -86,7 +87,7 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
     const uint32_t v = *reinterpret_cast<const uint32_t*>(src.p + static_cast<ptrdiff_t>(yy) * src.stride + xx);
     ra[k] = v, rl[k] = v;
   }
-  for (int k = tid; k < np * kModes; k += 256) acc[k] = 0.f;
+  for (int k = tid; k < kMultiBlocks * kModes; k += 256) macc[k] = 0.f;
   __syncthreads();
   if (oy == 0 && tid < kRS / 4) ra[tid] = 0x80808080u;    // above copy: row -1 is 128 at the frame top
   if (ox == 0 && tid < kRRows) reg[kRBytes + tid * kRS + 3] = 128;  // left copy: column -1 at the frame left
@@ -113,12 +114,13 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
   const half4 hh = hadamard_operand(lane, false), nhh = hadamard_operand(lane, true);
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   const int j16 = lane & 15, g = lane >> 4;
+  uint32_t* __restrict__ const o = out + (static_cast<size_t>(sby) * sb_cols + sbx) * kBlocks * kModes;
   for (int step = kGroupSteps * q + wave; step < kGroupSteps * (q + 1); step += 4) {
     const ShapeInfo sh = g_shape[step >> 4];
     const uint32_t e = g_itile[16 * step + j16];
     const int x = 4 * static_cast<int>(e & 15), y = 4 * static_cast<int>((e >> 4) & 15);
     const int pj0 = 4 * static_cast<int>((e >> 8) & 7), pi = 4 * static_cast<int>((e >> 11) & 7) + g;
-    const int lp = static_cast<int>(e >> 14) - p0;
+    const int pos = static_cast<int>((e >> 14) & 511), block = static_cast<int>(e >> 23);
     const int w = sh.w, h = sh.h;
     const int blk = (y + 1) * kRS + x + 4;  // offset of sample (y, x) in the above copy
     const uint32_t cur = *reinterpret_cast<const uint32_t*>(reg + blk + pi * kRS + pj0);
@@ -126,104 +128,116 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
     const int lft = reg[kRBytes + blk + pi * kRS - 1];                              // left[pi]
     const int tr = reg[blk - kRS + w - 1], bl = reg[kRBytes + blk + (h - 1) * kRS - 1];
     const int tl = reg[blk - kRS - 1];
-    const int dc = static_cast<int>(dcv[lp]);
+    const int dc = static_cast<int>(dcv[pos - p0]);
     const f32x4 ccur = mfma16(hh, row_halves(cur), zero);
     int a[4];
 #pragma unroll
     for (int jj = 0; jj < 4; jj++) a[jj] = static_cast<int>((a4 >> (8 * jj)) & 255);
 
-    // Each mode's |coefficient| sums, a quarter tile per lane; summed per block below.
-    float part[kModes];
-    auto score = [&](int m, uint32_t p01, uint32_t p23) { part[m] = abs_sum4(mfma16(nhh, as_half4(p01, p23), ccur)); };
-
-    // 0 DC, 1 V, 2 H (intra_common.h:89-96).
-    {
-      const uint32_t pd = (static_cast<uint32_t>(dc) | 0x6400u) * 0x10001u;
-      score(0, pd, pd);
-      score(1, halves01(a4), halves23(a4));
-      const uint32_t ph = (static_cast<uint32_t>(lft) | 0x6400u) * 0x10001u;
-      score(2, ph, ph);
-    }
-    // 3 Smooth: (h-i) a[j] + (i+1) bl + (w-j) l[i] + (j+1) tr, rounded, / (w + h + 2).
-    {
-      const int base = (pi + 1) * bl + (w - pj0) * lft + (pj0 + 1) * tr + sh.den / 2, dl = tr - lft, al = h - pi;
-      uint32_t v[4];
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) v[jj] = __umulhi(static_cast<uint32_t>(al * a[jj] + base + jj * dl), sh.magic);
-      score(3, pack_lo16(v[0], v[1]) | 0x64006400u, pack_lo16(v[2], v[3]) | 0x64006400u);
-    }
-    // 4 Smooth V: ((h-1-i) a[j] + (i+1) left[h-1] + h/2) / h.
-    {
-      const int c = (pi + 1) * bl + h / 2 + (0x6400 << sh.lh), al = h - 1 - pi;
-      uint32_t v[4];
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) v[jj] = static_cast<uint32_t>((al * a[jj] + c) >> sh.lh);
-      score(4, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
-    }
-    // 5 Smooth H: ((w-1-j) l[i] + (j+1) above[w-1] + w/2) / w.
-    {
-      const int c = (w - 1 - pj0) * lft + (pj0 + 1) * tr + w / 2 + (0x6400 << sh.lw), dl = tr - lft;
-      uint32_t v[4];
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) v[jj] = static_cast<uint32_t>((c + jj * dl) >> sh.lw);
-      score(5, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
-    }
-    // 6 Paeth: the neighbour nearest to above + left - top-left, ties to left, then above.
-    {
-      const int pa = abs(lft - tl);
-      uint32_t v[4];
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) {
-        const int pl = abs(a[jj] - tl), ptl = abs(a[jj] + lft - 2 * tl);
-        const int p = (pl <= pa && pl <= ptl) ? lft : (pa <= ptl ? a[jj] : tl);
-        v[jj] = static_cast<uint32_t>(p) | 0x6400u;
-      }
-      score(6, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
-    }
-    // 7..12 directional (intra_common.h:61-85): two samples and a 5-bit weight per predicted sample.
-#pragma unroll
-    for (int mm = 0; mm < kDirModes; mm++) {
-      const uint4 t = *reinterpret_cast<const uint4*>(g_dir + sh.dir + (mm * h + pi) * w + pj0);
-      const uint32_t te[4] = {t.x, t.y, t.z, t.w};
-      uint32_t v[4];
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) {
-        const int o = blk + static_cast<int>(static_cast<int16_t>(te[jj] & 0xffffu));
-        const int f = static_cast<int>((te[jj] >> 16) & 31), st = static_cast<int>(te[jj] >> 21);
-        const int e0 = reg[o], e1 = reg[o + st];
-        v[jj] = static_cast<uint32_t>(e0 + (((e1 - e0) * f + 16) >> 5) + 0x6400);
-      }
-      score(7 + mm, pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
-    }
-    // Block sums, two modes at a time: a tile's 16 coefficients sit in lanes j, j + 16, j + 32, j + 48,
-    // so the row sums leave mode m's tile totals in row 0 and mode m + 1's in row 1; then the block's
-    // SEG columns (an aligned group) are summed within the row, and one lane per block adds it.
-    auto flush = [&](auto seg_c) {
+    // The step's work for blocks of SEG tiles (the shape's; uniform over the step).
+    auto body = [&](auto seg_c) {
       constexpr int SEG = decltype(seg_c)::value;
-#pragma unroll
-      for (int m = 0; m < kModes; m += 2) {
-        float t = sum_rows16(part[m], m + 1 < kModes ? part[m + 1] : 0.f);
+      // A mode's SATD per lane is a quarter tile (|coefficients| of row g of tile j). Two modes at a
+      // time: a tile's 16 coefficients sit in lanes j, j + 16, j + 32, j + 48, so the row sums leave
+      // mode m's tile totals in row 0 and mode m + 1's in row 1; the block's SEG columns (an aligned
+      // group) are then summed within the row, and one lane per block and mode keeps the total: a
+      // store for a block whose tiles are all in this step, an add to the LDS accumulator for the
+      // 32x32, 32x16 and 16x32 blocks, whose tiles span several steps.
+      auto emit = [&](int m, float pa, float pb) {
+        float t = sum_rows16(pa, pb);
         t = sum_halves32(t, t);
         if constexpr (SEG >= 2) t += as_f32(dpp<kDppXor1>(as_u32f(t)));
         if constexpr (SEG >= 4) t += as_f32(dpp<kDppXor2>(as_u32f(t)));
         if constexpr (SEG >= 8) t += as_f32(dpp<kDppHalfMirror>(as_u32f(t)));
         if constexpr (SEG >= 16) t += as_f32(dpp<kDppMirror>(as_u32f(t)));
         const int mode = m + g;
-        if (g < 2 && (j16 & (SEG - 1)) == 0 && mode < kModes) atomicAdd(&acc[lp * kModes + mode], t);
+        if (g < 2 && (j16 & (SEG - 1)) == 0 && mode < kModes) {
+          if (SEG == 16 && pos < kMultiBlocks)
+            atomicAdd(&macc[pos * kModes + mode], t);
+          else
+            o[block * kModes + mode] = static_cast<uint32_t>(t);
+        }
+      };
+      auto sat = [&](uint32_t p01, uint32_t p23) { return abs_sum4(mfma16(nhh, as_half4(p01, p23), ccur)); };
+
+      // 0 DC, 1 V, 2 H (intra_common.h:89-96).
+      const uint32_t pd = (static_cast<uint32_t>(dc) | 0x6400u) * 0x10001u;
+      emit(0, sat(pd, pd), sat(halves01(a4), halves23(a4)));
+      const uint32_t ph = (static_cast<uint32_t>(lft) | 0x6400u) * 0x10001u;
+      const float s2 = sat(ph, ph);
+      // 3 Smooth: (h-i) a[j] + (i+1) bl + (w-j) l[i] + (j+1) tr, rounded, / (w + h + 2).
+      float s3;
+      {
+        const int base = (pi + 1) * bl + (w - pj0) * lft + (pj0 + 1) * tr + sh.den / 2, dl = tr - lft, al = h - pi;
+        uint32_t v[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) v[jj] = __umulhi(static_cast<uint32_t>(al * a[jj] + base + jj * dl), sh.magic);
+        s3 = sat(pack_lo16(v[0], v[1]) | 0x64006400u, pack_lo16(v[2], v[3]) | 0x64006400u);
       }
+      emit(2, s2, s3);
+      // 4 Smooth V: ((h-1-i) a[j] + (i+1) left[h-1] + h/2) / h.
+      float s4;
+      {
+        const int c = (pi + 1) * bl + h / 2 + (0x6400 << sh.lh), al = h - 1 - pi;
+        uint32_t v[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) v[jj] = static_cast<uint32_t>((al * a[jj] + c) >> sh.lh);
+        s4 = sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+      }
+      // 5 Smooth H: ((w-1-j) l[i] + (j+1) above[w-1] + w/2) / w.
+      float s5;
+      {
+        const int c = (w - 1 - pj0) * lft + (pj0 + 1) * tr + w / 2 + (0x6400 << sh.lw), dl = tr - lft;
+        uint32_t v[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) v[jj] = static_cast<uint32_t>((c + jj * dl) >> sh.lw);
+        s5 = sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+      }
+      emit(4, s4, s5);
+      // 6 Paeth: the neighbour nearest to above + left - top-left, ties to left, then above.
+      float s6;
+      {
+        const int pa = abs(lft - tl);
+        uint32_t v[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          const int pl = abs(a[jj] - tl), ptl = abs(a[jj] + lft - 2 * tl);
+          const int p = (pl <= pa && pl <= ptl) ? lft : (pa <= ptl ? a[jj] : tl);
+          v[jj] = static_cast<uint32_t>(p) | 0x6400u;
+        }
+        s6 = sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+      }
+      // 7..12 directional (intra_common.h:61-85): two samples and a 5-bit weight per predicted sample.
+      auto dir = [&](int mm) {
+        const uint4 t = *reinterpret_cast<const uint4*>(g_dir + sh.dir + (mm * h + pi) * w + pj0);
+        const uint32_t te[4] = {t.x, t.y, t.z, t.w};
+        uint32_t v[4];
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++) {
+          const int off = blk + static_cast<int>(static_cast<int16_t>(te[jj] & 0xffffu));
+          const int f = static_cast<int>((te[jj] >> 16) & 31), st = static_cast<int>(te[jj] >> 21);
+          const int e0 = reg[off], e1 = reg[off + st];
+          v[jj] = static_cast<uint32_t>(e0 + (((e1 - e0) * f + 16) >> 5) + 0x6400);
+        }
+        return sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
+      };
+      emit(6, s6, dir(0));
+      emit(8, dir(1), dir(2));
+      emit(10, dir(3), dir(4));
+      emit(12, dir(5), 0.f);
     };
     switch (sh.seg) {
-      case 16: flush(std::integral_constant<int, 16>{}); break;
-      case 8: flush(std::integral_constant<int, 8>{}); break;
-      case 4: flush(std::integral_constant<int, 4>{}); break;
-      default: flush(std::integral_constant<int, 2>{}); break;
+      case 16: body(std::integral_constant<int, 16>{}); break;
+      case 8: body(std::integral_constant<int, 8>{}); break;
+      case 4: body(std::integral_constant<int, 4>{}); break;
+      default: body(std::integral_constant<int, 2>{}); break;
     }
   }
   __syncthreads();
-  uint32_t* o = out + (static_cast<size_t>(sby) * sb_cols + sbx) * kBlocks * kModes;
+  // The multi-step blocks (positions 0 .. kMultiBlocks - 1, in the first groups).
   for (int k = tid; k < np * kModes; k += 256) {
     const int i = k / kModes, m = k - i * kModes;
-    o[g_pos_block[p0 + i] * kModes + m] = static_cast<uint32_t>(acc[k]);
+    if (p0 + i < kMultiBlocks) o[g_pos_block[p0 + i] * kModes + m] = static_cast<uint32_t>(macc[(p0 + i) * kModes + m]);
   }
 }
 
@@ -285,12 +299,14 @@ void upload_intra_tiles(hipStream_t s) {
       for (int ty = 0; ty < h / 4; ty++)
         for (int tx = 0; tx < w / 4; tx++) {
           tiles[nt++] = static_cast<uint32_t>(bx / 4) | static_cast<uint32_t>(by / 4) << 4 | static_cast<uint32_t>(tx) << 8 |
-                        static_cast<uint32_t>(ty) << 11 | static_cast<uint32_t>(np) << 14;
+                        static_cast<uint32_t>(ty) << 11 | static_cast<uint32_t>(np) << 14 | static_cast<uint32_t>(b) << 23;
           tile_pos.push_back(np);
         }
       pos_block[np++] = static_cast<uint16_t>(b);
     }
     if (nt != kTilesPerShape * (si + 1)) throw std::logic_error("intra shape tile count");
+    // The shapes whose blocks span several 16-tile steps come first: positions 0 .. kMultiBlocks - 1.
+    if (si == 2 && np != kMultiBlocks) throw std::logic_error("intra multi-step block count");
     for (int mm = 0; mm < kDirModes; mm++)
       for (int i = 0; i < h; i++)
         for (int j = 0; j < w; j++) {
