@@ -38,7 +38,7 @@ SAD_OPS, SATD_OPS = 3, 7  # int ops per pixel (sub, abs, add; diff, 2 butterfly 
 SAD_PEAK_TPX, SAD_MEASURED_TPX = 256 * 64 * 4 * 2.4e9 / 1e12, 123.9
 # HBM bytes the five motion-search launches of one frame move (rocprofv3 FETCH_SIZE and WRITE_SIZE
 # per dispatch, scaled by a 64 MiB device copy: profiles/me_traffic_1080p10_7refs.md).
-ME_TRAFFIC_BYTES = 352.9e6
+ME_TRAFFIC_BYTES = 355.0e6
 
 
 def log(*a):
