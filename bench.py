@@ -572,7 +572,7 @@ def main():
                       refs16=[y16[r] for r in refs], bit_depth=10)
             if j + 1 < len(frames):
                 HE.prepare(y8[frames[j + 1]], frames[j + 1], y16=y16[frames[j + 1]], bit_depth=10)
-            HE.collect()
+            HE.collect(export=False)  # the DecisionMap as an encoder receives it, not copied back out
             ts.append(time.perf_counter() - t)
         e2e = round(1.0 / (sum(ts[2:]) / len(ts[2:])), 2)
         HE.close()

@@ -956,6 +956,18 @@ int celv2hip_wait_row(celv2hip_engine* e, int row, celv2_sb* out) {
   });
 }
 
+int celv2hip_row_records(celv2hip_engine* e, int row, const celv2_sb** out) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    if (!out) throw Failure(CELV2_ERR_ARG, "HipEngine: no output pointer");
+    if (!e->launched) throw Failure(CELV2_ERR_STATE, "HipEngine: no launch to collect");
+    if (e->resident) throw Failure(CELV2_ERR_STATE, "HipEngine: resident mode keeps results on the device");
+    if (row < 0 || row >= e->sb_rows) throw Failure(CELV2_ERR_ARG, "HipEngine: no such row");
+    check(hipEventSynchronize(e->row_done[row]), "row wait");
+    *out = reinterpret_cast<const celv2_sb*>(static_cast<const char*>(e->rec_host) + static_cast<size_t>(row) * e->sb_cols * sizeof(celv2_sb));
+  });
+}
+
 int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out) {
   if (int rc = validate(e)) return rc;
   return guarded([&] {

@@ -2,6 +2,7 @@
 // the same object the encoder would: launch / prepare / collect with a RowGate, and the resulting
 // DecisionMap exported as flat records (include/celv2hip.h).
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -17,6 +18,9 @@ struct Handle {
   std::unique_ptr<celv2::HipEngine> e;
   DecisionMap dm;
   int width, height;
+  // Frames by key, as a pipeline holds its sources and reconstructions: a keyed frame is copied in
+  // once; a launch keeps its source and references, prepare() adds the next source.
+  std::map<long long, Frame420> frames;
 };
 
 Frame420 frame_of(int w, int h, const uint8_t* y8, const uint16_t* y16, int bit_depth) {
@@ -58,15 +62,26 @@ int celv2_engine_launch(void* p, const uint8_t* src8, const uint16_t* src16, int
                         long long src_key) {
   Handle* h = static_cast<Handle*>(p);
   try {
-    const Frame420 src = frame_of(h->width, h->height, src8, src16, bit_depth);
-    std::vector<Frame420> refs;
-    for (int i = 0; i < nrefs; i++) refs.push_back(frame_of(h->width, h->height, refs8 ? refs8[i] : nullptr, refs16 ? refs16[i] : nullptr, bit_depth));
+    std::map<long long, Frame420> keep;
+    std::vector<Frame420> unkeyed;
+    unkeyed.reserve(static_cast<size_t>(nrefs) + 1);
+    // The frame under key (copied in unless held already), or an unkeyed copy.
+    auto frame = [&](long long key, const uint8_t* y8, const uint16_t* y16) -> const Frame420* {
+      if (key < 0) return &unkeyed.emplace_back(frame_of(h->width, h->height, y8, y16, bit_depth));
+      auto it = keep.find(key);
+      if (it != keep.end()) return &it->second;
+      auto held = h->frames.find(key);
+      if (held != h->frames.end()) return &keep.emplace(key, std::move(held->second)).first->second;
+      return &keep.emplace(key, frame_of(h->width, h->height, y8, y16, bit_depth)).first->second;
+    };
+    const Frame420* src = frame(src_key, src8, src16);
     std::vector<const Frame420*> rp;
-    for (const Frame420& f : refs) rp.push_back(&f);
+    for (int i = 0; i < nrefs; i++) rp.push_back(frame(keys ? keys[i] : -1, refs8 ? refs8[i] : nullptr, refs16 ? refs16[i] : nullptr));
     std::vector<long long> k;
     if (keys) k.assign(keys, keys + nrefs);
     h->e->set_lambda(lam);
-    h->e->launch(src, rp, qindex, h->dm, k, src_key);
+    h->e->launch(*src, rp, qindex, h->dm, k, src_key);
+    h->frames = std::move(keep);
     return 0;
   } catch (const std::invalid_argument& ex) {
     g_err = ex.what();
@@ -80,7 +95,13 @@ int celv2_engine_launch(void* p, const uint8_t* src8, const uint16_t* src16, int
 int celv2_engine_prepare(void* p, const uint8_t* y8, const uint16_t* y16, int bit_depth, long long key) {
   Handle* h = static_cast<Handle*>(p);
   try {
-    h->e->prepare(frame_of(h->width, h->height, y8, y16, bit_depth), key);
+    if (key >= 0) {
+      auto it = h->frames.find(key);
+      if (it == h->frames.end()) it = h->frames.emplace(key, frame_of(h->width, h->height, y8, y16, bit_depth)).first;
+      h->e->prepare(it->second, key);
+    } else {
+      h->e->prepare(frame_of(h->width, h->height, y8, y16, bit_depth), key);
+    }
     return 0;
   } catch (const std::exception& ex) {
     g_err = ex.what();
@@ -88,7 +109,8 @@ int celv2_engine_prepare(void* p, const uint8_t* y8, const uint16_t* y16, int bi
   }
 }
 
-// Collects into the DecisionMap and exports it; rows_ready: the gate's ready rows afterwards.
+// Collects into the DecisionMap and exports it (out NULL: not exported); rows_ready: the gate's ready
+// rows afterwards.
 int celv2_engine_collect(void* p, celv2_sb* out, int* rows_ready) {
   Handle* h = static_cast<Handle*>(p);
   RowGate gate;

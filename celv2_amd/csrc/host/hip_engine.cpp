@@ -1,10 +1,17 @@
 // celv2::HipEngine (include/celv2/hip_engine.h) over the C ABI of include/celv2hip.h.
 #include "celv2/hip_engine.h"
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <mutex>
+#include <thread>
+#include <type_traits>
+#include <vector>
 
 #include "../../../include/celv2hip.h"
 #include "celv2/lambda.h"
@@ -47,7 +54,34 @@ celv2_block record_of(const BlockDecision& d) {
 
 }  // namespace
 
+// The records' structs have the DecisionMap types' layout (same sizes, field offsets and 1-byte
+// enums), so a record converts by copying bytes; where a compiler lays them out otherwise, field by
+// field.
+constexpr bool kSameLayout =
+    std::is_trivially_copyable_v<BlockDecision> && std::is_trivially_copyable_v<BlockCandidates> &&
+    sizeof(Part) == 1 && sizeof(IntraMode) == 1 && sizeof(bool) == 1 && sizeof(BlockDecision) == sizeof(celv2_block) &&
+    offsetof(BlockDecision, w4) == offsetof(celv2_block, w4) && offsetof(BlockDecision, h4) == offsetof(celv2_block, h4) &&
+    offsetof(BlockDecision, inter) == offsetof(celv2_block, inter) && offsetof(BlockDecision, intra) == offsetof(celv2_block, intra) &&
+    offsetof(BlockDecision, mv_row) == offsetof(celv2_block, mv_row) && offsetof(BlockDecision, mv_col) == offsetof(celv2_block, mv_col) &&
+    offsetof(BlockDecision, ref) == offsetof(celv2_block, ref) && offsetof(BlockDecision, ref2) == offsetof(celv2_block, ref2) &&
+    offsetof(BlockDecision, mv2_row) == offsetof(celv2_block, mv2_row) && offsetof(BlockDecision, mv2_col) == offsetof(celv2_block, mv2_col) &&
+    sizeof(BlockCandidates) == sizeof(celv2_cands) && offsetof(BlockCandidates, inter_cost) == offsetof(celv2_cands, inter_cost) &&
+    offsetof(BlockCandidates, intra_cost) == offsetof(celv2_cands, intra_cost) && offsetof(BlockCandidates, inter) == offsetof(celv2_cands, inter) &&
+    offsetof(BlockCandidates, intra) == offsetof(celv2_cands, intra) && offsetof(BlockCandidates, ref_cost) == offsetof(celv2_cands, ref_cost) &&
+    offsetof(BlockCandidates, ref_mv_col) == offsetof(celv2_cands, ref_mv_col) && offsetof(BlockCandidates, ref_mv_row) == offsetof(celv2_cands, ref_mv_row) &&
+    BlockCandidates::kRefs == CELV2_MAX_REFS;
+
 void sb_record_to_decision(const celv2_sb& rec, SbDecision& d) {
+  if constexpr (kSameLayout) {
+    // inter / intra bytes of the records are 0 or 1 and a mode index: valid values of bool and IntraMode.
+    d.tree.resize(rec.ntree);
+    std::memcpy(static_cast<void*>(d.tree.data()), rec.tree, static_cast<size_t>(rec.ntree));
+    d.blocks.resize(rec.nblocks);
+    std::memcpy(static_cast<void*>(d.blocks.data()), rec.blocks, static_cast<size_t>(rec.nblocks) * sizeof(celv2_block));
+    d.cands.resize(rec.ncands);
+    std::memcpy(static_cast<void*>(d.cands.data()), rec.cands, static_cast<size_t>(rec.ncands) * sizeof(celv2_cands));
+    return;
+  }
   d.tree.resize(rec.ntree);
   for (int k = 0; k < rec.ntree; k++) d.tree[k] = static_cast<Part>(rec.tree[k]);
   d.blocks.resize(rec.nblocks);
@@ -125,21 +159,46 @@ void HipEngine::launch(const Frame420& src, const std::vector<const Frame420*>& 
 }
 
 void HipEngine::collect(DecisionMap& dm, RowGate& gate) {
-  try {
-    std::vector<int> order(sb_rows_);
-    const int n = celv2hip_row_order(h_, order.data(), sb_rows_);
-    if (n != sb_rows_) throw std::runtime_error("HipEngine::collect: no launch to collect");
-    rows_.resize(sb_cols_);
-    for (int r : order) {
-      const int rc = celv2hip_wait_row(h_, r, rows_.data());
-      if (rc != CELV2_OK) raise(rc, "HipEngine::collect");
-      for (int c = 0; c < sb_cols_; c++) sb_record_to_decision(rows_[c], dm.sbs[static_cast<size_t>(r) * sb_cols_ + c]);
-      gate.publish_one(r);
+  std::vector<int> order(sb_rows_);
+  const int n = celv2hip_row_order(h_, order.data(), sb_rows_);
+  if (n != sb_rows_) {
+    const auto e = std::make_exception_ptr(std::runtime_error("HipEngine::collect: no launch to collect"));
+    gate.fail(e);
+    celv2hip_sync(h_);
+    std::rethrow_exception(e);
+  }
+  // Rows are taken in queue order by up to 4 threads (this one included): each waits for its row,
+  // converts it straight from the engine's pinned buffer and publishes it. The first failure stops
+  // the others taking rows, fails the gate and is rethrown once every thread has finished.
+  std::atomic<int> next{0};
+  std::atomic<bool> failed{false};
+  std::exception_ptr err;
+  std::mutex err_mu;
+  auto work = [&] {
+    for (int i; !failed.load() && (i = next.fetch_add(1)) < n;) {
+      try {
+        const celv2_sb* row = nullptr;
+        const int rc = celv2hip_row_records(h_, order[i], &row);
+        if (rc != CELV2_OK) raise(rc, "HipEngine::collect");
+        for (int c = 0; c < sb_cols_; c++) sb_record_to_decision(row[c], dm.sbs[static_cast<size_t>(order[i]) * sb_cols_ + c]);
+        gate.publish_one(order[i]);
+      } catch (...) {
+        std::lock_guard<std::mutex> lock(err_mu);
+        if (!err) err = std::current_exception();
+        failed = true;
+      }
     }
-  } catch (...) {
-    gate.fail(std::current_exception());
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const int threads = std::max(1, std::min({4, n, static_cast<int>(hw ? hw : 1)}));
+  std::vector<std::thread> pool;
+  for (int k = 1; k < threads; k++) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  if (err) {
+    gate.fail(err);
     celv2hip_sync(h_);  // returns with no work pending
-    throw;
+    std::rethrow_exception(err);
   }
 }
 

@@ -168,10 +168,11 @@ class HipEngine:
         if rc != 0:
             raise EngineError(lib().celv2_engine_last_error().decode())
 
-    def collect(self):
-        out = np.zeros(self.sb_cols * self.sb_rows, dtype=SB_DTYPE)
+    def collect(self, export=True):
+        """(records, ready rows); export False: the DecisionMap stays in the engine (timing), records None."""
+        out = np.zeros(self.sb_cols * self.sb_rows, dtype=SB_DTYPE) if export else None
         ready = I(0)
-        rc = lib().celv2_engine_collect(self.h, out.ctypes.data, ctypes.byref(ready))
+        rc = lib().celv2_engine_collect(self.h, out.ctypes.data if export else None, ctypes.byref(ready))
         if rc != 0:
             raise EngineError(lib().celv2_engine_last_error().decode())
         return out, ready.value

@@ -56,7 +56,6 @@ class HipEngine : public DecisionEngine {
   std::string name_;
   bool full_depth_ = false;
   int64_t lambda_ = -1;
-  std::vector<celv2_sb> rows_;  // one row of exported records (collect)
 };
 
 // One exported superblock record as the encoder takes it (the role of sb_out_to_decision,

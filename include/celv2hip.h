@@ -132,6 +132,11 @@ int celv2hip_row_order(const celv2hip_engine* e, int* rows, int cap);
  * the last launch and writes its sb_cols records to out. */
 int celv2hip_wait_row(celv2hip_engine* e, int row, celv2_sb* out);
 
+/* The same wait without the copy: *out points at the row's sb_cols records in the engine's pinned
+ * host buffer, valid until the engine's next launch. No counterpart in the reference (the C++ engine's
+ * collect converts records to a DecisionMap straight from there). */
+int celv2hip_row_records(celv2hip_engine* e, int row, const celv2_sb** out);
+
 /* Waits for all queued work (used before timing and in tests). */
 int celv2hip_sync(celv2hip_engine* e);
 
