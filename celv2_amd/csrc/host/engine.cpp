@@ -168,7 +168,10 @@ struct celv2hip_engine {
   std::vector<std::array<hipEvent_t, 2>> pring;
   hipEvent_t* ev = nullptr;  // the current launch's 6 events
   long long prepares = 0, window_launch = 0, window_prepare = 0;
-  unsigned long long* counters = nullptr;  // device: [0] unique MVs scored, [1] compound blocks, [2] inter candidates, [16..]: ME searches
+  // Device event counters, each 64 slots 16 words apart (kernels/common.h): counter k at
+  // counters + k * kCounterWords: 0 unique MVs scored, 1 compound blocks, 2 inter candidates,
+  // 3 pruner fallbacks, 4 ME centres searched.
+  unsigned long long* counters = nullptr;
   unsigned long long* counters_host = nullptr;
   int nrefs_last = 0;
   bool launched = false;
@@ -351,8 +354,8 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     check(hipMalloc(&e->sb_cost, e->nsb() * sizeof(int64_t)), "hipMalloc sb_cost");
     check(hipMalloc(&e->rec_dev, e->nsb() * sizeof(celv2_sb)), "hipMalloc records");
     check(hipHostMalloc(&e->rec_host, e->nsb() * sizeof(celv2_sb), hipHostMallocDefault), "hipHostMalloc records");
-    check(hipMalloc(&e->counters, 80 * 16 * sizeof(unsigned long long)), "hipMalloc counters");
-    check(hipHostMalloc(reinterpret_cast<void**>(&e->counters_host), 80 * 16 * sizeof(unsigned long long), hipHostMallocDefault), "hipHostMalloc counters");
+    check(hipMalloc(&e->counters, kCounterBytes), "hipMalloc counters");
+    check(hipHostMalloc(reinterpret_cast<void**>(&e->counters_host), kCounterBytes, hipHostMallocDefault), "hipHostMalloc counters");
     e->row_done.resize(e->sb_rows);
     for (hipEvent_t& ev : e->row_done) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
     e->ring.resize(celv2hip_engine::kRing);
@@ -362,7 +365,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     for (auto& a : e->pring)
       for (hipEvent_t& ev : a) check(hipEventCreate(&ev), "hipEventCreate");
     e->ev = e->ring[0].data();
-    check(hipMemset(e->counters, 0, 80 * 16 * sizeof(unsigned long long)), "hipMemset counters");
+    check(hipMemset(e->counters, 0, kCounterBytes), "hipMemset counters");
     upload_intra_tiles(e->stream);
     upload_inter_tables(e->stream);
     upload_final_tiles(e->stream);
@@ -756,7 +759,7 @@ void launch_batch(const LaunchIn* in, int n) {
       a.nrefs = nr, a.cols = e->fcols[l], a.rows = e->frows[l];
       a.pcols = l + 1 < kLevels ? e->fcols[l + 1] : 0, a.prows = l + 1 < kLevels ? e->frows[l + 1] : 0;
       a.level0 = l == 0, a.phase_bytes = used[i][0]->p.phase_bytes;
-      a.searches = e->counters + 16;
+      a.searches = e->counters + 4 * kCounterWords;
       a.kmap = use_kmap[i] ? e->kmap : nullptr, a.sb_cols = e->sb_cols, a.sb_rows = e->sb_rows, a.level = l;
       const bool feats = l == 2 && (e->pruner || e->want_features);
       for (int r = 0; r < nr; r++) {
@@ -795,7 +798,7 @@ void launch_batch(const LaunchIn* in, int n) {
         e->features_valid = true;
         if (e->pruner) {
           cnn_infer(e->infer.x, 1, e->sb_rows, e->sb_cols, e->pruner->params, e->pruner->pb, nullptr, in[i].nrefs,
-                    e->prune_threshold, e->prune_band, e->kmap, e->counters + 3, S);
+                    e->prune_threshold, e->prune_band, e->kmap, e->counters + 3 * kCounterWords, S);
           use_kmap[i] = true;
         }
         check(hipGetLastError(), "pruner kernels");
@@ -811,7 +814,7 @@ void launch_batch(const LaunchIn* in, int n) {
     ia.nrefs = in[i].nrefs, ia.cols = e->fcols[0], ia.rows = e->frows[0], ia.sb_cols = e->sb_cols;
     ia.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
     ia.best = e->best;
-    ia.counter = e->counters, ia.candidates = e->counting ? e->counters + 2 : nullptr;
+    ia.counter = e->counters, ia.candidates = e->counting ? e->counters + 2 * kCounterWords : nullptr;
     ia.kmap = use_kmap[i] ? e->kmap : nullptr;
     for (int r = 0; r < in[i].nrefs; r++) ia.ref[r] = used[i][r]->p.lv[0], ia.field[r] = e->fields[r][0];
     return ia;
@@ -822,7 +825,7 @@ void launch_batch(const LaunchIn* in, int n) {
     fa.src = cur[i]->p.lv[0];
     fa.nrefs = in[i].nrefs, fa.sb_cols = e->sb_cols, fa.lam = in[i].lam;
     fa.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
-    fa.best = e->best, fa.intra = cur[i]->intra, fa.out = e->rec_dev, fa.counter = e->counters + 1;
+    fa.best = e->best, fa.intra = cur[i]->intra, fa.out = e->rec_dev, fa.counter = e->counters + kCounterWords;
     fa.kmap = use_kmap[i] ? e->kmap : nullptr, fa.sb_cost = e->sb_cost;
     for (int r = 0; r < in[i].nrefs; r++) fa.ref[r] = used[i][r]->p.lv[0];
     return fa;
@@ -1031,7 +1034,7 @@ int celv2hip_stats_reset(celv2hip_engine* e) {
   if (int rc = validate(e)) return rc;
   return guarded([&] {
     e->sync_all();
-    check(hipMemset(e->counters, 0, 80 * 16 * sizeof(unsigned long long)), "hipMemset counters");
+    check(hipMemset(e->counters, 0, kCounterBytes), "hipMemset counters");
     e->window_launch = e->launches, e->window_prepare = e->prepares;
   });
 }
@@ -1061,12 +1064,13 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
       acc[6] += t;
     }
     for (int k = 0; k < 7 && k < cap_ms; k++) ms[k] = acc[k];
-    check(hipMemcpy(e->counters_host, e->counters, 80 * 16 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "hipMemcpy counters");
-    unsigned long long searches = 0;
-    for (int i = 0; i < 64; i++) searches += e->counters_host[16 + 16 * i];
-    const long long v[6] = {n, static_cast<long long>(e->counters_host[0]), static_cast<long long>(searches),
-                            static_cast<long long>(e->counters_host[1]), static_cast<long long>(e->counters_host[2]),
-                            static_cast<long long>(e->counters_host[3])};
+    check(hipMemcpy(e->counters_host, e->counters, kCounterBytes, hipMemcpyDeviceToHost), "hipMemcpy counters");
+    auto total = [&](int k) {
+      long long t = 0;
+      for (int i = 0; i < kCounterSlots; i++) t += static_cast<long long>(e->counters_host[k * kCounterWords + kCounterStride * i]);
+      return t;
+    };
+    const long long v[6] = {n, total(0), total(4), total(1), total(2), total(3)};
     for (int k = 0; k < 6 && k < cap_counts; k++) counts[k] = v[k];
   });
   return rc ? -rc : static_cast<int>(n);

@@ -191,7 +191,7 @@ __global__ void __launch_bounds__(256) k_infer(Infer a) {
       if (a.kmap) {
         bool unsure;
         a.kmap[p] = static_cast<uint8_t>(limit_of(z, a.nrefs, a.threshold, a.band, unsure));
-        if (unsure && a.fallbacks) atomicAdd(a.fallbacks, 1ull);
+        if (unsure && a.fallbacks) count(a.fallbacks, static_cast<int>(p), 1ull);
       }
     }
   }
@@ -310,7 +310,7 @@ __global__ void k_limits(const float* z, int m, int nrefs, float threshold, floa
   if (p >= m) return;
   bool unsure;
   kmap[p] = static_cast<uint8_t>(limit_of(z + static_cast<size_t>(p) * kOut, nrefs, threshold, band, unsure));
-  if (unsure && fallbacks) atomicAdd(fallbacks, 1ull);
+  if (unsure && fallbacks) count(fallbacks, p, 1ull);
 }
 
 unsigned grid(long long n) { return static_cast<unsigned>((n + 255) / 256); }

@@ -190,6 +190,15 @@ __device__ __forceinline__ int mv_x(uint32_t m) { return static_cast<int16_t>(m 
 __device__ __forceinline__ int mv_y(uint32_t m) { return static_cast<int16_t>(m >> 16); }
 
 // Candidate-block geometry (src/gpu/sb_blocks.h:172-187).
+// Event counters are spread over 64 slots 16 words (128 bytes) apart, so that workgroups finishing
+// together do not queue on one address; the host sums the slots.
+constexpr int kCounterSlots = 64, kCounterStride = 16;
+constexpr int kCounterWords = kCounterSlots * kCounterStride;  // one counter
+constexpr int kCounterKinds = 5;                                // the engine's counters
+constexpr size_t kCounterBytes = static_cast<size_t>(kCounterKinds) * kCounterWords * sizeof(unsigned long long);
+__device__ __forceinline__ void count(unsigned long long* base, int slot, unsigned long long v) {
+  atomicAdd(base + kCounterStride * (slot & (kCounterSlots - 1)), v);
+}
 __device__ __forceinline__ int level_first(int level) { return ((1 << (2 * level)) - 1) / 3; }
 __device__ __forceinline__ void block_geom(int b, int& x, int& y, int& w, int& h) {
   const int sq = b / 5, kind = b - 5 * (b / 5);

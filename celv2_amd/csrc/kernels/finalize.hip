@@ -22,7 +22,7 @@ namespace {
 
 constexpr int kAllTiles = 3072;  // 4x4 tiles of all 425 blocks: per level 256 (whole) + 4 x 128 (halves)
 constexpr int64_t kMax = INT64_MAX;
-constexpr int kThreads = 512;  // 8 waves: two superblocks per CU still fit in one round
+constexpr int kThreads = 1024;  // 16 waves: the compound rows' gathers spread over more waves (2 superblocks per CU)
 
 __device__ uint32_t g_all_tiles[kAllTiles];  // b | tx << 16 | ty << 24
 __device__ uint8_t g_tile_seg[kAllTiles / 16];  // tiles per block (at most 16) of each 16-tile step
@@ -281,7 +281,7 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
     int k = 0;
     for (int b = tid; b < kBlocks; b += kThreads) k += (rs[b] >= 0 && pair_allowed(ra[b], rs[b], n)) ? 1 : 0;
     for (int m = 32; m >= 1; m >>= 1) k += __shfl_xor(k, m, 64);
-    if ((tid & 63) == 0 && k) atomicAdd(a.counter, static_cast<unsigned long long>(k));
+    if ((tid & 63) == 0 && k) count(a.counter, static_cast<int>(sb) * (kThreads / 64) + (tid >> 6), static_cast<unsigned long long>(k));
   }
 }
 

@@ -199,14 +199,14 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
     const int cell = g_cand_cell[b * kMaxCands + (key & 127u)];
     out[b] = make_uint2(key >> 7, mv[cell]);
   }
-  if (tid == 0 && a.counter) atomicAdd(a.counter, static_cast<unsigned long long>(U));
+  if (tid == 0 && a.counter) count(a.counter, sbx + 7 * sby + r, static_cast<unsigned long long>(U));
   if (a.candidates) {
     // Non-repeated candidates: per block, the distinct MVs among its candidates (a repeat is skipped
     // by the reference, repeated_candidate, src/gpu/inter_common.h:65-89), marked in the task loop.
     unsigned long long n = 0;
     for (int b = tid; b < kBlocks; b += 256) n += static_cast<unsigned long long>(__popcll(seen[b][0]) + __popcll(seen[b][1]));
     for (int m = 32; m >= 1; m >>= 1) n += static_cast<unsigned long long>(__shfl_xor(static_cast<long long>(n), m, 64));
-    if (lane == 0) atomicAdd(a.candidates, n);
+    if (lane == 0) count(a.candidates, 4 * (sbx + 7 * sby + r) + wave, n);
   }
 }
 

@@ -39,7 +39,7 @@ struct MeLevelArgs {
   uint32_t* out[kMaxRefs];
   int nrefs, cols, rows, pcols, prows, level0;
   long long phase_bytes;
-  unsigned long long* searches;  // centres searched, spread over 64 slots 16 words apart (may be null)
+  unsigned long long* searches;  // centres searched, a counter of 64 slots (count(), common.h) (may be null)
   // Per-superblock reference limits (may be null): superblock s uses references 0 .. kmap[s] - 1.
   // At levels 0 and 1 a block is searched for reference r only when a superblock that keeps r reads
   // it: its own, or a neighbour whose ring it lies in (1 block at level 0, 2 at level 1).
@@ -70,8 +70,8 @@ struct InterArgs {
   int nrefs, cols, rows, sb_cols, row0, nrows;
   long long phase_bytes;
   uint2* best;
-  unsigned long long* counter;     // unique MVs scored (may be null)
-  unsigned long long* candidates;  // non-repeated (block, MV) candidates scored (may be null)
+  unsigned long long* counter;     // unique MVs scored, a counter of 64 slots (may be null)
+  unsigned long long* candidates;  // non-repeated (block, MV) candidates scored, 64 slots (may be null)
   const uint8_t* kmap;             // per-superblock reference limits (may be null)
 };
 void add_inter(Batch<InterArgs>& b, const InterArgs& a);
@@ -88,7 +88,7 @@ struct FinalizeArgs {
   const uint2* best;       // from launch_inter (unused when nrefs == 0)
   const uint32_t* intra;   // from launch_intra
   void* out;               // celv2_sb records, one per superblock of the frame
-  unsigned long long* counter;  // compound blocks scored (may be null)
+  unsigned long long* counter;  // compound blocks scored, a counter of 64 slots (may be null)
   const uint8_t* kmap;     // per-superblock reference limits (may be null): n = min(nrefs, kmap[sb])
   int64_t* sb_cost;        // the superblock's decision cost (the 64x64 square's DP cost) (may be null)
 };

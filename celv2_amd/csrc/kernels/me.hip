@@ -273,7 +273,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   if (lane == 0) {
     if (a.cost_out[r]) a.cost_out[r][by * a.cols + bx] = best_cost;
     a.out[r][by * a.cols + bx] = pack_mv(mvx, mvy);
-    if (a.searches) atomicAdd(a.searches + 16 * ((bx + 7 * by + r) & 63), two ? 2ull : 1ull);
+    if (a.searches) count(a.searches, bx + 7 * by + r, two ? 2ull : 1ull);
   }
 }
 
