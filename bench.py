@@ -183,11 +183,8 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
         step(s)
     E.sync()
     wall = time.perf_counter() - t0
-    ms, cnt = E.stats()
     fps = steps / wall
-    n = max(1, cnt["launches"])
-    per = {k: round(v / n, 4) for k, v in ms.items() if k != "intra_prepare"}
-    per["intra"] = round(ms["intra_prepare"] / n, 4)
+    per = {k: round(v, 4) for k, v in stage_times(E, step, warmup + steps, steps).items()}
     # Untimed: inter candidates counted, regret, limits and fallbacks on every decidable frame.
     E.set_counting(True)
     E.stats_reset()
@@ -218,6 +215,21 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
             "threshold": threshold, "band": band, "stage_ms_per_frame": per,
             "inter_candidates_per_frame": round(inter_l), "compound_blocks_per_frame": round(c2["compound_blocks"] / nd),
             "network": f"conv3x3 11-32, conv3x3 32-32, conv1x1 32-6, BF16 on MFMA, trained on {cfg['train_superblocks']} superblocks of synthetic clips (celv2_amd/pruner/train.py)"}
+
+
+def stage_times(E, step, first, steps):
+    """Per-frame stage times (ms) over `steps` more calls of step(s) from s = first, with the stage
+    events on for that window only."""
+    E.set_timing(True)
+    E.stats_reset()
+    for s in range(first, first + steps):
+        step(s)
+    ms, cnt = E.stats()
+    E.set_timing(False)
+    n = max(1, cnt["launches"])
+    per = {k: v / n for k, v in ms.items() if k != "intra_prepare"}
+    per["intra"] = ms["intra_prepare"] / n
+    return per
 
 
 def _timed(step, engines, prepare, steps, warmup):
@@ -510,14 +522,16 @@ def main():
         last = step(s)
     E.sync()
     wall = time.perf_counter() - t0
-    ms, cnt = E.stats()
+    _, cnt = E.stats()
     fps = args.steps / wall
     n = cnt["launches"]
-    per = {k: v / max(1, n) for k, v in ms.items() if k != "intra_prepare"}
-    per["intra"] = ms["intra_prepare"] / max(1, n)
-    log(f"timed {args.steps} frames in {wall:.3f}s: {fps:.1f} frames/s; per frame ms {json.dumps({k: round(v, 3) for k, v in per.items()})}")
-    log(f"counts over the window: {cnt}")
     gpu_map = E.fetch()
+    # Stage times: the same number of frames again with the stage events on (each event between two
+    # kernels costs the stream a few microseconds, so they stay out of the timed window).
+    per = stage_times(E, step, args.warmup + args.steps, args.steps)
+    log(f"timed {args.steps} frames in {wall:.3f}s: {fps:.1f} frames/s; per frame ms (separate window with stage events) "
+        f"{json.dumps({k: round(v, 3) for k, v in per.items()})}")
+    log(f"counts over the window: {cnt}")
     # Non-repeated inter candidates per frame, counted on 4 more frames outside the timed window.
     E.set_counting(True)
     E.stats_reset()

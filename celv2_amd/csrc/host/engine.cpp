@@ -164,6 +164,10 @@ struct celv2hip_engine {
   // Per-launch stage events in a ring (kRing launches): start, references built, motion search,
   // inter scores, finalize, end; and per prepare(): intra scores start / end.
   static constexpr int kRing = 512;
+  // Stage events are recorded only with timing on (celv2hip_set_timing): each record between two
+  // kernels costs the stream a few microseconds. Launches since the last reset without them.
+  bool timing = false;
+  long long window_untimed = 0;
   std::vector<std::array<hipEvent_t, 6>> ring;
   std::vector<std::array<hipEvent_t, 2>> pring;
   hipEvent_t* ev = nullptr;  // the current launch's 6 events
@@ -638,10 +642,13 @@ void run_intra(const std::vector<std::pair<celv2hip_engine*, SrcSlot*>>& items, 
       add_intra(b, e->intra_item(*items[j].second));
       rings.push_back(&e->pring[e->prepares++ % celv2hip_engine::kRing]);
     }
-    for (auto* r : rings) check(hipEventRecord((*r)[0], st), "hipEventRecord");
+    const bool timed = items[j0].first->timing;
+    if (timed)
+      for (auto* r : rings) check(hipEventRecord((*r)[0], st), "hipEventRecord");
     launch_intra(b, st);
     check(hipGetLastError(), "intra kernel");
-    for (auto* r : rings) check(hipEventRecord((*r)[1], st), "hipEventRecord");
+    if (timed)
+      for (auto* r : rings) check(hipEventRecord((*r)[1], st), "hipEventRecord");
   }
 }
 
@@ -704,7 +711,8 @@ void launch_batch(const LaunchIn* in, int n) {
     const LaunchIn& x = in[i];
     celv2hip_engine* e = x.e;
     e->ev = e->ring[e->launches % celv2hip_engine::kRing].data();
-    check(hipEventRecord(e->ev[0], S), "hipEventRecord");
+    if (!e->timing) e->window_untimed++;
+    if (e->timing) check(hipEventRecord(e->ev[0], S), "hipEventRecord");
     int use = -1;
     for (int k = 0; k < 2; k++)
       if (x.src_key >= 0 && e->srcs[k].key == x.src_key) use = k;
@@ -745,7 +753,7 @@ void launch_batch(const LaunchIn* in, int n) {
   }
   run_builds(jobs, S);
   run_intra(intra_now, S);
-  for (int i = 0; i < n; i++) check(hipEventRecord(in[i].e->ev[1], S), "hipEventRecord");
+  for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[1], S), "hipEventRecord");
 
   // Motion search, top level first, every reference of every engine per launch.
   for (int l = kLevels - 1; l >= 0; l--) {
@@ -805,7 +813,7 @@ void launch_batch(const LaunchIn* in, int n) {
       }
     }
   }
-  for (int i = 0; i < n; i++) check(hipEventRecord(in[i].e->ev[2], S), "hipEventRecord");
+  for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[2], S), "hipEventRecord");
 
   auto inter_args = [&](int i) {
     celv2hip_engine* e = in[i].e;
@@ -852,7 +860,7 @@ void launch_batch(const LaunchIn* in, int n) {
       if (bi.n == kMaxItems) launch_inter(bi, S), bi = Batch<InterArgs>();
     }
     launch_inter(bi, S);
-    for (int i = 0; i < n; i++) check(hipEventRecord(in[i].e->ev[3], S), "hipEventRecord");
+    for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[3], S), "hipEventRecord");
     Batch<FinalizeArgs> bf;
     for (int i = 0; i < n; i++) {
       FinalizeArgs fa = finalize_args(i);
@@ -864,7 +872,7 @@ void launch_batch(const LaunchIn* in, int n) {
     check(hipGetLastError(), "row kernels");
     for (int i = 0; i < n; i++) {
       celv2hip_engine* e = in[i].e;
-      check(hipEventRecord(e->ev[4], S), "hipEventRecord");
+      if (e->timing) check(hipEventRecord(e->ev[4], S), "hipEventRecord");
       if (!e->resident) {
         const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
         check(hipMemcpyAsync(e->rec_host, e->rec_dev, row_bytes * e->sb_rows, hipMemcpyDeviceToHost, S), "hipMemcpyAsync records");
@@ -894,12 +902,12 @@ void launch_batch(const LaunchIn* in, int n) {
                            hipMemcpyDeviceToHost, S), "hipMemcpyAsync row");
       check(hipEventRecord(e->row_done[r], S), "hipEventRecord");
     }
-    check(hipEventRecord(e->ev[3], S), "hipEventRecord");
-    check(hipEventRecord(e->ev[4], S), "hipEventRecord");
+    if (e->timing) check(hipEventRecord(e->ev[3], S), "hipEventRecord");
+    if (e->timing) check(hipEventRecord(e->ev[4], S), "hipEventRecord");
   }
   for (int i = 0; i < n; i++) {
     celv2hip_engine* e = in[i].e;
-    check(hipEventRecord(e->ev[5], S), "hipEventRecord");
+    if (e->timing) check(hipEventRecord(e->ev[5], S), "hipEventRecord");
     e->release(*cur[i], S);
     e->nrefs_last = in[i].nrefs;
     e->limits_last = use_kmap[i] != 0;
@@ -1035,7 +1043,7 @@ int celv2hip_stats_reset(celv2hip_engine* e) {
   return guarded([&] {
     e->sync_all();
     check(hipMemset(e->counters, 0, kCounterBytes), "hipMemset counters");
-    e->window_launch = e->launches, e->window_prepare = e->prepares;
+    e->window_launch = e->launches, e->window_prepare = e->prepares, e->window_untimed = 0;
   });
 }
 
@@ -1048,7 +1056,7 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
     const long long np = e->prepares - e->window_prepare;
     if (n > celv2hip_engine::kRing || np > celv2hip_engine::kRing) throw Failure(CELV2_ERR_STATE, "stats window longer than the event ring");
     double acc[7] = {};
-    for (long long i = e->window_launch; i < e->launches; i++) {
+    for (long long i = e->window_launch; i < e->launches && !e->window_untimed; i++) {
       auto& a = e->ring[i % celv2hip_engine::kRing];
       const int pairs[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {0, 5}};
       for (int k = 0; k < 6; k++) {
@@ -1057,7 +1065,7 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
         acc[k] += t;
       }
     }
-    for (long long i = e->window_prepare; i < e->prepares; i++) {
+    for (long long i = e->window_prepare; i < e->prepares && !e->window_untimed; i++) {
       auto& a = e->pring[i % celv2hip_engine::kRing];
       float t = 0;
       check(hipEventElapsedTime(&t, a[0], a[1]), "hipEventElapsedTime");
@@ -1074,6 +1082,11 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
     for (int k = 0; k < 6 && k < cap_counts; k++) counts[k] = v[k];
   });
   return rc ? -rc : static_cast<int>(n);
+}
+
+int celv2hip_set_timing(celv2hip_engine* e, int on) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] { e->timing = on != 0; });
 }
 
 int celv2hip_device_alloc(int device, long long bytes, void** out) {

@@ -50,6 +50,7 @@ def lib():
         L.celv2hip_fetch_row.argtypes = [P, I, P]
         L.celv2hip_sync.argtypes = [P]
         L.celv2hip_stats_reset.argtypes = [P]
+        L.celv2hip_set_timing.argtypes = [P, I]
         L.celv2hip_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), I, ctypes.POINTER(LL), I]
         L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
@@ -325,6 +326,10 @@ class RawEngine:
 
     def set_counting(self, on):
         self._check(lib().celv2hip_set_counting(self.h, 1 if on else 0), "set_counting")
+
+    def set_timing(self, on):
+        """Stage events for stats() ms (off by default; they cost the stream a few microseconds each)."""
+        self._check(lib().celv2hip_set_timing(self.h, 1 if on else 0), "set_timing")
 
     def stats_reset(self):
         self._check(lib().celv2hip_stats_reset(self.h), "stats_reset")

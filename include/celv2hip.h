@@ -150,6 +150,10 @@ int celv2hip_sync(celv2hip_engine* e);
  * celv2hip_set_counting), [5] superblocks the pruner sent to its fallback. Returns the number of
  * launches (negative status on failure). */
 int celv2hip_stats_reset(celv2hip_engine* e);
+/* on != 0: later launches and prepares record the stage events the ms figures come from (off by
+ * default: each record between two kernels costs the stream a few microseconds). ms are all zero
+ * for a window with any launch made with timing off; counts are kept either way. */
+int celv2hip_set_timing(celv2hip_engine* e, int on);
 int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts, int cap_counts);
 
 /* ---- Timing mode: device-resident throughput ------------------------------------------------ */
