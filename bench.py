@@ -34,11 +34,11 @@ VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 lanes x 2.4 GHz, one in
 SAD_OPS, SATD_OPS = 3, 7  # int ops per pixel (sub, abs, add; diff, 2 butterfly passes, abs, add)
 # The motion search's ceiling: v_sad_u8 takes 4 byte pairs per lane per instruction, so the chip's
 # published clock gives 256 CUs x 64 lanes x 4 x 2.4 GHz pixel-SADs per second; a microbenchmark of
-# independent v_sad_u8 chains on the MI355X reached 123.9 T/s (profiles/sad_u8_rate.md).
-SAD_PEAK_TPX, SAD_MEASURED_TPX = 256 * 64 * 4 * 2.4e9 / 1e12, 123.9
+# independent v_sad_u8 chains on the MI355X reached 125.4 T/s (profiles/sad_u8_rate.md).
+SAD_PEAK_TPX, SAD_MEASURED_TPX = 256 * 64 * 4 * 2.4e9 / 1e12, 125.4
 # HBM bytes the five motion-search launches of one frame move (rocprofv3 FETCH_SIZE and WRITE_SIZE
 # per dispatch, scaled by a 64 MiB device copy: profiles/me_traffic_1080p10_7refs.md).
-ME_TRAFFIC_BYTES = 355.0e6
+ME_TRAFFIC_BYTES = 351.2e6
 
 
 def log(*a):
