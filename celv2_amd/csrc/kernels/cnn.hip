@@ -104,22 +104,29 @@ struct Infer {
   float threshold, band;
 };
 
-__global__ void __launch_bounds__(256) k_infer(Infer a) {
+constexpr int kInferThreads = 512;  // 8 waves: layer 2's 8 tiles one per wave
+__global__ void __launch_bounds__(kInferThreads) k_infer(Infer a) {
   __shared__ uint16_t xs[kXH * kXW * kFeatures];
   __shared__ uint16_t z1s[kZ1 * kHidden];
   __shared__ uint16_t z2s[kTileH * kTileW * kHidden];
   __shared__ float z3s[kTileH * kTileW * 8];
+  // The BF16 weights in LDS (the products read them at every step).
+  __shared__ uint16_t w1s[9 * kFeatures * kHidden], w2s[kK2 * kHidden], w3s[kHidden * kOut];
   const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, i = lane & 15, q = lane >> 4;
+  constexpr int kWaves = kInferThreads / 64;
+  for (int e = tid; e < 9 * kFeatures * kHidden; e += kInferThreads) w1s[e] = a.pb[kW1 + e];
+  for (int e = tid; e < kK2 * kHidden; e += kInferThreads) w2s[e] = a.pb[kW2 + e];
+  for (int e = tid; e < kHidden * kOut; e += kInferThreads) w3s[e] = a.pb[kW3 + e];
   const int tx0 = static_cast<int>(blockIdx.x) * kTileW, ty0 = static_cast<int>(blockIdx.y) * kTileH;
   const size_t frame = static_cast<size_t>(blockIdx.z) * a.h * a.w;
-  for (int e = tid; e < kXH * kXW * kFeatures; e += 256) {
+  for (int e = tid; e < kXH * kXW * kFeatures; e += kInferThreads) {
     const int c = e % kFeatures, pos = e / kFeatures, yy = ty0 - 2 + pos / kXW, xx = tx0 - 2 + pos % kXW;
     const bool in = yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
     xs[e] = bf16_bits(in ? a.x[(frame + static_cast<size_t>(yy) * a.w + xx) * kFeatures + c] : 0.f);
   }
   __syncthreads();
   // Layer 1 on the 6 x 18 region (7 row tiles x 2 column tiles); 0 outside the grid (layer 2's padding).
-  for (int t = wave; t < 14; t += 4) {
+  for (int t = wave; t < 14; t += kWaves) {
     const int tm = t >> 1, tn = t & 1, m = 16 * tm + i, col = 16 * tn + i;
     const int ry = m / kZ1W, rx = m - ry * kZ1W;
     f32x4v acc = {0.f, 0.f, 0.f, 0.f};
@@ -130,7 +137,7 @@ __global__ void __launch_bounds__(256) k_infer(Infer a) {
         const int kk = k0 + 4 * q + j, tap = kk / kFeatures, c = kk - tap * kFeatures, dy = tap / 3 - 1, dx = tap % 3 - 1;
         const bool ok = kk < 9 * kFeatures;
         av[j] = static_cast<short>(ok && m < kZ1 ? xs[((ry + 1 + dy) * kXW + rx + 1 + dx) * kFeatures + c] : 0);
-        bv[j] = static_cast<short>(ok ? a.pb[kW1 + kk * kHidden + col] : 0);
+        bv[j] = static_cast<short>(ok ? w1s[kk * kHidden + col] : 0);
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
@@ -145,7 +152,7 @@ __global__ void __launch_bounds__(256) k_infer(Infer a) {
   }
   __syncthreads();
   // Layer 2: row tile tm = tile row, its 16 rows = the tile's columns.
-  for (int t = wave; t < 2 * kTileH; t += 4) {
+  for (int t = wave; t < 2 * kTileH; t += kWaves) {
     const int tm = t >> 1, tn = t & 1, col = 16 * tn + i;
     f32x4v acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < 9 * kHidden; k0 += 16) {
@@ -154,7 +161,7 @@ __global__ void __launch_bounds__(256) k_infer(Infer a) {
       for (int j = 0; j < 4; j++) {
         const int kk = k0 + 4 * q + j, tap = kk / kHidden, c = kk - tap * kHidden, dy = tap / 3 - 1, dx = tap % 3 - 1;
         av[j] = static_cast<short>(z1s[((tm + 1 + dy) * kZ1W + i + 1 + dx) * kHidden + c]);
-        bv[j] = static_cast<short>(a.pb[kW2 + kk * kHidden + col]);
+        bv[j] = static_cast<short>(w2s[kk * kHidden + col]);
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
@@ -163,7 +170,7 @@ __global__ void __launch_bounds__(256) k_infer(Infer a) {
   }
   __syncthreads();
   // Layer 3 (1x1, 6 outputs): one row tile per wave.
-  {
+  if (wave < kTileH) {
     const int tm = wave;
     f32x4v acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < kHidden; k0 += 16) {
@@ -172,7 +179,7 @@ __global__ void __launch_bounds__(256) k_infer(Infer a) {
       for (int j = 0; j < 4; j++) {
         const int kk = k0 + 4 * q + j;
         av[j] = static_cast<short>(z2s[(tm * kTileW + i) * kHidden + kk]);
-        bv[j] = static_cast<short>(i < kOut ? a.pb[kW3 + kk * kOut + i] : 0);
+        bv[j] = static_cast<short>(i < kOut ? w3s[kk * kOut + i] : 0);
       }
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
     }
@@ -402,7 +409,7 @@ void cnn_infer(const float* x, int b, int h, int wd, const float* params, const 
                float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s) {
   if (b <= 0 || h <= 0 || wd <= 0) return;
   const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band};
-  k_infer<<<dim3((wd + kTileW - 1) / kTileW, (h + kTileH - 1) / kTileH, b), 256, 0, s>>>(a);
+  k_infer<<<dim3((wd + kTileW - 1) / kTileW, (h + kTileH - 1) / kTileH, b), kInferThreads, 0, s>>>(a);
 }
 
 void cnn_limits(const float* logits, int m, int nrefs, float threshold, float band, uint8_t* kmap,
