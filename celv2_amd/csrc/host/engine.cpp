@@ -130,6 +130,11 @@ struct celv2hip_engine {
   int width, height, sb_size, device, pw, ph, sb_cols, sb_rows;
   hipStream_t stream = nullptr;  // launches
   hipStream_t side = nullptr;    // prepare(): the next source's planes and intra scores
+  // A new reference's 15 phase planes, built beside the coarse pyramid levels and the motion search
+  // of levels 4..1 (only level 0's refinement, inter and compound scores read them): sub_start is
+  // recorded on the launch stream after the import, sub_done on this stream after the phase planes.
+  hipStream_t aux = nullptr;
+  hipEvent_t sub_start = nullptr, sub_done = nullptr;
   hipEvent_t order_ev = nullptr;  // orders this engine's stream around batched launches
   // Recorded before the last launch's level-0 motion search: prepare() starts the next source's
   // intra scores there, beside the long latency-bound launches (level 0, inter, finalize) rather than
@@ -185,6 +190,7 @@ struct celv2hip_engine {
     if (device >= 0) (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     if (side) (void)hipStreamSynchronize(side);
+    if (aux) (void)hipStreamSynchronize(aux);
     auto free_up = [](Upload& u) {
       if (u.host) (void)hipHostFree(u.host);
       if (u.dev) (void)hipFree(u.dev);
@@ -221,6 +227,9 @@ struct celv2hip_engine {
       for (hipEvent_t e : a) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
     if (side) (void)hipStreamDestroy(side);
+    if (aux) (void)hipStreamDestroy(aux);
+    if (sub_start) (void)hipEventDestroy(sub_start);
+    if (sub_done) (void)hipEventDestroy(sub_done);
     if (order_ev) (void)hipEventDestroy(order_ev);
     if (deep_ev) (void)hipEventDestroy(deep_ev);
   }
@@ -228,6 +237,7 @@ struct celv2hip_engine {
   void sync_all() {
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     check(hipStreamSynchronize(side), "hipStreamSynchronize side");
+    check(hipStreamSynchronize(aux), "hipStreamSynchronize aux");
   }
 
   size_t nsb() const { return static_cast<size_t>(sb_cols) * sb_rows; }
@@ -342,6 +352,9 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     if (e->name.empty()) e->name = "HIP device " + std::to_string(device);
     check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&e->sub_start, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&e->sub_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->deep_ev, hipEventDisableTiming), "hipEventCreate");
     for (SrcSlot& s : e->srcs) {
@@ -610,8 +623,11 @@ int celv2hip_set_counting(celv2hip_engine* e, int on) {
 namespace {
 
 // Plane builds of any engines in batched launches: every import, then every reference's phase
-// planes, then each coarser level of every pyramid.
-void run_builds(const std::vector<celv2hip_engine::BuildJob>& jobs, hipStream_t st) {
+// planes, then each coarser level of every pyramid. With `e` (the launching engine), the phase planes
+// go to its aux stream after the imports and the caller waits for e->sub_done before they are read;
+// returns whether any were queued there.
+bool run_builds(const std::vector<celv2hip_engine::BuildJob>& jobs, hipStream_t st, celv2hip_engine* e = nullptr) {
+  bool on_aux = false;
   for (size_t j0 = 0; j0 < jobs.size(); j0 += kMaxItems) {
     const size_t j1 = std::min(jobs.size(), j0 + kMaxItems);
     Batch<ImportItem> bi;
@@ -621,7 +637,15 @@ void run_builds(const std::vector<celv2hip_engine::BuildJob>& jobs, hipStream_t 
       if (jobs[j].phases) add_subpel(bs, SubpelItem{jobs[j].p->lv[0], jobs[j].p->phase_bytes});
     }
     launch_import(bi, st);
-    launch_subpel(bs, st);
+    if (e && bs.total() > 0) {
+      check(hipEventRecord(e->sub_start, st), "hipEventRecord");
+      check(hipStreamWaitEvent(e->aux, e->sub_start, 0), "hipStreamWaitEvent");
+      launch_subpel(bs, e->aux);
+      check(hipEventRecord(e->sub_done, e->aux), "hipEventRecord");
+      on_aux = true;
+    } else {
+      launch_subpel(bs, st);
+    }
     for (int l = 1; l < kLevels; l++) {
       Batch<DownItem> bd;
       for (size_t j = j0; j < j1; j++) add_downsample(bd, DownItem{jobs[j].p->lv[l - 1], jobs[j].p->lv[l]});
@@ -629,6 +653,7 @@ void run_builds(const std::vector<celv2hip_engine::BuildJob>& jobs, hipStream_t 
     }
   }
   check(hipGetLastError(), "plane kernels");
+  return on_aux;
 }
 
 // Intra scores of the given engines' source slots, with each engine's prepare-ring events around.
@@ -751,7 +776,8 @@ void launch_batch(const LaunchIn* in, int n) {
       used[i][ri] = slot;
     }
   }
-  run_builds(jobs, S);
+  // Only one batch of jobs per aux stream use: more than kMaxItems jobs build on S itself.
+  const bool phases_pending = jobs.size() <= static_cast<size_t>(kMaxItems) ? run_builds(jobs, S, e0) : run_builds(jobs, S);
   run_intra(intra_now, S);
   for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[1], S), "hipEventRecord");
 
@@ -781,11 +807,13 @@ void launch_batch(const LaunchIn* in, int n) {
       add_me_level(b, a);
       if (b.n == kMaxItems) launch_me_level(b, S), b = Batch<MeLevelArgs>();
     }
-    if (l == 0)
+    if (l == 0) {
       for (int i = 0; i < n; i++) {
         check(hipEventRecord(in[i].e->deep_ev, S), "hipEventRecord");
         in[i].e->deep_pending = true;
       }
+      if (phases_pending) check(hipStreamWaitEvent(S, e0->sub_done, 0), "hipStreamWaitEvent");  // level 0 refines on them
+    }
     launch_me_level(b, S);
     check(hipGetLastError(), "me kernel");
     if (l == 2) {
