@@ -26,8 +26,10 @@ __device__ uint16_t g_cell_first[kRegion * kRegion + 1];  // tasks of region cel
 __device__ uint16_t g_cell_tasks[kTasks];                 // b << 7 | k
 __device__ uint8_t g_cand_cell[kBlocks * kMaxCands];     // region cell of candidate k of block b
 
+// v + v of lane ^ 16 (m = 16) or lane ^ 32 (m = 32) by gfx950's row swaps (VALU, no LDS).
 __device__ __forceinline__ uint32_t xor_add(uint32_t v, int m) {
-  return v + static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), m, 64));
+  const auto r = m == 16 ? __builtin_amdgcn_permlane16_swap(v, v, false, false) : __builtin_amdgcn_permlane32_swap(v, v, false, false);
+  return r[0] + r[1];
 }
 // Lane grid of the superblock's 8x8 squares: lane = 8 * sqy + sqx. Neighbour sums within a 16-lane
 // row by DPP: lane ^ 1 (next square across), lane ^ 8 (next square down), lane ^ 2, and lane ^ 4 for
