@@ -942,9 +942,11 @@ void launch_batch(const LaunchIn* in, int n) {
     e->last_refs = used[i];
     e->launched = true;
   }
-  for (int i = 1; i < n; i++) {
+  // One event on S orders every other engine's later work after the batch (each record between
+  // kernels costs S a few microseconds).
+  if (n > 1) {
     check(hipEventRecord(e0->order_ev, S), "hipEventRecord");
-    check(hipStreamWaitEvent(in[i].e->stream, e0->order_ev, 0), "hipStreamWaitEvent");
+    for (int i = 1; i < n; i++) check(hipStreamWaitEvent(in[i].e->stream, e0->order_ev, 0), "hipStreamWaitEvent");
   }
 }
 
