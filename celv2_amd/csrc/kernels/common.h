@@ -89,101 +89,6 @@ __device__ __forceinline__ uint32_t bload4(const BPlane& b, int voff, int soff) 
 
 __device__ __forceinline__ uint32_t sad4(uint32_t a, uint32_t b, uint32_t acc) { return __builtin_amdgcn_sad_u8(a, b, acc); }
 
-// Sum of |4x4 Walsh-Hadamard coefficients| of d (row-major), unnormalised
-// (hadamard4x4, src/gpu/intra_common.h:131-146).
-__device__ __forceinline__ int wht4_abs_sum(const int d[16]) {
-  int r[16];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int s01 = d[4 * i] + d[4 * i + 1], d01 = d[4 * i] - d[4 * i + 1];
-    const int s23 = d[4 * i + 2] + d[4 * i + 3], d23 = d[4 * i + 2] - d[4 * i + 3];
-    r[4 * i] = s01 + s23, r[4 * i + 1] = d01 + d23, r[4 * i + 2] = s01 - s23, r[4 * i + 3] = d01 - d23;
-  }
-  int t = 0;
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const int s01 = r[c] + r[4 + c], d01 = r[c] - r[4 + c], s23 = r[8 + c] + r[12 + c], d23 = r[8 + c] - r[12 + c];
-    t += abs(s01 + s23) + abs(d01 + d23) + abs(s01 - s23) + abs(d01 - d23);
-  }
-  return t;
-}
-
-// SATD of one 4x4 tile from four rows of packed bytes: cur c[i] against prediction p[i].
-__device__ __forceinline__ int satd4x4_rows(const uint32_t c[4], const uint32_t p[4]) {
-  int d[16];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) d[4 * i + j] = static_cast<int>((c[i] >> (8 * j)) & 255) - static_cast<int>((p[i] >> (8 * j)) & 255);
-  return wht4_abs_sum(d);
-}
-
-// ---- Two 4x4 SATDs at once in packed 16-bit arithmetic ----------------------------------------------
-// Tile A's samples go to the low halves, tile B's to the high halves; every butterfly is one
-// v_pk_add/sub_u16 (int16 wrap-around equals the exact value: |coefficient| <= 4080). The last
-// stage uses |a + b| + |a - b| = 2 max(|a|, |b|), so each column needs 2 maxima of absolute values.
-typedef short s16x2 __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ s16x2 as_s16x2(uint32_t v) { return __builtin_bit_cast(s16x2, v); }
-__device__ __forceinline__ uint32_t as_u32(s16x2 v) { return __builtin_bit_cast(uint32_t, v); }
-// Byte j of a into the low half, byte j of b into the high half (zero-extended).
-template <int j>
-__device__ __forceinline__ s16x2 pair_bytes(uint32_t a, uint32_t b) {
-  constexpr uint32_t sel = static_cast<uint32_t>(j) | (0x0Cu << 8) | ((4u + j) << 16) | (0x0Cu << 24);
-  return as_s16x2(__builtin_amdgcn_perm(b, a, sel));
-}
-__device__ __forceinline__ s16x2 abs2(s16x2 v) {
-  const s16x2 n = s16x2{0, 0} - v;
-  return __builtin_elementwise_max(v, n);
-}
-// The current block's two tiles as packed pairs (hoisted out of loops over predictions).
-struct TilePair {
-  s16x2 v[4][4];
-};
-__device__ __forceinline__ TilePair tile_pair(const uint32_t ca[4], const uint32_t cb[4]) {
-  TilePair t;
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    t.v[i][0] = pair_bytes<0>(ca[i], cb[i]), t.v[i][1] = pair_bytes<1>(ca[i], cb[i]);
-    t.v[i][2] = pair_bytes<2>(ca[i], cb[i]), t.v[i][3] = pair_bytes<3>(ca[i], cb[i]);
-  }
-  return t;
-}
-// Packed sums of |coefficients| / 2 of the differences cur - pred (tile A low half, B high half).
-__device__ __forceinline__ s16x2 wht_pair_half(const s16x2 d[4][4]) {
-  s16x2 r[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const s16x2 s01 = d[i][0] + d[i][1], d01 = d[i][0] - d[i][1], s23 = d[i][2] + d[i][3], d23 = d[i][2] - d[i][3];
-    r[i][0] = s01 + s23, r[i][1] = d01 + d23, r[i][2] = s01 - s23, r[i][3] = d01 - d23;
-  }
-  s16x2 acc = {0, 0};
-#pragma unroll
-  for (int c = 0; c < 4; c++) {
-    const s16x2 s01 = r[0][c] + r[1][c], d01 = r[0][c] - r[1][c], s23 = r[2][c] + r[3][c], d23 = r[2][c] - r[3][c];
-    acc += __builtin_elementwise_max(abs2(s01), abs2(s23)) + __builtin_elementwise_max(abs2(d01), abs2(d23));
-  }
-  return acc;
-}
-__device__ __forceinline__ uint32_t unpack_satd_pair(s16x2 acc) {
-  const uint32_t a = as_u32(acc);
-  return ((a & 0xffffu) << 1) | ((a >> 16) << 17);
-}
-// SATD(A) | SATD(B) << 16 of the current tiles c against predicted rows pa (tile A) and pb (tile B).
-__device__ __forceinline__ uint32_t satd4x4_pair(const TilePair& c, const uint32_t pa[4], const uint32_t pb[4]) {
-  s16x2 d[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    d[i][0] = c.v[i][0] - pair_bytes<0>(pa[i], pb[i]), d[i][1] = c.v[i][1] - pair_bytes<1>(pa[i], pb[i]);
-    d[i][2] = c.v[i][2] - pair_bytes<2>(pa[i], pb[i]), d[i][3] = c.v[i][3] - pair_bytes<3>(pa[i], pb[i]);
-  }
-  return unpack_satd_pair(wht_pair_half(d));
-}
-__device__ __forceinline__ uint32_t satd4x4_pair(const uint32_t ca[4], const uint32_t pa[4], const uint32_t cb[4],
-                                                 const uint32_t pb[4]) {
-  return satd4x4_pair(tile_pair(ca, cb), pa, pb);
-}
-
 // Quarter-pel MVs are packed as (int16 mvx) | (int16 mvy) << 16.
 __device__ __forceinline__ uint32_t pack_mv(int x, int y) { return (static_cast<uint32_t>(x) & 0xffffu) | (static_cast<uint32_t>(y) << 16); }
 __device__ __forceinline__ int mv_x(uint32_t m) { return static_cast<int16_t>(m & 0xffffu); }

@@ -6,16 +6,21 @@
 // superblock (its 64 cells and the ring, src/gpu/inter_common.h:225-243), and every candidate block's
 // 4x4 tiles lie on the superblock's 4x4 grid. So per distinct MV the kernel scores the whole 64x64
 // area once - 256 tile SATDs - and sums tiles up the block quadtree; each block's cost for any
-// candidate is then a lookup. A workgroup takes one (superblock, reference); each of its 4 waves
-// takes every fourth distinct MV on its own, with no workgroup barrier between MVs, the ref rows
-// through buffer loads and the cell -> (block, candidate) tables from LDS. Within a block, candidate
-// k's cost is (SATD << 7 | k) and the minimum of
-// those keys (LDS atomicMin, order-free) is the first strict minimum in candidate order, exactly the
+// candidate is then a lookup. The tile transforms run on the matrix cores (hadamard.h): per MV, 16
+// products of HH with the source tiles' halves, each continued by the product of -HH with the
+// prediction's, so the coefficients of cur - pred come out exact in f32 and the VALU only forms the
+// prediction operands and sums |coefficients| (by v_permlane16/32_swap across lane rows and DPP).
+// A workgroup takes one (superblock, reference); each of its 4 waves takes every fourth distinct MV
+// on its own, with no workgroup barrier between MVs, the ref rows through buffer loads and the
+// cell -> (block, candidate) tables from LDS. Within a block, candidate k's cost is (SATD << 7 | k)
+// and the minimum of those keys (LDS atomicMin, order-free) is the first strict minimum in candidate
+// order, exactly the
 // reference's choice (a repeated MV costs the same as its first occurrence and never wins; the
 // lambda * (3 + r) term is the same for every candidate of a reference and cannot change the order).
 #include <stdexcept>
 
 #include "common.h"
+#include "hadamard.h"
 #include "launch.h"
 
 namespace celv2hip {
@@ -26,14 +31,14 @@ __device__ uint16_t g_cell_first[kRegion * kRegion + 1];  // tasks of region cel
 __device__ uint16_t g_cell_tasks[kTasks];                 // b << 7 | k
 __device__ uint8_t g_cand_cell[kBlocks * kMaxCands];     // region cell of candidate k of block b
 
-// v + v of lane ^ 16 (m = 16) or lane ^ 32 (m = 32) by gfx950's row swaps (VALU, no LDS).
-__device__ __forceinline__ uint32_t xor_add(uint32_t v, int m) {
-  const auto r = m == 16 ? __builtin_amdgcn_permlane16_swap(v, v, false, false) : __builtin_amdgcn_permlane32_swap(v, v, false, false);
-  return r[0] + r[1];
+template <int kCtrl>
+__device__ __forceinline__ float fdpp(float v) { return as_f32(dpp<kCtrl>(as_u32f(v))); }
+// v + v of lane ^ 16 (m = 16) or lane ^ 32 (m = 32), for the integer-valued float scores.
+__device__ __forceinline__ float fswap_add(float v, int m) {
+  const uint32_t x = as_u32f(v);
+  const auto r = m == 16 ? __builtin_amdgcn_permlane16_swap(x, x, false, false) : __builtin_amdgcn_permlane32_swap(x, x, false, false);
+  return as_f32(r[0]) + as_f32(r[1]);
 }
-// Lane grid of the superblock's 8x8 squares: lane = 8 * sqy + sqx. Neighbour sums within a 16-lane
-// row by DPP: lane ^ 1 (next square across), lane ^ 8 (next square down), lane ^ 2, and lane ^ 4 for
-// values already equal over each quad.
 
 __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
   __shared__ uint32_t mv[kRegion * kRegion];
@@ -100,75 +105,92 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
   }
   __syncthreads();
 
-  // This lane's 8x8 square of the superblock: rows as two words.
-  const int sqx = lane & 7, sqy = lane >> 3;
-  uint32_t cw[8][2];
+  // MV-independent operands of the products. Lane l = j + 16 g (j = l & 15, g = l >> 4) feeds
+  // column j of products 2p and 2p + 1 (p = 0..7) with row g of a tile: the left (e = 0) and right
+  // (e = 1) tile of tile row j >> 3 of 8x8 square (j & 7, p), superblock row 8p + 4 (j >> 3) + g,
+  // columns 8 (j & 7) + 4e .. + 3. One 12-byte load per lane and p covers both products, and each
+  // load instruction spans 8 rows of 64 (+4) bytes.
+  const int j = lane & 15, g = lane >> 4;
+  const bool upper = j < 8;  // the square's upper tile row; lanes j and j + 8 share a square
+  const int prow = 4 * (j >> 3) + g, pcol = 8 * (j & 7);
+  const half4 hh = hadamard_operand(lane, false), nhh = hadamard_operand(lane, true);
+  half4 cb[16];  // the source tiles' rows as halves 1024 + v
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const uint32_t* q = reinterpret_cast<const uint32_t*>(a.src.p + static_cast<ptrdiff_t>(oy + 8 * sqy + i) * a.src.stride + ox + 8 * sqx);
-    cw[i][0] = q[0], cw[i][1] = q[1];
+  for (int p = 0; p < 8; p++) {
+    const uint32_t* q = reinterpret_cast<const uint32_t*>(a.src.p + static_cast<ptrdiff_t>(oy + 8 * p + prow) * a.src.stride + ox + pcol);
+    cb[2 * p] = row_halves(q[0]), cb[2 * p + 1] = row_halves(q[1]);
   }
-  TilePair cpair[2];  // the square's tile rows as packed pairs (tiles (ty, 0) low, (ty, 1) high)
-#pragma unroll
-  for (int ty = 0; ty < 2; ty++) {
-    const uint32_t ca[4] = {cw[4 * ty][0], cw[4 * ty + 1][0], cw[4 * ty + 2][0], cw[4 * ty + 3][0]};
-    const uint32_t cb[4] = {cw[4 * ty][1], cw[4 * ty + 1][1], cw[4 * ty + 2][1], cw[4 * ty + 3][1]};
-    cpair[ty] = tile_pair(ca, cb);
-  }
+  // After the reductions below, lane (j, g) of register set s holds square (j & 7, 4s + g); for the
+  // quadtree above 8x8 the two sets merge into one register, lane (j, g) = square (sx, sy):
+  const int sx = j & 7, sy = g + 4 * (j >> 3);
   const DPlane ref = a.ref[r];
   const BPlane rb = as_buffer(ref);
   const int U = nuniq;
   uint32_t* const mysc = sc[wave];
-  // The 8 rows of the lane's square under MV m: 12 aligned bytes each (row i at soffset i * stride,
-  // a multiple of 4), and the byte shift of the square's first sample.
-  auto load_rows = [&](uint32_t m, uint32_t (&d)[8][3], int& sh) {
-    const int mx = mv_x(m), my = mv_y(m);
-    const int X = clamp_origin(ox + (mx >> 2), ref.w, 64), Y = clamp_origin(oy + (my >> 2), ref.h, 64);
-    const int off = static_cast<int>(a.phase_bytes) * ((my & 3) * 4 + (mx & 3)) + corner_offset(rb, Y + 8 * sqy, X + 8 * sqx);
-    sh = off & 3;
-#pragma unroll
-    for (int i = 0; i < 8; i++) bload12(rb, off & ~3, i * rb.stride, d[i][0], d[i][1], d[i][2]);
-  };
   // Each wave takes the distinct MVs u = wave, wave + 4, ... on its own (its own LDS scores; the
   // block minima are order-free atomics).
   for (int u = wave; u < U; u += 4) {
-    uint32_t dn[8][3];
-    int shn = 0;
-    load_rows(mv[urep[u]], dn, shn);
-    uint32_t pw[8][2];
+    const uint32_t m = mv[urep[u]];
+    const int mx = mv_x(m), my = mv_y(m);
+    const int X = clamp_origin(ox + (mx >> 2), ref.w, 64), Y = clamp_origin(oy + (my >> 2), ref.h, 64);
+    const int off = static_cast<int>(a.phase_bytes) * ((my & 3) * 4 + (mx & 3)) + corner_offset(rb, Y + prow, X + pcol);
+    const int sh = off & 3;  // the same for every lane (X + pcol = X mod 4)
+    // Per square row p: the lane's quarter (4 coefficients) of the left tile's SATD, of the right
+    // tile's, and their sum; each tile's transform is HH (1024 + cur) - HH (1024 + pred), exact in
+    // f32. Then the sums over the row groups g, 4 square rows per register set.
+    float rl[2], rr[2], rh[2];
 #pragma unroll
-    for (int i = 0; i < 8; i++) pw[i][0] = __builtin_amdgcn_alignbyte(dn[i][1], dn[i][0], shn), pw[i][1] = __builtin_amdgcn_alignbyte(dn[i][2], dn[i][1], shn);
-    // Tiles (ty, 0) and (ty, 1) of the square in one packed pass per tile row.
-    uint32_t t[2][2];
+    for (int s = 0; s < 2; s++) {
+      uint32_t d[4][3];
 #pragma unroll
-    for (int ty = 0; ty < 2; ty++) {
-      const uint32_t pa[4] = {pw[4 * ty][0], pw[4 * ty + 1][0], pw[4 * ty + 2][0], pw[4 * ty + 3][0]};
-      const uint32_t pb[4] = {pw[4 * ty][1], pw[4 * ty + 1][1], pw[4 * ty + 2][1], pw[4 * ty + 3][1]};
-      const uint32_t two = satd4x4_pair(cpair[ty], pa, pb);
-      t[ty][0] = two & 0xffffu, t[ty][1] = two >> 16;
+      for (int i = 0; i < 4; i++) bload12(rb, off & ~3, 8 * (4 * s + i) * rb.stride, d[i][0], d[i][1], d[i][2]);
+      float ql[4], qr[4], qh[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int p = 4 * s + i;
+        const uint32_t w0 = __builtin_amdgcn_alignbyte(d[i][1], d[i][0], sh), w1 = __builtin_amdgcn_alignbyte(d[i][2], d[i][1], sh);
+        const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+        const f32x4 c0 = mfma16(nhh, row_halves(w0), mfma16(hh, cb[2 * p], z));
+        const f32x4 c1 = mfma16(nhh, row_halves(w1), mfma16(hh, cb[2 * p + 1], z));
+        ql[i] = abs_sum4(c0), qr[i] = abs_sum4(c1), qh[i] = ql[i] + qr[i];
+      }
+      rl[s] = sum_halves32(sum_rows16(ql[0], ql[1]), sum_rows16(ql[2], ql[3]));
+      rr[s] = sum_halves32(sum_rows16(qr[0], qr[1]), sum_rows16(qr[2], qr[3]));
+      rh[s] = sum_halves32(sum_rows16(qh[0], qh[1]), sum_rows16(qh[2], qh[3]));
     }
-    // Level 3 (8x8 and its halves): this lane's square.
-    const int b3 = (21 + 8 * sqy + sqx) * 5;
-    const uint32_t W = t[0][0] + t[0][1] + t[1][0] + t[1][1];
-    mysc[b3] = W, mysc[b3 + 1] = t[0][0] + t[0][1], mysc[b3 + 2] = t[1][0] + t[1][1];
-    mysc[b3 + 3] = t[0][0] + t[1][0], mysc[b3 + 4] = t[0][1] + t[1][1];
-    // Level 2 (16x16): lanes (2a..2a+1, 2b..2b+1).
-    const uint32_t hp = W + dpp<kDppXor1>(W), vp = W + dpp<kDppRor8>(W), quad = hp + dpp<kDppRor8>(hp);
-    const int b2 = (5 + 4 * (sqy >> 1) + (sqx >> 1)) * 5;
-    if (!(sqx & 1) && !(sqy & 1)) mysc[b2] = quad, mysc[b2 + 1] = hp, mysc[b2 + 3] = vp;
-    if (!(sqx & 1) && (sqy & 1)) mysc[b2 + 2] = hp;
-    if ((sqx & 1) && !(sqy & 1)) mysc[b2 + 4] = vp;
-    // Level 1 (32x32).
-    const uint32_t hp2 = quad + dpp<kDppXor2>(quad), vp2 = xor_add(quad, 16), quad2 = xor_add(hp2, 16);
-    const int b1 = (1 + 2 * (sqy >> 2) + (sqx >> 2)) * 5;
-    if (!(sqx & 3) && !(sqy & 3)) mysc[b1] = quad2, mysc[b1 + 1] = hp2, mysc[b1 + 3] = vp2;
-    if (!(sqx & 3) && (sqy & 3) == 2) mysc[b1 + 2] = hp2;
-    if ((sqx & 3) == 2 && !(sqy & 3)) mysc[b1 + 4] = vp2;
-    // Level 0 (64x64).
-    const uint32_t hp1 = quad2 + dpp<kDppHalfMirror>(quad2), vp1 = xor_add(quad2, 32), quad1 = xor_add(hp1, 32);
-    if (lane == 0) mysc[0] = quad1, mysc[1] = hp1, mysc[3] = vp1;
-    if (lane == 32) mysc[2] = hp1;
-    if (lane == 4) mysc[4] = vp1;
+    // Level 3 (8x8 and its halves). Lane j of a set holds the upper tile row's sums (rh: top half;
+    // rl, rr: the left and right tiles), lane j + 8 the lower row's.
+    float w[2];
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      w[s] = rh[s] + fdpp<kDppRor8>(rh[s]);
+      // Left half in the upper lane, right half in the lower: each lane sends its partner the tile
+      // the partner needs.
+      const float lr = (upper ? rl[s] : rr[s]) + fdpp<kDppRor8>(upper ? rr[s] : rl[s]);
+      const int b3 = (21 + 8 * (4 * s + g) + sx) * 5;
+      if (upper) mysc[b3] = as_u32f(w[s]);
+      mysc[b3 + (upper ? 1 : 2)] = as_u32f(rh[s]);
+      mysc[b3 + (upper ? 3 : 4)] = as_u32f(lr);
+    }
+    const float W = upper ? w[0] : w[1];
+    // Level 2 (16x16): squares sx ^ 1 at lane ^ 1, sy ^ 1 at lane ^ 16.
+    const float hp = W + fdpp<kDppXor1>(W), vp = fswap_add(W, 16), quad = fswap_add(hp, 16);
+    const int b2 = (5 + 4 * (sy >> 1) + (sx >> 1)) * 5;
+    if (!(sx & 1) && !(sy & 1)) mysc[b2] = as_u32f(quad), mysc[b2 + 1] = as_u32f(hp), mysc[b2 + 3] = as_u32f(vp);
+    if (!(sx & 1) && (sy & 1)) mysc[b2 + 2] = as_u32f(hp);
+    if ((sx & 1) && !(sy & 1)) mysc[b2 + 4] = as_u32f(vp);
+    // Level 1 (32x32): sx ^ 2 at lane ^ 2, sy ^ 2 at lane ^ 32.
+    const float hp2 = quad + fdpp<kDppXor2>(quad), vp2 = fswap_add(quad, 32), quad2 = fswap_add(hp2, 32);
+    const int b1 = (1 + 2 * (sy >> 2) + (sx >> 2)) * 5;
+    if (!(sx & 3) && !(sy & 3)) mysc[b1] = as_u32f(quad2), mysc[b1 + 1] = as_u32f(hp2), mysc[b1 + 3] = as_u32f(vp2);
+    if (!(sx & 3) && (sy & 3) == 2) mysc[b1 + 2] = as_u32f(hp2);
+    if ((sx & 3) == 2 && !(sy & 3)) mysc[b1 + 4] = as_u32f(vp2);
+    // Level 0 (64x64): the other 4 squares across within the lane's 8 (values are uniform within
+    // quads), the other 4 square rows at lane ^ 8.
+    const float hp1 = quad2 + fdpp<kDppHalfMirror>(quad2), vp1 = quad2 + fdpp<kDppRor8>(quad2), quad1 = hp1 + fdpp<kDppRor8>(hp1);
+    if (lane == 0) mysc[0] = as_u32f(quad1), mysc[1] = as_u32f(hp1), mysc[3] = as_u32f(vp1);
+    if (lane == 8) mysc[2] = as_u32f(hp1);
+    if (lane == 4) mysc[4] = as_u32f(vp1);
     // The wave's scores are in LDS before any lane reads another lane's.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -184,7 +206,7 @@ __global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
         for (int i = f0 + lane; i < f1; i += 64) {
           const uint32_t e = cell_tasks[i];
           const int b = static_cast<int>(e >> 7);
-          atomicMin(&best[b], (mysc[b] << 7) | (e & 127u));
+          atomicMin(&best[b], (static_cast<uint32_t>(as_f32(mysc[b])) << 7) | (e & 127u));
           if (a.candidates) atomicOr(&seen[b][u >> 6], 1ull << (u & 63));
         }
       }
