@@ -84,7 +84,7 @@ void layout(Pyramid& p, int pw, int ph, bool phases) {
     int stride;
     const size_t b = plane_bytes(w, h, &stride);
     p.lv[l] = DPlane{p.mem + off + kSlack + static_cast<size_t>(kBorder) * stride + kBorder, stride, w, h};
-    if (l == 0 && phases) p.phase_bytes = static_cast<long long>(b);
+    if (l == 0 && phases) p.phase_bytes = static_cast<long long>(b);  // a multiple of 64: the kernels' phase_offset
     off += b * ((l == 0 && phases) ? kPhases : 1);
     w /= 2, h /= 2;
   }

@@ -36,7 +36,9 @@ struct DPlane {
   int stride, w, h;
 };
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// v clamped into [lo, hi] (lo <= hi at every use), written as the median of (v, lo, hi) so that a
+// per-lane clamp is one v_med3_i32.
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(min(v, lo), min(max(v, lo), hi)); }
 
 // Origin of a size-wide window moved into the bordered range; reads through it equal clamped reads
 // (every sample beyond the border repeats the edge it would clamp to).
@@ -74,6 +76,17 @@ __device__ __forceinline__ BPlane as_buffer(const DPlane& p) {
 }
 // Offset of sample (y, x) of a bordered plane from its corner.
 __device__ __forceinline__ int corner_offset(const BPlane& b, int y, int x) { return (y + kBorder) * b.stride + x + kBorder; }
+// The same for per-lane coordinates inside the bordered range (0 <= y + kBorder < 2^24): a full-rate
+// 24-bit multiply instead of the quarter-rate 32-bit one.
+__device__ __forceinline__ int corner_offset_lane(const BPlane& b, int y, int x) {
+  return static_cast<int>(__umul24(static_cast<unsigned>(y + kBorder), static_cast<unsigned>(b.stride))) + x + kBorder;
+}
+// Byte offset of quarter-pel phase `phase` (0..15) from phase 0, for planes phase_bytes apart;
+// phase_bytes is a multiple of 64 (a plane's rows are a multiple of 64 bytes, its slack 512), and
+// phase_bytes / 64 < 2^24.
+__device__ __forceinline__ int phase_offset(int phase, long long phase_bytes) {
+  return static_cast<int>(__umul24(static_cast<unsigned>(phase), static_cast<unsigned>(phase_bytes >> 6)) << 6);
+}
 // 12 aligned bytes at voffset + soffset (both multiples of 4).
 __device__ __forceinline__ void bload12(const BPlane& b, int voff, int soff, uint32_t& d0, uint32_t& d1, uint32_t& d2) {
   const auto v = __builtin_amdgcn_raw_buffer_load_b96(b.r, voff, soff, 0);
@@ -123,9 +136,12 @@ __device__ __forceinline__ int mode_bits(int m) { return m == 0 ? 1 : (m < 4 ? 3
 // mod 16, row_mirror maps lane i of each 16 to 15 - i (the other 8 of the row).
 constexpr int kDppXor1 = 0xB1, kDppXor2 = 0x4E, kDppRor4 = 0x124, kDppRor8 = 0x128, kDppMirror = 0x140,
               kDppHalfMirror = 0x141;
+// Every control here reads a lane of the same row, so no lane is out of range and the zero fill
+// (bound_ctrl) never applies; with it, the compiler folds the move into the consuming v_add / v_min
+// as a DPP operand instead of issuing a separate v_mov_b32_dpp.
 template <int kCtrl>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
-  return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), kCtrl, 0xf, 0xf, false));
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), kCtrl, 0xf, 0xf, true));
 }
 
 // Wave-wide minimum of a 32-bit key (all 64 lanes take part; the result is wave-uniform).

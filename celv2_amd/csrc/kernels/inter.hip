@@ -40,7 +40,7 @@ __device__ __forceinline__ float fswap_add(float v, int m) {
   return as_f32(r[0]) + as_f32(r[1]);
 }
 
-__global__ void __launch_bounds__(256) k_inter(Batch<InterArgs> batch) {
+__global__ void __launch_bounds__(256, 6) k_inter(Batch<InterArgs> batch) {
   __shared__ uint32_t mv[kRegion * kRegion];
   __shared__ int cuid[kRegion * kRegion];
   __shared__ int urep[kRegion * kRegion];

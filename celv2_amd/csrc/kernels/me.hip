@@ -239,7 +239,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
         const int qx = qcx + odx[cc] * step, qy = qcy + ody[cc] * step;
         const int X = clamp_origin(8 * bx + (qx >> 2), ref.w, 8) + 4 * tx;
         const int Y = clamp_origin(8 * by + 4 * ty + (qy >> 2), ref.h, 4) + g;
-        const int off = static_cast<int>(a.phase_bytes) * ((qy & 3) * 4 + (qx & 3)) + corner_offset(rb, Y, X);
+        const int off = phase_offset((qy & 3) * 4 + (qx & 3), a.phase_bytes) + corner_offset_lane(rb, Y, X);
         uint32_t d0, d1;
         bload8(rb, off & ~3, 0, d0, d1);
         prow[cc] = __builtin_amdgcn_alignbyte(d1, d0, off & 3);
