@@ -39,6 +39,8 @@ struct DPlane {
 // v clamped into [lo, hi] (lo <= hi at every use), written as the median of (v, lo, hi) so that a
 // per-lane clamp is one v_med3_i32.
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return max(min(v, lo), min(max(v, lo), hi)); }
+// The same for wave-uniform values, in the form the compiler keeps in scalar instructions.
+__device__ __forceinline__ int clamp_uniform(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // Origin of a size-wide window moved into the bordered range; reads through it equal clamped reads
 // (every sample beyond the border repeats the edge it would clamp to).

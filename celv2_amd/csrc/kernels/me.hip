@@ -72,9 +72,9 @@ constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u 
 // superblocks around it (0 outside the frame or without limits), lanes 0..15 the block's 16 words
 // (row i's two words in lanes 2i, 2i + 1; every later use of the block's samples takes them from
 // here by lane shuffles, not from memory again: the search is bound by the vector-memory data path),
-// lanes 0..4 the coarser MVs under and beside it.
+// the coarser MVs under and beside it are read by scalar loads where they are used.
 struct Opening {
-  uint32_t km = 0, mine = 0, cm = 0;
+  uint32_t km = 0, mine = 0;
 };
 
 struct BlockPos {
@@ -106,11 +106,6 @@ __device__ __forceinline__ Opening open_block(const MeLevelArgs& a, const BlockP
   }
   const BPlane cb = as_buffer(a.cur);
   if (lane < 16) o.mine = bload4(cb, corner_offset(cb, 8 * q.by + (lane >> 1), 8 * q.bx + 4 * (lane & 1)), 0);
-  if (a.coarser[q.r] && lane < 5) {
-    const int ox = lane == 1 ? -1 : lane == 2 ? 1 : 0, oy = lane == 3 ? -1 : lane == 4 ? 1 : 0;
-    const int qx = clampi(q.bx / 2 + ox, 0, a.pcols - 1), qy = clampi(q.by / 2 + oy, 0, a.prows - 1);
-    o.cm = a.coarser[q.r][qy * a.pcols + qx];
-  }
   return o;
 }
 
@@ -124,21 +119,27 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   const DPlane ref = a.ref[r];
   const BPlane rb = as_buffer(ref);
 
-  // The block, every lane holding every word (scalar registers after readlane).
+  // The block, every lane holding every word: scalar loads of its 8 rows (its address is uniform and
+  // the kernel writes no source plane), no VALU.
   uint32_t c[16];
 #pragma unroll
-  for (int i = 0; i < 16; i++) c[i] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(o.mine), i));
+  for (int i = 0; i < 8; i++) {
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(a.cur.p + static_cast<ptrdiff_t>(8 * by + i) * a.cur.stride + 8 * bx);
+    c[2 * i] = row[0], c[2 * i + 1] = row[1];
+  }
 
   int p1x = 0, p1y = 0, p2x = 0, p2y = 0;
   bool two = false;
   if (a.coarser[r]) {
     // Centres k = 0..4: the coarser MV under the block and its left, right, above and below
-    // neighbours (clamped into the coarser field), doubled; k = 5: zero.
-    const uint32_t cm = o.cm;
+    // neighbours (clamped into the coarser field), doubled; k = 5: zero. Uniform addresses, and the
+    // coarser field is the previous launch's output: scalar loads.
     int cx[6], cy[6];
 #pragma unroll
     for (int k = 0; k < 5; k++) {
-      const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(cm), k));
+      const int ox = k == 1 ? -1 : k == 2 ? 1 : 0, oy = k == 3 ? -1 : k == 4 ? 1 : 0;
+      const int qx = clamp_uniform(bx / 2 + ox, 0, a.pcols - 1), qy = clamp_uniform(by / 2 + oy, 0, a.prows - 1);
+      const uint32_t m = a.coarser[r][qy * a.pcols + qx];
       cx[k] = 2 * mv_x(m), cy[k] = 2 * mv_y(m);
     }
     cx[5] = 0, cy[5] = 0;
