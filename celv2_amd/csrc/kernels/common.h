@@ -47,6 +47,9 @@ __device__ __forceinline__ int clamp_uniform(int v, int lo, int hi) { return v <
 __device__ __forceinline__ int clamp_origin(int o, int extent, int size) {
   return clampi(o, -kBorder, extent + kBorder - size);
 }
+__device__ __forceinline__ int clamp_origin_uniform(int o, int extent, int size) {
+  return clamp_uniform(o, -kBorder, extent + kBorder - size);
+}
 
 // 4 bytes starting at an arbitrary address, from two aligned dword loads (never past the bytes the
 // two dwords cover, which the plane's slack keeps inside the allocation).
@@ -78,16 +81,23 @@ __device__ __forceinline__ BPlane as_buffer(const DPlane& p) {
 }
 // Offset of sample (y, x) of a bordered plane from its corner.
 __device__ __forceinline__ int corner_offset(const BPlane& b, int y, int x) { return (y + kBorder) * b.stride + x + kBorder; }
-// The same for per-lane coordinates inside the bordered range (0 <= y + kBorder < 2^24): a full-rate
-// 24-bit multiply instead of the quarter-rate 32-bit one.
+// s * v for a wave-uniform s and a per-lane v, both below 2^24, as the full-rate v_mul_u32_u24.
+// (Written as the instruction itself: from __umul24 the compiler emits the quarter-rate 32-bit
+// multiply whenever it can bound one operand.)
+__device__ __forceinline__ uint32_t umul24_sv(uint32_t s, uint32_t v) {
+  uint32_t r;
+  asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "s"(s), "v"(v));
+  return r;
+}
+// The same for per-lane coordinates inside the bordered range (0 <= y + kBorder < 2^24).
 __device__ __forceinline__ int corner_offset_lane(const BPlane& b, int y, int x) {
-  return static_cast<int>(__umul24(static_cast<unsigned>(y + kBorder), static_cast<unsigned>(b.stride))) + x + kBorder;
+  return static_cast<int>(umul24_sv(static_cast<uint32_t>(b.stride), static_cast<uint32_t>(y + kBorder))) + x + kBorder;
 }
 // Byte offset of quarter-pel phase `phase` (0..15) from phase 0, for planes phase_bytes apart;
 // phase_bytes is a multiple of 64 (a plane's rows are a multiple of 64 bytes, its slack 512), and
 // phase_bytes / 64 < 2^24.
 __device__ __forceinline__ int phase_offset(int phase, long long phase_bytes) {
-  return static_cast<int>(__umul24(static_cast<unsigned>(phase), static_cast<unsigned>(phase_bytes >> 6)) << 6);
+  return static_cast<int>(umul24_sv(static_cast<uint32_t>(phase_bytes >> 6), static_cast<uint32_t>(phase)) << 6);
 }
 // 12 aligned bytes at voffset + soffset (both multiples of 4).
 __device__ __forceinline__ void bload12(const BPlane& b, int voff, int soff, uint32_t& d0, uint32_t& d1, uint32_t& d2) {

@@ -57,6 +57,30 @@ __device__ __forceinline__ half4 hadamard_operand(int lane, bool negate) {
   return as_half4(w[0], w[1]);
 }
 
+// The A operands of all 64 lanes as a constant table: one 8-byte load per lane instead of the
+// parity arithmetic, for kernels that build them once per small unit of work. -HH is HH with every
+// half's sign bit flipped.
+struct HadamardLanes {
+  uint32_t w[64][2];
+  constexpr HadamardLanes() : w{} {
+    for (int lane = 0; lane < 64; lane++)
+      for (int p = 0; p < 2; p++) {
+        uint32_t v = 0;
+        for (int h = 0; h < 2; h++) {
+          const int i = lane & 15, g = lane >> 4, k = 2 * p + h;
+          const int par = (__builtin_popcount((i >> 2) & g) + __builtin_popcount((i & 3) & k)) & 1;
+          v |= (par ? 0xBC00u : 0x3C00u) << (16 * h);
+        }
+        w[lane][p] = v;
+      }
+  }
+};
+__device__ constexpr HadamardLanes kHadamardLanes{};
+__device__ __forceinline__ void hadamard_operands(int lane, half4& hh, half4& nhh) {
+  const uint32_t w0 = kHadamardLanes.w[lane][0], w1 = kHadamardLanes.w[lane][1];
+  hh = as_half4(w0, w1), nhh = as_half4(w0 ^ 0x80008000u, w1 ^ 0x80008000u);
+}
+
 // Sum of |coefficients| this lane holds (a quarter of one tile's SATD).
 __device__ __forceinline__ float abs_sum4(f32x4 d) {
   return __builtin_fabsf(d[0]) + __builtin_fabsf(d[1]) + __builtin_fabsf(d[2]) + __builtin_fabsf(d[3]);

@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(256, 6) k_inter(Batch<InterArgs> batch) {
     const uint32_t m = mv[urep[u]];
     const int mx = mv_x(m), my = mv_y(m);
     const int X = clamp_origin(ox + (mx >> 2), ref.w, 64), Y = clamp_origin(oy + (my >> 2), ref.h, 64);
-    const int off = static_cast<int>(a.phase_bytes) * ((my & 3) * 4 + (mx & 3)) + corner_offset(rb, Y + prow, X + pcol);
+    const int off = phase_offset((my & 3) * 4 + (mx & 3), a.phase_bytes) + corner_offset_lane(rb, Y + prow, X + pcol);
     const int sh = off & 3;  // the same for every lane (X + pcol = X mod 4)
     // Per square row p: the lane's quarter (4 coefficients) of the left tile's SATD, of the right
     // tile's, and their sum; each tile's transform is HH (1024 + cur) - HH (1024 + pred), exact in

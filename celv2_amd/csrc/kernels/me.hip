@@ -149,7 +149,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
 #pragma unroll
     for (int k = 1; k < 6; k++)
       if (kk == k) mx = cx[k], my = cy[k];
-    const int off = corner_offset(rb, clamp_origin(8 * by + my, ref.h, 8) + row, clamp_origin(8 * bx + mx, ref.w, 8));
+    const int off = corner_offset_lane(rb, clamp_origin(8 * by + my, ref.h, 8) + row, clamp_origin(8 * bx + mx, ref.w, 8));
     uint32_t d0, d1, d2;
     bload12(rb, off & ~3, 0, d0, d1, d2);
     const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, off & 3), w1 = __builtin_amdgcn_alignbyte(d2, d1, off & 3);
@@ -177,11 +177,11 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
 #pragma unroll
   for (int i = 0; i < 3; i++) {
     const int k = lane + 64 * i, row = k / kWinDwords, d = k - row * kWinDwords;
-    koff[i] = row * rb.stride + 4 * d;
+    koff[i] = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(row))) + 4 * d;
     kdst[i] = k < kWin * kWinDwords ? row * kWinStride + d : -1;
   }
-  const int X1 = clamp_origin(8 * bx + p1x - kRange, ref.w, kWin), Y1 = clamp_origin(8 * by + p1y - kRange, ref.h, kWin);
-  const int X2 = clamp_origin(8 * bx + p2x - kRange, ref.w, kWin), Y2 = clamp_origin(8 * by + p2y - kRange, ref.h, kWin);
+  const int X1 = clamp_origin_uniform(8 * bx + p1x - kRange, ref.w, kWin), Y1 = clamp_origin_uniform(8 * by + p1y - kRange, ref.h, kWin);
+  const int X2 = clamp_origin_uniform(8 * bx + p2x - kRange, ref.w, kWin), Y2 = clamp_origin_uniform(8 * by + p2y - kRange, ref.h, kWin);
   stage_window(rb, X1, Y1, koff, kdst, win1);
   if (two) stage_window(rb, X2, Y2, koff, kdst, win2);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -199,12 +199,14 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   }
   const uint32_t kill5 = g3 == 2 ? 0xffffffffu : 0u;
   uint32_t best_cost;
-  const uint32_t k1 = wave_min_u32(act ? score_window(win1, corner_offset(rb, Y1, X1) & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
+  // A window's first sample sits at byte X & 3 of its aligned row (rows are a multiple of 64 bytes
+  // apart, the border 64 samples wide).
+  const uint32_t k1 = wave_min_u32(act ? score_window(win1, X1 & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
   const int i1 = static_cast<int>(k1 & 511);
   int mvx = p1x + i1 % 17 - kRange, mvy = p1y + i1 / 17 - kRange;
   best_cost = k1 >> 9;
   if (two) {
-    const uint32_t k2 = wave_min_u32(act ? score_window(win2, corner_offset(rb, Y2, X2) & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
+    const uint32_t k2 = wave_min_u32(act ? score_window(win2, X2 & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
     if ((k2 >> 9) < (k1 >> 9)) {
       best_cost = k2 >> 9;
       const int i2 = static_cast<int>(k2 & 511);
@@ -217,7 +219,8 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     // j' = 16 c + j of product c is position o = j' >> 2, tile t = j' & 3 (t = 2 ty + tx); lane
     // (j, g) supplies row g of that tile. Positions 9..11 are padding (never chosen).
     const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;
-    const half4 hh = hadamard_operand(lane, false), nhh = hadamard_operand(lane, true);
+    half4 hh, nhh;
+    hadamard_operands(lane, hh, nhh);
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     // Row g of the block's tile (tx, ty): word 2 (4 ty + g) + tx of the block.
     const uint32_t crow = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * (4 * ty + g) + tx, 64));
@@ -229,21 +232,41 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
       const int o = min(4 * cc + (j >> 2), 8);
       odx[cc] = static_cast<int>((kRefDx >> (2 * o)) & 3u) - 1, ody[cc] = static_cast<int>((kRefDy >> (2 * o)) & 3u) - 1;
     }
+    // The lane's row and tile within the block, as an offset (rows within the bordered range).
+    const int lane_off = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(4 * ty + g))) + 4 * tx;
     int qcx = 4 * mvx, qcy = 4 * mvy;
 #pragma unroll
     for (int step = 2; step >= 1; step--) {
       // The three products' rows first, then the three products back to back, then their sums
       // (every partial sum an integer below 2^24: any order is exact).
+      // Row offsets. When no position of the step moves a window origin into its clamp (checked
+      // once for the wave), the offset is the centre's, plus the lane's row and tile, plus the
+      // position's phase and whole-sample carry; otherwise each origin is clamped as at_q() reads.
+      int off[3];
+      const bool inner = 8 * bx + ((qcx - step) >> 2) >= -kBorder && 8 * bx + ((qcx + step) >> 2) <= ref.w + kBorder - 8 &&
+                         8 * by + ((qcy - step) >> 2) >= -kBorder && 8 * by + 4 + ((qcy + step) >> 2) <= ref.h + kBorder - 4;
+      if (inner) {
+        const int centre = corner_offset(rb, 8 * by + (qcy >> 2), 8 * bx + (qcx >> 2)) + lane_off;
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+          const int ex = (qcx & 3) + odx[cc] * step, ey = (qcy & 3) + ody[cc] * step;  // -2 .. 5
+          off[cc] = centre + phase_offset((ey & 3) * 4 + (ex & 3), a.phase_bytes) + __mul24(ey >> 2, rb.stride) + (ex >> 2);
+        }
+      } else {
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {
+          const int qx = qcx + odx[cc] * step, qy = qcy + ody[cc] * step;
+          const int X = clamp_origin(8 * bx + (qx >> 2), ref.w, 8) + 4 * tx;
+          const int Y = clamp_origin(8 * by + 4 * ty + (qy >> 2), ref.h, 4) + g;
+          off[cc] = phase_offset((qy & 3) * 4 + (qx & 3), a.phase_bytes) + corner_offset_lane(rb, Y, X);
+        }
+      }
       uint32_t prow[3];
 #pragma unroll
       for (int cc = 0; cc < 3; cc++) {
-        const int qx = qcx + odx[cc] * step, qy = qcy + ody[cc] * step;
-        const int X = clamp_origin(8 * bx + (qx >> 2), ref.w, 8) + 4 * tx;
-        const int Y = clamp_origin(8 * by + 4 * ty + (qy >> 2), ref.h, 4) + g;
-        const int off = phase_offset((qy & 3) * 4 + (qx & 3), a.phase_bytes) + corner_offset_lane(rb, Y, X);
         uint32_t d0, d1;
-        bload8(rb, off & ~3, 0, d0, d1);
-        prow[cc] = __builtin_amdgcn_alignbyte(d1, d0, off & 3);
+        bload8(rb, off[cc] & ~3, 0, d0, d1);
+        prow[cc] = __builtin_amdgcn_alignbyte(d1, d0, off[cc] & 3);
       }
       f32x4 m[3];
 #pragma unroll
