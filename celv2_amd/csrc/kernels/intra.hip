@@ -208,9 +208,13 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
         s6 = sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
       }
       // 7..12 directional (intra_common.h:61-85): two samples and a 5-bit weight per predicted sample.
+      // The table through a buffer resource: the lane's entry offset once, each mode's part in the
+      // scalar offset (no per-load address arithmetic).
+      const __amdgpu_buffer_rsrc_t dtab = __builtin_amdgcn_make_buffer_rsrc(g_dir, static_cast<short>(0), static_cast<int>(sizeof(g_dir)), 0x00020000);
+      const int dlane = 4 * (static_cast<int>(umul24_sv(static_cast<uint32_t>(w), static_cast<uint32_t>(pi))) + pj0);
       auto dir = [&](int mm) {
-        const uint4 t = *reinterpret_cast<const uint4*>(g_dir + sh.dir + (mm * h + pi) * w + pj0);
-        const uint32_t te[4] = {t.x, t.y, t.z, t.w};
+        const auto t = __builtin_amdgcn_raw_buffer_load_b128(dtab, dlane, 4 * (sh.dir + mm * h * w), 0);
+        const uint32_t te[4] = {t[0], t[1], t[2], t[3]};
         uint32_t v[4];
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) {
