@@ -22,34 +22,42 @@ namespace celv2hip {
 namespace {
 
 constexpr int kWin = 2 * kRange + 8;  // 24: a search window's side
-constexpr int kWinDwords = 7;         // an aligned row of a window: 28 bytes cover 24 from any offset
-constexpr int kWinStride = 8;         // dwords per window row in LDS
+// A window is staged as 4 copies of its rows, copy s holding the aligned row's bytes from s on, so a
+// scoring lane reads its 8 samples at any byte offset o (0..19) as 2 aligned words of copy o & 3.
+constexpr int kCopyWords = 6;  // words per copied row: words 0..5 cover offsets up to 19 + 8
 // 25 rows: the last dy group's sixth position (past the range, its key forced to the maximum) reads
 // row 24, which stays inside the array.
-constexpr int kWinLds = (kWin + 1) * kWinStride;
+constexpr int kCopyLds = (kWin + 1) * kCopyWords;  // one copy
+constexpr int kWinLds = 4 * kCopyLds;
 
-// Reads a window's rows (aligned dwords from the row's first byte rounded down) into LDS; the
-// lane's share is dword k = lane + 64 i of the 24 x 7.
+// Stages a window: item k = lane + 64 i (k < 24 x 6) is word d of row r of every copy, from the
+// aligned row's words d and d + 1 (rows from the window's first byte rounded down).
 __device__ __forceinline__ void stage_window(const BPlane& rb, int X, int Y, const int (&koff)[3], const int (&kdst)[3],
                                              uint32_t* win) {
   const int base = corner_offset(rb, Y, X) & ~3;
 #pragma unroll
   for (int i = 0; i < 3; i++)
-    if (kdst[i] >= 0) win[kdst[i]] = bload4(rb, base + koff[i], 0);
+    if (kdst[i] >= 0) {
+      uint32_t lo, hi;
+      bload8(rb, base + koff[i], 0, lo, hi);
+      win[kdst[i]] = lo;
+      win[kCopyLds + kdst[i]] = __builtin_amdgcn_alignbyte(hi, lo, 1);
+      win[2 * kCopyLds + kdst[i]] = __builtin_amdgcn_alignbyte(hi, lo, 2);
+      win[3 * kCopyLds + kdst[i]] = __builtin_amdgcn_alignbyte(hi, lo, 3);
+    }
 }
 
 // The lane's best key over its positions from a staged window (byte shift sh of the window's first
 // sample): pen[k] / idx[k] hold each position's penalty and index (kill5: the lane has 5 positions).
 __device__ __forceinline__ uint32_t score_window(const uint32_t* win, int sh, int dxi, int dy0, const uint32_t c[16],
                                                  const uint32_t pen[6], const uint32_t idx[6], uint32_t kill5) {
-  const int o = sh + dxi, q = o >> 2, s = o & 3;
+  const int o = sh + dxi;
+  const uint32_t* copy = win + (o & 3) * kCopyLds + (o >> 2);
   uint32_t w[13][2];
 #pragma unroll
   for (int rr = 0; rr < 13; rr++) {
-    const uint32_t* row = win + (dy0 + rr) * kWinStride + q;
-    const uint32_t d0 = row[0], d1 = row[1], d2 = row[2];
-    w[rr][0] = __builtin_amdgcn_alignbyte(d1, d0, s);
-    w[rr][1] = __builtin_amdgcn_alignbyte(d2, d1, s);
+    const uint32_t* row = copy + (dy0 + rr) * kCopyWords;
+    w[rr][0] = row[0], w[rr][1] = row[1];
   }
   uint32_t best = 0xffffffffu;
 #pragma unroll
@@ -176,9 +184,9 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   int koff[3], kdst[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    const int k = lane + 64 * i, row = k / kWinDwords, d = k - row * kWinDwords;
+    const int k = lane + 64 * i, row = k / kCopyWords, d = k - row * kCopyWords;
     koff[i] = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(row))) + 4 * d;
-    kdst[i] = k < kWin * kWinDwords ? row * kWinStride + d : -1;
+    kdst[i] = k < kWin * kCopyWords ? row * kCopyWords + d : -1;
   }
   const int X1 = clamp_origin_uniform(8 * bx + p1x - kRange, ref.w, kWin), Y1 = clamp_origin_uniform(8 * by + p1y - kRange, ref.h, kWin);
   const int X2 = clamp_origin_uniform(8 * bx + p2x - kRange, ref.w, kWin), Y2 = clamp_origin_uniform(8 * by + p2y - kRange, ref.h, kWin);
