@@ -4,9 +4,10 @@
 // src/gpu/inter_common.h:306-358):
 //   1. the six candidate centres' SADs, 8 lanes per centre (one block row each), summed by DPP;
 //      the centre ranking is then the reference's sequential rule, wave-uniform (scalar code);
-//   2. each searched centre's 24 x 24 window is read from the plane once (168 coalesced dwords) into
-//      the wave's LDS area; lane = (dx, third of the dy range), 51 lanes, takes its 13 rows from LDS
-//      and scores its 5-6 positions with v_sad_u8 against the block held in SGPRs, the distance
+//   2. each searched centre's 24 x 24 window is read from the plane once (144 coalesced word pairs)
+//      into the wave's LDS area as 4 byte-shifted copies; lane = (dx, third of the dy range), 51
+//      lanes, takes its 13 rows as aligned words of the copy its offset selects and scores its 5-6
+//      positions with v_sad_u8 against the block held in SGPRs (scalar loads), the distance
 //      penalty folded into the first v_sad_u8's accumulator; the first least cost in dy-major order
 //      is the minimum of the key cost << 9 | position index (DPP + 4 readlanes);
 //   3. level 0: the half-pel then quarter-pel refinement: per step the 9 positions x 4 tiles are 36
