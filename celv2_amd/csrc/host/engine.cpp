@@ -84,7 +84,11 @@ void layout(Pyramid& p, int pw, int ph, bool phases) {
     int stride;
     const size_t b = plane_bytes(w, h, &stride);
     p.lv[l] = DPlane{p.mem + off + kSlack + static_cast<size_t>(kBorder) * stride + kBorder, stride, w, h};
-    if (l == 0 && phases) p.phase_bytes = static_cast<long long>(b);  // a multiple of 64: the kernels' phase_offset
+    if (l == 0 && phases) {
+      // The kernels address phase planes as (phase x phase_bytes / 64) << 6 with a 24-bit multiply.
+      if (b % 64 != 0 || b / 64 >= (size_t{1} << 24)) throw Failure(CELV2_ERR_ARG, "frame too large for the phase-plane layout");
+      p.phase_bytes = static_cast<long long>(b);
+    }
     off += b * ((l == 0 && phases) ? kPhases : 1);
     w /= 2, h /= 2;
   }
