@@ -169,6 +169,18 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   return min(min(r0, r1), min(r2, r3));
 }
 
+// The same when v is already equal over each aligned quad of lanes (the xor-1 and xor-2 steps would
+// change nothing).
+__device__ __forceinline__ uint32_t wave_min_quads_u32(uint32_t v) {
+  v = min(v, dpp<kDppRor4>(v));
+  v = min(v, dpp<kDppRor8>(v));  // every lane: its row's minimum
+  const uint32_t r0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
+  const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
+  const uint32_t r2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
+  const uint32_t r3 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
+  return min(min(r0, r1), min(r2, r3));
+}
+
 // Sums over aligned lane groups of 4 and of 8 (every lane of the group gets the sum).
 __device__ __forceinline__ uint32_t quad_sum(uint32_t v) {
   v += dpp<kDppXor1>(v);

@@ -296,7 +296,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
       const uint32_t c2 = static_cast<uint32_t>(s2) + kQpelPenalty;
       uint32_t best = (c01 << 4) | static_cast<uint32_t>(o01);
       if (o2 < 9) best = min(best, (c2 << 4) | static_cast<uint32_t>(o2));
-      const uint32_t key = wave_min_u32(best);
+      const uint32_t key = wave_min_quads_u32(best);  // best is equal over each quad (lanes 4o' .. 4o' + 3)
       const int ob = static_cast<int>(key & 15);
       qcx += (static_cast<int>((kRefDx >> (2 * ob)) & 3u) - 1) * step;
       qcy += (static_cast<int>((kRefDy >> (2 * ob)) & 3u) - 1) * step;
