@@ -6,6 +6,7 @@
 
 `python -m celv2_amd.build` or `build()` compiles every time it is called.
 """
+import hashlib
 import os
 import shutil
 import subprocess
@@ -36,6 +37,7 @@ HIP_FLAGS = [
     "-amdgpu-mfma-vgpr-form",
     "-Wall",
     "-Wno-unused-function",
+    f"-I{os.path.join(ROOT, 'include')}",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}"]
 
@@ -51,14 +53,32 @@ def _run(cmd):
     return out
 
 
+def source_hash():
+    """A digest of every native source of libcelv2hip.so (csrc/ and include/), embedded in the library
+    at build time; engine.lib() refuses a library whose digest differs from the sources beside it."""
+    paths = []
+    for base in (CSRC, os.path.join(ROOT, "include")):
+        for d, _, files in os.walk(base):
+            if os.path.basename(d) == "_build":
+                continue
+            paths += [os.path.join(d, f) for f in files if f.endswith((".hip", ".cpp", ".h"))]
+    h = hashlib.sha256()
+    for p in sorted(paths, key=lambda p: os.path.relpath(p, ROOT)):
+        h.update(os.path.relpath(p, ROOT).encode())
+        with open(p, "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def build_hip():
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
     jobs = []
     for k in KERNELS:
         jobs.append([HIPCC, *HIP_FLAGS, "-c", "-x", "hip", os.path.join(CSRC, "kernels", k + ".hip"), "-o", os.path.join(OBJ, k + ".o")])
+    digest = f'-DCELV2_SOURCE_HASH="{source_hash()}"'
     for h in HOST:
-        jobs.append([HIPCC, *HOST_FLAGS, "-c", os.path.join(CSRC, "host", h + ".cpp"), "-o", os.path.join(OBJ, h + ".o")])
+        jobs.append([HIPCC, *HOST_FLAGS, digest, "-c", os.path.join(CSRC, "host", h + ".cpp"), "-o", os.path.join(OBJ, h + ".o")])
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         for out in ex.map(_run, jobs):
             if out.strip():
@@ -72,7 +92,8 @@ def build_hip():
 def build_synth():
     os.makedirs(LIB, exist_ok=True)
     tmp = os.path.join(LIB, "libcelv2synth.so.tmp")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-shared", "-o", tmp, os.path.join(CSRC, "synth", "synth.cpp"), "-lpthread"])
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}", "-shared", "-o", tmp,
+          os.path.join(CSRC, "synth", "synth.cpp"), "-lpthread"])
     os.replace(tmp, os.path.join(LIB, "libcelv2synth.so"))
 
 

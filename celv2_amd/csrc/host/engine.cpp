@@ -2,7 +2,7 @@
 // per-launch queue of kernels on the engine's main HIP stream, prepare() on a side stream, and row
 // completion events.
 //
-// Queue order of a launch (VulkanEngine::launch, src/vk/vk_engine.cpp:207-312, re-planned for HIP):
+// Queue order of a launch (VulkanEngine::launch, src/vk/vk_engine.cpp:173-278, re-planned for HIP):
 //   source: the prepare()d pyramid and intra scores under src_key (built on the side stream, which
 //   the main stream waits for), else import + pyramid + intra now;
 //   references: a slot holding the key, else a free slot: import + 15 phase planes + pyramid;
@@ -22,15 +22,9 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/celv2hip.h"
+#include "celv2hip.h"
 #include "../kernels/launch.h"
 #include "pruner.h"
-
-namespace celv2hip {
-void upload_intra_tiles(hipStream_t s);
-void upload_inter_tables(hipStream_t s);
-void upload_final_tiles(hipStream_t s);
-}  // namespace celv2hip
 
 using namespace celv2hip;
 
@@ -246,11 +240,19 @@ struct celv2hip_engine {
 
   size_t nsb() const { return static_cast<size_t>(sb_cols) * sb_rows; }
 
+  // A pyramid is set up whole or not at all: a layout that fails frees its memory and leaves p empty.
   void ensure_pyramid(Pyramid& p, bool phases) {
     if (p.mem) return;
-    p.bytes = pyramid_bytes(pw, ph, phases);
-    check(hipMalloc(&p.mem, p.bytes), "hipMalloc pyramid");
-    layout(p, pw, ph, phases);
+    Pyramid q;
+    q.bytes = pyramid_bytes(pw, ph, phases);
+    check(hipMalloc(&q.mem, q.bytes), "hipMalloc pyramid");
+    try {
+      layout(q, pw, ph, phases);
+    } catch (...) {
+      (void)hipFree(q.mem);
+      throw;
+    }
+    p = q;
   }
 
   // Puts a frame's luma where the import kernel reads it: in place if on the device, else through
@@ -310,6 +312,19 @@ struct celv2hip_engine {
 
 namespace {
 
+// The kernels' constant tables on `device` (the current device), uploaded by the first engine created
+// there; a failed upload leaves the device's flag unset, so the next creation tries again.
+constexpr int kMaxDevices = 64;
+void upload_tables_once(int device) {
+  static std::once_flag flags[kMaxDevices];
+  if (device >= kMaxDevices) throw Failure(CELV2_ERR_DEVICE, "HipEngine: device index above 63");
+  std::call_once(flags[device], [] {
+    check(upload_intra_tiles(), "intra tables");
+    check(upload_inter_tables(), "inter tables");
+    check(upload_final_tiles(), "compound tables");
+  });
+}
+
 int validate(celv2hip_engine* e) {
   if (!e) {
     g_error = "null engine";
@@ -350,6 +365,14 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     e->width = width, e->height = height, e->sb_size = sb_size, e->device = device;
     e->pw = (width + 7) & ~7, e->ph = (height + 7) & ~7;
     e->sb_cols = (width + 63) / 64, e->sb_rows = (height + 63) / 64;
+    // The kernels address a reference's 16 quarter-pel planes with 32-bit signed byte offsets from
+    // its level-0 corner (buffer resources of 2^31 - 1 bytes): all 16 must span less than that.
+    {
+      int stride;
+      const unsigned long long span = static_cast<unsigned long long>(plane_bytes(e->pw, e->ph, &stride)) * kPhases;
+      if (span > 0x7fffffffull)
+        throw Failure(CELV2_ERR_SIZE, "HipEngine: frame too large (a reference's 16 quarter-pel planes must span under 2 GiB; about 11,500 x 11,500 at most)");
+    }
     hipDeviceProp_t prop;
     check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     e->name = prop.name[0] ? prop.name : prop.gcnArchName;
@@ -387,10 +410,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
       for (hipEvent_t& ev : a) check(hipEventCreate(&ev), "hipEventCreate");
     e->ev = e->ring[0].data();
     check(hipMemset(e->counters, 0, kCounterBytes), "hipMemset counters");
-    upload_intra_tiles(e->stream);
-    upload_inter_tables(e->stream);
-    upload_final_tiles(e->stream);
-    check(hipStreamSynchronize(e->stream), "table upload");
+    upload_tables_once(device);
     *out = e.release();
   });
 }
@@ -870,7 +890,7 @@ void launch_batch(const LaunchIn* in, int n) {
     for (int r = 0; r < in[i].nrefs; r++) fa.ref[r] = used[i][r]->p.lv[0];
     return fa;
   };
-  // Rows: row streams as VulkanEngine orders them (src/vk/vk_engine.cpp:288-294).
+  // Rows: row streams as VulkanEngine orders them (src/vk/vk_engine.cpp:254-260).
   for (int i = 0; i < n; i++) {
     celv2hip_engine* e = in[i].e;
     const int streams = std::max(1, std::min(e->row_streams, e->sb_rows));

@@ -7,7 +7,7 @@
 #include <string>
 #include <vector>
 
-#include "../../../include/celv2hip.h"
+#include "celv2hip.h"
 #include "celv2/hip_engine.h"
 
 namespace {
@@ -134,5 +134,10 @@ int celv2_engine_description(void* p, char* buf, int cap) {
 }
 
 int celv2_sizeof_sb(void) { return static_cast<int>(sizeof(celv2_sb)); }
+
+#ifndef CELV2_SOURCE_HASH
+#define CELV2_SOURCE_HASH "unknown"
+#endif
+const char* celv2hip_source_hash(void) { return CELV2_SOURCE_HASH; }
 
 }  // extern "C"

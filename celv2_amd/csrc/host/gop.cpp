@@ -1,7 +1,7 @@
 // The frame loop's coding schedule (include/celv2/gop.h): which frame is coded when, hidden or shown,
 // at which qindex delta, and from which references.
 //
-// The schedule is the part of ChunkCoder::code (src/chunk_coder.cpp:155-600) that does not look at the
+// The schedule is the part of ChunkCoder::code (src/chunk_coder.cpp:155-616) that does not look at the
 // pictures; the reference slots follow CodecState::begin_frame (src/state/codec_state.cpp:476-498) and
 // the ranking Pipeline::references hands the engine (src/pipeline.cpp:298-309).
 #include "celv2/gop.h"

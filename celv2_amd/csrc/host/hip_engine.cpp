@@ -13,8 +13,11 @@
 #include <type_traits>
 #include <vector>
 
-#include "../../../include/celv2hip.h"
 #include "celv2/lambda.h"
+#include "celv2hip.h"
+#ifdef CELV2_IN_TREE
+#include "gpu/kernels_ref.h"  // CelV2's lambda_q8 (defined in src/core/quant.cpp)
+#endif
 
 namespace celv2 {
 

@@ -13,18 +13,18 @@ namespace celv2hip {
 
 constexpr int kBorder = 64;         // >= the widest window read at one origin (a 64x64 block)
 constexpr int kSlack = 256;         // bytes before / after each plane allocation for aligned over-reads
-constexpr int kLevels = 5;          // src/gpu/inter_common.h:209
-constexpr int kPhases = 16;         // src/gpu/inter_common.h:176
+constexpr int kLevels = 5;          // src/gpu/inter_common.h:44
+constexpr int kPhases = 16;         // src/gpu/inter_common.h:11
 constexpr int kModes = 13;          // src/gpu/intra_common.h:42
-constexpr int kMaxRefs = 7;         // src/gpu/inter_common.h:211
+constexpr int kMaxRefs = 7;         // src/gpu/inter_common.h:46
 constexpr int kBlocks = 425;        // src/gpu/sb_blocks.h:10
 constexpr int kSquares = 85;        // src/gpu/sb_blocks.h:9
-constexpr int kRange = 8;           // src/gpu/inter_common.h:200, 205
-constexpr int kPelPenalty = 4;      // src/gpu/inter_common.h:220
-constexpr int kQpelPenalty = 16;    // src/gpu/inter_common.h:305
+constexpr int kRange = 8;           // src/gpu/inter_common.h:35, 40
+constexpr int kPelPenalty = 4;      // src/gpu/inter_common.h:55
+constexpr int kQpelPenalty = 16;    // src/gpu/inter_common.h:140
 constexpr int kSplitBits = 2;       // src/gpu/intra_common.h:33
-constexpr int kIntraOnInter = 32;   // src/gpu/inter_common.h:215
-constexpr int kNewmvBits = 3;       // src/gpu/inter_common.h:210
+constexpr int kIntraOnInter = 32;   // src/gpu/inter_common.h:50
+constexpr int kNewmvBits = 3;       // src/gpu/inter_common.h:45
 constexpr int kMaxCands = 72;       // candidates of the 64x64 block: 64 cells + 8 neighbours
 constexpr int kRegion = 10;         // 10x10 8x8-cells: a superblock's cells and the ring around it
 constexpr int kTasks = 4296;        // (block, candidate) pairs of a superblock
@@ -119,7 +119,7 @@ __device__ __forceinline__ uint32_t pack_mv(int x, int y) { return (static_cast<
 __device__ __forceinline__ int mv_x(uint32_t m) { return static_cast<int16_t>(m & 0xffffu); }
 __device__ __forceinline__ int mv_y(uint32_t m) { return static_cast<int16_t>(m >> 16); }
 
-// Candidate-block geometry (src/gpu/sb_blocks.h:172-187).
+// Candidate-block geometry (src/gpu/sb_blocks.h:17-27).
 // Event counters are spread over 64 slots 16 words (128 bytes) apart, so that workgroups finishing
 // together do not queue on one address; the host sums the slots.
 constexpr int kCounterSlots = 64, kCounterStride = 16;

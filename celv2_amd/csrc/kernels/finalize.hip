@@ -1,6 +1,6 @@
 // Per superblock: each candidate block's reference choice and compound (best_both,
-// src/gpu/kernels_ref.cpp:104-131), its intra choice, its record (BlockCandidates), the bottom-up
-// partition decision (decide_square, src/gpu/kernels_ref.cpp:178-251) and the exported superblock
+// src/gpu/kernels_ref.cpp:43-104), its intra choice, its record (BlockCandidates), the bottom-up
+// partition decision (decide_square, src/gpu/kernels_ref.cpp:124-198) and the exported superblock
 // record (celv2_sb, include/celv2hip.h).
 //
 // One workgroup per superblock. Compound SATDs run on the matrix cores (hadamard.h): the 3072 tiles of
@@ -10,8 +10,9 @@
 // exact integers (order-free). The partition DP runs level by level with one thread per square, and
 // one thread walks the chosen tree.
 #include <stdexcept>
+#include <vector>
 
-#include "../../../include/celv2hip.h"
+#include "celv2hip.h"
 #include "common.h"
 #include "hadamard.h"
 #include "launch.h"
@@ -27,13 +28,13 @@ constexpr int kThreads = 1024;  // 16 waves: the compound rows' gathers spread o
 __device__ uint32_t g_all_tiles[kAllTiles];  // b | tx << 16 | ty << 24
 __device__ uint8_t g_tile_seg[kAllTiles / 16];  // tiles per block (at most 16) of each 16-tile step
 
-__device__ __forceinline__ bool pair_allowed(int a, int b, int n) {  // compound_pair_ok, inter_common.h:364-367
+__device__ __forceinline__ bool pair_allowed(int a, int b, int n) {  // compound_pair_ok, inter_common.h:199-202
   const int lo = a < b ? a : b, lim = n - 2 < 2 ? n - 2 : 2;
   return a != b && lo <= lim;
 }
 
 // (p + q + 1) >> 1 of each byte pair (the compound average, block_compound_satd,
-// src/gpu/inter_common.h:370-385): (p | q) - ((p ^ q) >> 1).
+// src/gpu/inter_common.h:205-220): (p | q) - ((p ^ q) >> 1).
 __device__ __forceinline__ uint32_t avg_bytes(uint32_t p, uint32_t q) { return (p | q) - (((p ^ q) >> 1) & 0x7f7f7f7fu); }
 
 // Row g of the 4x4 tile at (x, y) of a bordered plane, the tile's origin clamped as a 4x4 window.
@@ -178,7 +179,7 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
       rec.ref_mv_row[r] = static_cast<int16_t>(has ? 2 * mv_y(rmv[r][b]) : 0);
     }
     out->cands[b] = rec;
-    const bool inter = ci < c;  // inter only when strictly cheaper (kernels_ref.cpp:145)
+    const bool inter = ci < c;  // inter only when strictly cheaper (kernels_ref.cpp:92, 103)
     bcost[b] = inter ? ci : c;
     bmode[b] = static_cast<uint8_t>(inter ? kModes : m);
     bref[b] = static_cast<uint8_t>(r1), bref2[b] = static_cast<uint8_t>(r2);
@@ -296,9 +297,10 @@ void launch_finalize(const Batch<FinalizeArgs>& b, hipStream_t s) {
 
 // All blocks' tiles, blocks in decreasing tile count (256 .. 2, every class a multiple of 16 tiles),
 // each block's tiles contiguous; seg[step] = min(16, tiles per block) of the step's blocks.
-void upload_final_tiles(hipStream_t s) {
-  static uint32_t table[kAllTiles];
-  static uint8_t seg[kAllTiles / 16];
+// Built in local memory and copied to this device's symbols (synchronous).
+hipError_t upload_final_tiles() {
+  std::vector<uint32_t> table(kAllTiles);
+  std::vector<uint8_t> seg(kAllTiles / 16);
   int n = 0;
   for (int count = 256; count >= 2; count /= 2)
     for (int b = 0; b < kBlocks; b++) {
@@ -315,8 +317,9 @@ void upload_final_tiles(hipStream_t s) {
         }
     }
   if (n != kAllTiles) throw std::logic_error("compound tile table size");
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_all_tiles), table, sizeof(table), 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_tile_seg), seg, sizeof(seg), 0, hipMemcpyHostToDevice, s);
+  hipError_t rc = hipMemcpyToSymbol(HIP_SYMBOL(g_all_tiles), table.data(), table.size() * sizeof(uint32_t));
+  if (rc == hipSuccess) rc = hipMemcpyToSymbol(HIP_SYMBOL(g_tile_seg), seg.data(), seg.size());
+  return rc;
 }
 
 }  // namespace celv2hip

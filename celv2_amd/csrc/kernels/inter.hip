@@ -1,9 +1,9 @@
 // Single-reference inter scores of a superblock: for each of the 425 candidate blocks, the first
-// least SATD over its MV candidates (best_both's per-reference loop, src/gpu/kernels_ref.cpp:104-119)
+// least SATD over its MV candidates (best_both's per-reference loop, src/gpu/kernels_ref.cpp:51-66)
 // and that MV.
 //
 // The candidates of all 425 blocks are MVs of the 10 x 10 cells of the level-0 field around the
-// superblock (its 64 cells and the ring, src/gpu/inter_common.h:225-243), and every candidate block's
+// superblock (its 64 cells and the ring, src/gpu/inter_common.h:57-78), and every candidate block's
 // 4x4 tiles lie on the superblock's 4x4 grid. So per distinct MV the kernel scores the whole 64x64
 // area once - 256 tile SATDs - and sums tiles up the block quadtree; each block's cost for any
 // candidate is then a lookup. The tile transforms run on the matrix cores (hadamard.h): per MV, 16
@@ -18,6 +18,7 @@
 // reference's choice (a repeated MV costs the same as its first occurrence and never wins; the
 // lambda * (3 + r) term is the same for every candidate of a reference and cannot change the order).
 #include <stdexcept>
+#include <vector>
 
 #include "common.h"
 #include "hadamard.h"
@@ -247,7 +248,7 @@ void launch_inter(const Batch<InterArgs>& b, hipStream_t s) {
 
 // Candidate k of block b as a cell of the 10 x 10 region around the superblock (region column / row
 // = superblock cell + 1): the block's cells in raster order, then left, above, right, below,
-// above-left, above-right, below-left, below-right (candidate_mv_index, src/gpu/inter_common.h:230-243,
+// above-left, above-right, below-left, below-right (candidate_mv_index, src/gpu/inter_common.h:65-78,
 // before its clamping, which the region's clamped load reproduces).
 static int region_cell(int b, int k) {
   const int sq = b / 5, kind = b % 5;
@@ -279,9 +280,10 @@ static int cand_total(int b) {
   return ((x + w - 1) / 8 - x / 8 + 1) * ((y + h - 1) / 8 - y / 8 + 1) + 8;
 }
 
-void upload_inter_tables(hipStream_t s) {
-  static uint16_t first[kRegion * kRegion + 1], tasks[kTasks];
-  static uint8_t cand[kBlocks * kMaxCands];
+// Builds the tables in local memory and copies them to this device's symbols (synchronous).
+hipError_t upload_inter_tables() {
+  std::vector<uint16_t> first(kRegion * kRegion + 1), tasks(kTasks);
+  std::vector<uint8_t> cand(kBlocks * kMaxCands);
   int ccount[kRegion * kRegion] = {};
   int total = 0;
   for (int b = 0; b < kBlocks; b++)
@@ -300,9 +302,10 @@ void upload_inter_tables(hipStream_t s) {
       tasks[first[c] + fill[c]++] = static_cast<uint16_t>((b << 7) | k);
     }
   if (total != kTasks || off != kTasks) throw std::logic_error("candidate task table size");
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cell_first), first, sizeof(first), 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cell_tasks), tasks, sizeof(tasks), 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_cand_cell), cand, sizeof(cand), 0, hipMemcpyHostToDevice, s);
+  hipError_t rc = hipMemcpyToSymbol(HIP_SYMBOL(g_cell_first), first.data(), first.size() * sizeof(uint16_t));
+  if (rc == hipSuccess) rc = hipMemcpyToSymbol(HIP_SYMBOL(g_cell_tasks), tasks.data(), tasks.size() * sizeof(uint16_t));
+  if (rc == hipSuccess) rc = hipMemcpyToSymbol(HIP_SYMBOL(g_cand_cell), cand.data(), cand.size());
+  return rc;
 }
 
 }  // namespace celv2hip

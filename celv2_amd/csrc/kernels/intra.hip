@@ -277,15 +277,16 @@ void launch_intra(const Batch<IntraItem>& b, hipStream_t s) {
   if (b.total() > 0) k_intra<<<b.total(), 256, 0, s>>>(b);
 }
 
-// The shape-sorted tile list, the block order, the group ranges and the directional sample table.
-void upload_intra_tiles(hipStream_t s) {
+// The shape-sorted tile list, the block order, the group ranges and the directional sample table,
+// built in local memory and copied to this device's symbols (synchronous).
+hipError_t upload_intra_tiles() {
   static const int kShapeWH[kShapes][2] = {{32, 32}, {32, 16}, {16, 32}, {16, 16}, {16, 8}, {8, 16}, {8, 8}, {8, 4}, {4, 8}};
   static const int kAngles[kDirModes] = {45, 67, 113, 135, 157, 203};
-  static uint32_t tiles[kIntraTiles];
-  static uint16_t pos_block[420];
-  static uint32_t dir[kDirEntries];
-  static ShapeInfo shapes[kShapes];
-  static int group_pos[kGroups + 1];
+  std::vector<uint32_t> tiles(kIntraTiles);
+  std::vector<uint16_t> pos_block(420);
+  std::vector<uint32_t> dir(kDirEntries);
+  ShapeInfo shapes[kShapes] = {};
+  int group_pos[kGroups + 1] = {};
   std::vector<int> tile_pos;
   int nt = 0, np = 0, nd = 0;
   for (int si = 0; si < kShapes; si++) {
@@ -329,11 +330,12 @@ void upload_intra_tiles(hipStream_t s) {
     const int last = tile_pos[16 * kGroupSteps * (q + 1) - 1];
     if (last + 1 != group_pos[q + 1] || group_pos[q + 1] - group_pos[q] > kMaxGroupBlocks) throw std::logic_error("intra group split");
   }
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_itile), tiles, sizeof(tiles), 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_pos_block), pos_block, sizeof(pos_block), 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_dir), dir, sizeof(dir), 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_shape), shapes, sizeof(shapes), 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(g_group_pos), group_pos, sizeof(group_pos), 0, hipMemcpyHostToDevice, s);
+  hipError_t rc = hipMemcpyToSymbol(HIP_SYMBOL(g_itile), tiles.data(), tiles.size() * sizeof(uint32_t));
+  if (rc == hipSuccess) rc = hipMemcpyToSymbol(HIP_SYMBOL(g_pos_block), pos_block.data(), pos_block.size() * sizeof(uint16_t));
+  if (rc == hipSuccess) rc = hipMemcpyToSymbol(HIP_SYMBOL(g_dir), dir.data(), dir.size() * sizeof(uint32_t));
+  if (rc == hipSuccess) rc = hipMemcpyToSymbol(HIP_SYMBOL(g_shape), shapes, sizeof(shapes));
+  if (rc == hipSuccess) rc = hipMemcpyToSymbol(HIP_SYMBOL(g_group_pos), group_pos, sizeof(group_pos));
+  return rc;
 }
 
 }  // namespace celv2hip

@@ -106,7 +106,10 @@ struct FeatureArgs {
 };
 void launch_features(const FeatureArgs& a, hipStream_t s);
 
-// Static tables the kernels read (built once per process on the host, copied to the device).
-void upload_tables(hipStream_t s);
+// Constant tables the kernels read: each builds its tables on the host and copies them to the
+// current device's symbols before returning (once per device, from the engine's creation).
+hipError_t upload_intra_tiles();
+hipError_t upload_inter_tables();
+hipError_t upload_final_tiles();
 
 }  // namespace celv2hip

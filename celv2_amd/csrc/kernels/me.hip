@@ -1,7 +1,7 @@
 // Hierarchical motion search, one pyramid level for every reference in one launch.
 //
 // One wave (64 lanes) searches one 8x8 block against one reference (me_cell + me_refine,
-// src/gpu/inter_common.h:306-358):
+// src/gpu/inter_common.h:135-193):
 //   1. the six candidate centres' SADs, 8 lanes per centre (one block row each), summed by DPP;
 //      the centre ranking is then the reference's sequential rule, wave-uniform (scalar code);
 //   2. each searched centre's 24 x 24 window is read from the plane once (144 coalesced word pairs)
@@ -73,7 +73,7 @@ __device__ __forceinline__ uint32_t score_window(const uint32_t* win, int sh, in
   return best;
 }
 
-// me_refine's 9 positions (src/gpu/inter_common.h:306-319), 2 bits each as offset + 1; 12..15 pad.
+// me_refine's 9 positions (src/gpu/inter_common.h:135-154), 2 bits each as offset + 1; 12..15 pad.
 constexpr uint32_t kRefDx = (1u << 0) | (0u << 2) | (1u << 4) | (2u << 6) | (0u << 8) | (2u << 10) | (0u << 12) | (1u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u << 8) | (1u << 10) | (2u << 12) | (2u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 
@@ -165,7 +165,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     const uint32_t c0 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row, 64));
     const uint32_t c1 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row + 1, 64));
     const uint32_t s = oct_sum(sad4(c1, w1, sad4(c0, w0, 0)));
-    // me_cell's ranking (src/gpu/inter_common.h:331-350): each distinct centre once, best and runner-up.
+    // me_cell's ranking (src/gpu/inter_common.h:164-186): each distinct centre once, best and runner-up.
     int s1 = 0x7fffffff, s2 = 0x7fffffff;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -181,7 +181,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     }
   }
 
-  // Full searches (me_search, src/gpu/inter_common.h:273-284) from windows staged in LDS.
+  // Full searches (me_search, src/gpu/inter_common.h:106-119) from windows staged in LDS.
   int koff[3], kdst[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
@@ -224,7 +224,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   }
 
   if (a.level0) {
-    // me_refine (src/gpu/inter_common.h:306-319) from the integer MV, in quarter pels. Column
+    // me_refine (src/gpu/inter_common.h:135-154) from the integer MV, in quarter pels. Column
     // j' = 16 c + j of product c is position o = j' >> 2, tile t = j' & 3 (t = 2 ty + tx); lane
     // (j, g) supplies row g of that tile. Positions 9..11 are padding (never chosen).
     const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;

@@ -13,7 +13,7 @@ namespace celv2hip {
 namespace {
 
 // 8-bit value of a frame sample: itself, or rounded from bit_depth bits exactly as round_to_8bit
-// (src/io/y4m.cpp:88-94): min(255, (v + 2^(d-9)) >> (d-8)).
+// (src/io/y4m.cpp:28-34): min(255, (v + 2^(d-9)) >> (d-8)).
 __device__ __forceinline__ uint32_t to8(uint32_t v, int bit_depth) {
   if (bit_depth <= 8) return v & 255;
   const int s = bit_depth - 8;
@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) k_import(Batch<ImportItem> b) {
   *reinterpret_cast<uint32_t*>(out.p + static_cast<ptrdiff_t>(y) * out.stride + x0) = word;
 }
 
-// Level l+1 from level l: the rounded 2x2 mean (downsample_px, src/gpu/inter_common.h:256-259).
+// Level l+1 from level l: the rounded 2x2 mean (downsample_px, src/gpu/inter_common.h:91-94).
 __global__ void __launch_bounds__(256) k_downsample(Batch<DownItem> b) {
   CELV2_PLANE_BLOCK(b, t.out, t, bx, by)
   const DPlane &in = t.in, &out = t.out;
@@ -73,14 +73,14 @@ __global__ void __launch_bounds__(256) k_downsample(Batch<DownItem> b) {
   *reinterpret_cast<uint32_t*>(out.p + static_cast<ptrdiff_t>(y) * out.stride + x0) = word;
 }
 
-// AV2 EIGHTTAP_REGULAR at quarter-pel phase q (subpel_tap, src/gpu/inter_common.h:178-184).
+// AV2 EIGHTTAP_REGULAR at quarter-pel phase q (subpel_tap, src/gpu/inter_common.h:12-19).
 constexpr int kTaps[4][8] = {{0, 0, 0, 128, 0, 0, 0, 0},
                                 {0, 2, -14, 110, 38, -10, 2, 0},
                                 {0, 2, -14, 76, 76, -14, 2, 0},
                                 {0, 2, -10, 38, 110, -14, 2, 0}};
 
 // Phases 1..15 (phase k = (qy, qx) = (k / 4, k % 4)) of level 0: horizontal then vertical 8-tap with
-// one rounding by 2^14, clamped to 0..255 (subpel_px, src/gpu/inter_common.h:187-198), at each
+// one rounding by 2^14, clamped to 0..255 (subpel_px, src/gpu/inter_common.h:20-33), at each
 // sample's clamped position. Phase k's plane starts phase_bytes * k after level 0's.
 //
 // A group of 4 samples lies wholly left of, inside or right of the frame (x0 is 4-aligned, w a

@@ -1,7 +1,7 @@
 // Deterministic synthetic clips for the decision-engine workloads (include/celv2synth.h).
 // Integer arithmetic throughout: hash-based value noise with a fixed-point smoothstep, fixed-point
 // object motion and small-angle rotation, Irwin-Hall (sum of 4 uniforms) grain.
-#include "../../../include/celv2synth.h"
+#include "celv2synth.h"
 
 #include <algorithm>
 #include <cstdint>

@@ -88,6 +88,14 @@ def lib():
         L.celv2_lambda_q8.restype = ctypes.c_int64
         if L.celv2_sizeof_sb() != SB_DTYPE.itemsize:
             raise RuntimeError("celv2_sb layout mismatch between the library and records.SB_DTYPE")
+        L.celv2hip_source_hash.restype = ctypes.c_char_p
+        if os.path.isdir(os.path.join(_HERE, "csrc")):
+            from .build import source_hash
+
+            built, here = L.celv2hip_source_hash().decode(), source_hash()
+            if built != here:
+                raise RuntimeError(f"{LIB_PATH} was built from other sources ({built}, sources {here}): "
+                                   "run __graft_entry__.build()")
         _LIB = L
     return _LIB
 

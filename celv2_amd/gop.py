@@ -3,9 +3,9 @@ and a driver that runs the HIP engine over it.
 
 `plan()` gives the stream's entries in order: keyframes, lone frames and mini-GOPs (hidden anchor,
 hidden pyramid frames, shown B frames, show-existing entries), each coded frame's qindex and ranked
-references (ChunkCoder::code, src/chunk_coder.cpp:155-600; the slots of CodecState::begin_frame,
+references (ChunkCoder::code, src/chunk_coder.cpp:155-616; the slots of CodecState::begin_frame,
 src/state/codec_state.cpp:476-498). `drive()` launches the engine on every coded frame the way
-Pipeline::code does (src/pipeline.cpp:124-146): the frame's references with their display indices as
+Pipeline::code does (src/pipeline.cpp:125-177): the frame's references with their display indices as
 keys, lambda_q8 of the frame's qindex, and the next coded frame prepared behind the launch.
 """
 import ctypes

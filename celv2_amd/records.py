@@ -1,7 +1,7 @@
 """numpy views of the flat decision-map records of include/celv2hip.h.
 
 `SB_DTYPE` matches `celv2_sb` byte for byte (BlockDecision / BlockCandidates / SbDecision of
-CelV2's src/gpu/decision_map.h:74-124, flattened); `compare_maps` reports the first difference
+CelV2's src/gpu/decision_map.h:22-78, flattened); `compare_maps` reports the first difference
 between two exported maps, field by field, the way the reference's tests compare DecisionMaps
 (tests/test_gpu_intra.cpp:144-165: trees, leaves and every candidate).
 """

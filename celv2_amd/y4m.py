@@ -1,6 +1,6 @@
 """y4m input (ctypes over the celv2_y4m_* C ABI of libcelv2hip.so, include/celv2/y4m.h): 4:2:0 at 8 or
 10 bits, frames padded to a multiple of 8 by edge replication, with the 8-bit planes the decisions
-read (CelV2's Y4mReader, src/io/y4m.cpp:57-160)."""
+read (CelV2's Y4mReader, src/io/y4m.cpp:56-158)."""
 import ctypes
 
 import numpy as np

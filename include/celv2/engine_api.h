@@ -5,8 +5,8 @@
 // include path):
 //   Frame420 (the fields the engines read)   src/io/y4m.h:11-23
 //   Part, IntraMode, BlockDecision,
-//   BlockCandidates, SbDecision, DecisionMap src/gpu/decision_map.h:69-130
-//   RowGate                                  src/gpu/decision_map.h:139-160, decision_map.cpp:19-62
+//   BlockCandidates, SbDecision, DecisionMap src/gpu/decision_map.h:17-78
+//   RowGate                                  src/gpu/decision_map.h:83-108, decision_map.cpp:19-61
 //   DecisionEngine                           src/gpu/decision_engine.h:9-51
 #pragma once
 #include <chrono>
@@ -66,7 +66,8 @@ struct DecisionMap {
 };
 
 // Rows of a DecisionMap handed from the collector to the tile threads; a failure is recorded once
-// and rethrown to every waiter.
+// and rethrown to every waiter. This standalone build keeps the ready rows as a bitmap and the
+// length of its leading run of set bits (ready_rows()); in-tree builds use CelV2's own RowGate.
 class RowGate {
  public:
   RowGate() = default;
@@ -80,11 +81,12 @@ class RowGate {
   std::chrono::steady_clock::time_point first_ready() const;
 
  private:
-  mutable std::mutex m_;
-  mutable std::condition_variable cv_;
-  int ready_ = 0;
-  std::vector<char> one_;
-  std::exception_ptr error_;
+  void mark_locked(int first, int last);  // sets rows first..last, then extends the leading run
+  mutable std::mutex mu_;
+  mutable std::condition_variable changed_;
+  std::vector<uint64_t> bits_;  // bit r of word r / 64: row r is ready
+  int run_ = 0;                 // rows 0 .. run_ - 1 are all ready
+  std::exception_ptr failure_;
   std::chrono::steady_clock::time_point first_ready_{};
 };
 

@@ -2,7 +2,7 @@
  * pyramid of hidden middle frames, each coded frame's qindex delta and its ranked references.
  *
  * It restates what CelV2's frame loop decides before any frame is coded (ChunkCoder::code,
- * src/chunk_coder.cpp:155-600, its LoopConfig src/chunk_coder.h:85-100, and the defaults main derives,
+ * src/chunk_coder.cpp:155-616, its LoopConfig src/chunk_coder.h:85-100, and the defaults main derives,
  * src/main.cpp:739-740) and what CodecState does with the reference slots (begin_frame,
  * src/state/codec_state.cpp:476-498):
  *   - the chunk's frames are taken in display order; a keyframe is coded alone, a run of 2 or more

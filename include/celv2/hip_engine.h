@@ -26,7 +26,7 @@ namespace celv2 {
 class HipEngine : public DecisionEngine {
  public:
   // Throws std::runtime_error when there is no HIP device, the size is not positive or sb_size is not
-  // 64 (VulkanEngine::VulkanEngine, src/vk/vk_engine.cpp:172-180).
+  // 64 (VulkanEngine::VulkanEngine, src/vk/vk_engine.cpp:138-146).
   HipEngine(int width, int height, int sb_size, int device = 0);
   ~HipEngine() override;
   HipEngine(const HipEngine&) = delete;

@@ -2,8 +2,8 @@
 // a multiple of 8 by edge replication, plus the 8-bit planes the decisions read.
 //
 // The same functions and behaviour as CelV2's src/io/y4m.{h,cpp} (make_frame, round_to_8bit,
-// pad_frame: y4m.cpp:12-55; Y4mReader: y4m.h:32-56, y4m.cpp:57-160; write_y4m_frame_bytes:
-// y4m.cpp:162-179), so a standalone caller feeds HipEngine the frames CelV2's pipeline would. In-tree
+// pad_frame: y4m.cpp:12-55; Y4mReader: y4m.h:32-56, y4m.cpp:56-158; write_y4m_frame_bytes:
+// y4m.cpp:160-179), so a standalone caller feeds HipEngine the frames CelV2's pipeline would. In-tree
 // builds use CelV2's own (this file is not compiled with -DCELV2_IN_TREE).
 #pragma once
 #include <stddef.h>

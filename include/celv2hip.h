@@ -9,8 +9,8 @@
  *   - frames are the padded 8-bit luma plane the decisions use (Frame420::y, src/io/y4m.h:11-23):
  *     pw x ph samples, pw = (width + 7) & ~7, ph = (height + 7) & ~7, stride pw; or the same plane
  *     at more than 8 bits (Frame420::hy), rounded on the device by min(255, (v + 2^(d-9)) >> (d-8))
- *     exactly as round_to_8bit (src/io/y4m.cpp:88-94);
- *   - motion vectors in the exported records are 1/8 pel (BlockDecision, src/gpu/decision_map.h:74-88);
+ *     exactly as round_to_8bit (src/io/y4m.cpp:28-34);
+ *   - motion vectors in the exported records are 1/8 pel (BlockDecision, src/gpu/decision_map.h:22-36);
  *   - costs are SATD * 256 + lambda_q8 * bits (src/gpu/kernels_ref.cpp:46-77); lambda_q8 is an input
  *     (the reference computes it on the host, src/vk/vk_engine.cpp:253).
  */
@@ -23,10 +23,10 @@
 extern "C" {
 #endif
 
-#define CELV2_SB_CAP 256    /* tree entries / leaves per 64x64 superblock (src/gpu/sb_out.h:10) */
+#define CELV2_SB_CAP 256    /* tree entries / leaves per 64x64 superblock (src/gpu/sb_out.h:11) */
 #define CELV2_SB_BLOCKS 425 /* candidate blocks per 64x64 superblock (src/gpu/sb_blocks.h:9-10) */
-#define CELV2_MAX_REFS 7    /* AV2's INTER_REFS_PER_FRAME (src/gpu/inter_common.h:211) */
-#define CELV2_NUM_MODES 13  /* intra modes decided (src/gpu/decision_map.h:71-72) */
+#define CELV2_MAX_REFS 7    /* AV2's INTER_REFS_PER_FRAME (src/gpu/inter_common.h:46) */
+#define CELV2_NUM_MODES 13  /* intra modes decided (src/gpu/decision_map.h:19-20) */
 #define CELV2_SINGLE_REF 255
 
 /* Status codes returned by every entry point; celv2hip_last_error() describes the last failure on
@@ -44,7 +44,7 @@ enum celv2_status {
 
 /* ---- The decision map as flat records (the engine's whole output) ---------------------------- */
 
-/* BlockDecision (src/gpu/decision_map.h:74-88). inter, intra: 0/1 and IntraMode. */
+/* BlockDecision (src/gpu/decision_map.h:22-36). inter, intra: 0/1 and IntraMode. */
 typedef struct celv2_block {
   uint8_t w4, h4, inter, intra;
   int16_t mv_row, mv_col;
@@ -52,7 +52,7 @@ typedef struct celv2_block {
   int16_t mv2_row, mv2_col;
 } celv2_block;
 
-/* BlockCandidates (src/gpu/decision_map.h:93-102). */
+/* BlockCandidates (src/gpu/decision_map.h:38-50). */
 typedef struct celv2_cands {
   int64_t inter_cost, intra_cost;
   celv2_block inter;
@@ -62,7 +62,7 @@ typedef struct celv2_cands {
   int16_t ref_mv_col[CELV2_MAX_REFS], ref_mv_row[CELV2_MAX_REFS];
 } celv2_cands;
 
-/* SbDecision (src/gpu/decision_map.h:106-124) of one 64x64 superblock: preorder tree (Part values
+/* SbDecision (src/gpu/decision_map.h:52-72) of one 64x64 superblock: preorder tree (Part values
  * None 0, Horz 1, Vert 2, Split 3), leaves in coding order, and every candidate block's choices in
  * src/gpu/sb_blocks.h order (ncands 0 where the superblock is not 64x64). */
 typedef struct celv2_sb {
@@ -94,7 +94,7 @@ const char* celv2hip_last_error(void);
 /* Number of HIP devices visible (0 without a GPU). */
 int celv2hip_device_count(int* n);
 
-/* VulkanEngine::VulkanEngine (src/vk/vk_engine.cpp:172-180): an engine for width x height frames,
+/* VulkanEngine::VulkanEngine (src/vk/vk_engine.cpp:138-146): an engine for width x height frames,
  * 64x64 superblocks only, on HIP device `device`. Fails with SIZE for a bad size. */
 int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_engine** out);
 int celv2hip_destroy(celv2hip_engine* e);
@@ -105,11 +105,11 @@ int celv2hip_device_name(const celv2hip_engine* e, char* buf, int cap);
 /* DecisionEngine::set_row_streams (src/gpu/decision_engine.h:30-33). */
 int celv2hip_set_row_streams(celv2hip_engine* e, int n);
 
-/* DecisionEngine::prepare (src/gpu/decision_engine.h:19-22; src/vk/vk_engine.cpp:188-205): queues the
+/* DecisionEngine::prepare (src/gpu/decision_engine.h:19-22; src/vk/vk_engine.cpp:154-171): queues the
  * next source's import, pyramid and intra scores behind the current work. key < 0: nothing. */
 int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key);
 
-/* DecisionEngine::launch (src/gpu/decision_engine.h:17-18; src/vk/vk_engine.cpp:207-312): queues the
+/* DecisionEngine::launch (src/gpu/decision_engine.h:17-18; src/vk/vk_engine.cpp:173-278): queues the
  * decision of every superblock row. refs nearest first, nrefs 0..7 (0: a key frame); ref_keys NULL
  * or one per reference (equal keys: the same picture, its pyramid and phases are reused). */
 int celv2hip_launch(celv2hip_engine* e, const celv2_luma* src, long long src_key, const celv2_luma* refs,
@@ -128,7 +128,7 @@ int celv2hip_launch_batch(celv2hip_engine* const* engines, int n, const celv2_lu
 /* The superblock rows in the order the last launch queued them (row streams); returns the count. */
 int celv2hip_row_order(const celv2hip_engine* e, int* rows, int cap);
 
-/* Part of DecisionEngine::collect (src/vk/vk_engine.cpp:314-328): waits for superblock row `row` of
+/* Part of DecisionEngine::collect (src/vk/vk_engine.cpp:280-294): waits for superblock row `row` of
  * the last launch and writes its sb_cols records to out. */
 int celv2hip_wait_row(celv2hip_engine* e, int row, celv2_sb* out);
 
@@ -262,6 +262,10 @@ int celv2_engine_prepare(void* engine, const uint8_t* y8, const uint16_t* y16, i
 int celv2_engine_collect(void* engine, celv2_sb* out, int* rows_ready);
 int celv2_engine_description(void* engine, char* buf, int cap);
 int celv2_sizeof_sb(void);
+
+/* Digest of the native sources the library was built from (the build's own check against a stale
+ * library; no counterpart in the reference). */
+const char* celv2hip_source_hash(void);
 
 #ifdef __cplusplus
 }

@@ -2,8 +2,8 @@
  *
  * Integer arithmetic only (hash-based value noise, fixed-point motion, Irwin-Hall grain), so the
  * same parameters give the same samples on every machine. Frames are written padded to a multiple
- * of 8 by edge replication (make_frame / pad_frame, src/io/y4m.cpp:72-114), at the requested bit
- * depth, plus the 8-bit luma the decisions read (round_to_8bit, src/io/y4m.cpp:88-94).
+ * of 8 by edge replication (make_frame / pad_frame, src/io/y4m.cpp:12-54), at the requested bit
+ * depth, plus the 8-bit luma the decisions read (round_to_8bit, src/io/y4m.cpp:28-34).
  */
 #ifndef CELV2SYNTH_H_
 #define CELV2SYNTH_H_

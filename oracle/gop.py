@@ -2,19 +2,19 @@
 
 Used only by tests/ to check celv2_gop_plan (celv2_amd/csrc/host/gop.cpp). Each step cites the
 reference it restates:
-  - the loop over keyframes, lone frames and mini-GOPs: src/chunk_coder.cpp:478-536, 590-596;
-  - the keyframe delta of a short shot: src/chunk_coder.cpp:486-502;
+  - the loop over keyframes, lone frames and mini-GOPs: src/chunk_coder.cpp:478-616 (keyframes 479-516, the mini-GOP's length 517-519, lone frames 544-550);
+  - the keyframe delta of a short shot: src/chunk_coder.cpp:486-501;
   - mids_of (the pyramid): src/chunk_coder.cpp:209-224;
   - code_minigop's order and deltas: src/chunk_coder.cpp:323-437;
   - the slots a frame refreshes: src/state/codec_state.cpp:476-498 (8 slots, seq_header.cpp:87);
   - the references handed to the engine: src/pipeline.cpp:298-309 over CodecState's ranked hints
-    (codec_state.h:62-69: ranked by display distance). AV2's tie rule is not in the reference tree;
+    (codec_state.h:84-88: ranked by display distance). AV2's tie rule is not in the reference tree;
     this restatement ranks the earlier of two equally distant frames first.
 Frames are dicts with the fields of celv2_gop_frame (include/celv2/gop.h).
 
 Pinning (tests/test_gop.py): hand-derived schedules of the C1 sequence (32 frames, 7 references,
 16-frame mini-GOPs), the reference's own test structures (tests/test_random_decisions.cpp:205-236,
-nearest past reference first; tests/test_cli.cpp:1046-1047, anchors at 8 and 16 with bframes 7)
+nearest past reference first; tests/test_cli.cpp:1044-1046, anchors at 8 and 16 with bframes 7)
 and invariants over a sweep (every frame shown once in display order, references coded before,
 distinct and nearest first).
 """

@@ -2,6 +2,8 @@
 // rules; each function cites the reference lines it follows.
 #include "oracle.h"
 
+#include <emmintrin.h>
+
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
@@ -37,7 +39,7 @@ int wht4_abs_sum(const int d[16]) {
   return total;
 }
 
-// inter_common.h:178-184 — AV2 EIGHTTAP_REGULAR at quarter-pel phases 0..3.
+// inter_common.h:12-19 — AV2 EIGHTTAP_REGULAR at quarter-pel phases 0..3.
 int qpel_tap(int q, int k) {
   static const int taps[4][8] = {{0, 0, 0, 128, 0, 0, 0, 0},
                                  {0, 2, -14, 110, 38, -10, 2, 0},
@@ -46,10 +48,21 @@ int qpel_tap(int q, int k) {
   return taps[q][k];
 }
 
-// inter_common.h:187-198 — horizontal then vertical 8-tap, one rounding by 2^14, clamped to 0..255;
+// inter_common.h:20-33 — horizontal then vertical 8-tap, one rounding by 2^14, clamped to 0..255;
 // rows with a zero vertical tap are skipped (their product is zero).
 int qpel_sample(const Plane8& s, int y, int x, int qy, int qx) {
   int acc = 0;
+  if (y >= 3 && x >= 3 && y + 5 <= s.h && x + 5 <= s.w) {  // every tap inside: plain reads
+    const uint8_t* r = s.px + static_cast<size_t>(y - 3) * s.stride + (x - 3);
+    for (int i = 0; i < 8; i++, r += s.stride) {
+      const int tv = qpel_tap(qy, i);
+      if (!tv) continue;
+      int row = 0;
+      for (int j = 0; j < 8; j++) row += qpel_tap(qx, j) * r[j];
+      acc += tv * row;
+    }
+    return std::clamp((acc + 8192) >> 14, 0, 255);
+  }
   for (int i = 0; i < 8; i++) {
     const int tv = qpel_tap(qy, i);
     if (!tv) continue;
@@ -61,20 +74,45 @@ int qpel_sample(const Plane8& s, int y, int x, int qy, int qx) {
   return std::clamp(acc, 0, 255);
 }
 
-// inter_common.h:256-259 — rounded 2x2 mean.
+// inter_common.h:91-94 — rounded 2x2 mean.
 int half_sample(const Plane8& s, int y, int x) {
   return (s.at(2 * y, 2 * x) + s.at(2 * y, 2 * x + 1) + s.at(2 * y + 1, 2 * x) + s.at(2 * y + 1, 2 * x + 1) + 2) >> 2;
 }
 
-// inter_common.h:261-269.
+// Direct addressing of the h x w area at (y, x) where it lies inside the plane, else null: clamped
+// reads there are plain reads. Only a speed path; the clamped form below each use covers the rest.
+static const uint8_t* direct(const Plane8& p, int y, int x, int h, int w) {
+  return y >= 0 && x >= 0 && y + h <= p.h && x + w <= p.w ? p.px + static_cast<size_t>(y) * p.stride + x : nullptr;
+}
+// The same for the area read by atq(y + i, x + j, qy, qx): phase plane (qy & 3) * 4 + (qx & 3).
+static const uint8_t* direct_q(const Plane8& p, int y, int x, int h, int w, int qy, int qx) {
+  Plane8 q = p;
+  q.px = p.px + static_cast<size_t>((qy & 3) * 4 + (qx & 3)) * p.phase;
+  return direct(q, y + (qy >> 2), x + (qx >> 2), h, w);
+}
+
+// inter_common.h:96-104.
 int sad_8x8(const Plane8& cur, int x, int y, const Plane8& ref, int mvx, int mvy) {
   int sad = 0;
+  const uint8_t* a = direct(cur, y, x, 8, 8);
+  const uint8_t* b = direct(ref, y + mvy, x + mvx, 8, 8);
+  if (a && b) {  // two 8-byte rows per psadbw (SSE2): the sum of |a - b| over the 64 samples
+    __m128i acc = _mm_setzero_si128();
+    for (int i = 0; i < 8; i += 2, a += 2 * cur.stride, b += 2 * ref.stride) {
+      const __m128i va = _mm_unpacklo_epi64(_mm_loadl_epi64(reinterpret_cast<const __m128i*>(a)),
+                                            _mm_loadl_epi64(reinterpret_cast<const __m128i*>(a + cur.stride)));
+      const __m128i vb = _mm_unpacklo_epi64(_mm_loadl_epi64(reinterpret_cast<const __m128i*>(b)),
+                                            _mm_loadl_epi64(reinterpret_cast<const __m128i*>(b + ref.stride)));
+      acc = _mm_add_epi64(acc, _mm_sad_epu8(va, vb));
+    }
+    return _mm_cvtsi128_si32(acc) + _mm_cvtsi128_si32(_mm_unpackhi_epi64(acc, acc));
+  }
   for (int i = 0; i < 8; i++)
     for (int j = 0; j < 8; j++) sad += std::abs(cur.at(y + i, x + j) - ref.at(y + i + mvy, x + j + mvx));
   return sad;
 }
 
-// inter_common.h:273-284 — +-range around (px, py), dy-major, first strict minimum of
+// inter_common.h:106-119 — +-range around (px, py), dy-major, first strict minimum of
 // SAD + 4 * (|dx| + |dy|).
 int full_search(const Plane8& cur, const Plane8& ref, int bx, int by, int px, int py, int range, int* mvx, int* mvy) {
   int best = 0x7fffffff;
@@ -87,21 +125,12 @@ int full_search(const Plane8& cur, const Plane8& ref, int bx, int by, int px, in
   return best;
 }
 
-// inter_common.h:287-298 — SATD of an 8x8 block against ref at a quarter-pel MV.
+// inter_common.h:121-133 — SATD of an 8x8 block against ref at a quarter-pel MV.
 int satd_8x8_q(const Plane8& cur, int x, int y, const Plane8& ref, int qmx, int qmy) {
-  int total = 0;
-  for (int ty = 0; ty < 8; ty += 4)
-    for (int tx = 0; tx < 8; tx += 4) {
-      int d[16];
-      for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++)
-          d[4 * i + j] = cur.at(y + ty + i, x + tx + j) - ref.atq(y + ty + i, x + tx + j, qmy, qmx);
-      total += wht4_abs_sum(d);
-    }
-  return total;
+  return static_cast<int>(inter_satd(cur, ref, x, y, 8, 8, qmx, qmy));
 }
 
-// inter_common.h:306-319 — half-pel step then quarter-pel step; the centre first, each of the 8
+// inter_common.h:135-154 — half-pel step then quarter-pel step; the centre first, each of the 8
 // neighbours (dy-major) charged 16 and taken only if strictly cheaper.
 void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx, int* mvy) {
   int cx = 4 * *mvx, cy = 4 * *mvy;
@@ -118,7 +147,7 @@ void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx,
   *mvx = cx, *mvy = cy;
 }
 
-// inter_common.h:327-358 — six candidate centres (coarser MV x2 under the block, its left, right,
+// inter_common.h:156-193 — six candidate centres (coarser MV x2 under the block, its left, right,
 // above and below neighbours, then zero), each distinct one ranked by SAD; the best and the
 // runner-up are both searched and the runner-up's result kept only if strictly cheaper.
 int search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
@@ -255,9 +284,22 @@ int64_t intra_satd(const Plane8& s, int x, int y, int w, int h, int mode) {
   return total;
 }
 
-// inter_common.h:388-400.
+// inter_common.h:222-235.
 int64_t inter_satd(const Plane8& cur, const Plane8& ref, int x, int y, int w, int h, int mvx, int mvy) {
   int64_t total = 0;
+  const uint8_t* a = direct(cur, y, x, h, w);
+  const uint8_t* b = direct_q(ref, y, x, h, w, mvy, mvx);
+  if (a && b) {
+    for (int ty = 0; ty < h; ty += 4)
+      for (int tx = 0; tx < w; tx += 4) {
+        int d[16];
+        for (int i = 0; i < 4; i++)
+          for (int j = 0; j < 4; j++)
+            d[4 * i + j] = a[static_cast<size_t>(ty + i) * cur.stride + tx + j] - b[static_cast<size_t>(ty + i) * ref.stride + tx + j];
+        total += wht4_abs_sum(d);
+      }
+    return total;
+  }
   for (int ty = 0; ty < h; ty += 4)
     for (int tx = 0; tx < w; tx += 4) {
       int d[16];
@@ -269,10 +311,26 @@ int64_t inter_satd(const Plane8& cur, const Plane8& ref, int x, int y, int w, in
   return total;
 }
 
-// inter_common.h:370-385 — the two predictions averaged with +1 rounding.
+// inter_common.h:205-220 — the two predictions averaged with +1 rounding.
 int64_t compound_satd(const Plane8& cur, const Plane8& r0, int mx0, int my0, const Plane8& r1, int mx1, int my1,
                       int x, int y, int w, int h) {
   int64_t total = 0;
+  const uint8_t* a = direct(cur, y, x, h, w);
+  const uint8_t* p0 = direct_q(r0, y, x, h, w, my0, mx0);
+  const uint8_t* p1 = direct_q(r1, y, x, h, w, my1, mx1);
+  if (a && p0 && p1) {
+    for (int ty = 0; ty < h; ty += 4)
+      for (int tx = 0; tx < w; tx += 4) {
+        int d[16];
+        for (int i = 0; i < 4; i++)
+          for (int j = 0; j < 4; j++) {
+            const size_t o = static_cast<size_t>(ty + i) * r0.stride + tx + j;
+            d[4 * i + j] = a[static_cast<size_t>(ty + i) * cur.stride + tx + j] - ((p0[o] + p1[static_cast<size_t>(ty + i) * r1.stride + tx + j] + 1) >> 1);
+          }
+        total += wht4_abs_sum(d);
+      }
+    return total;
+  }
   for (int ty = 0; ty < h; ty += 4)
     for (int tx = 0; tx < w; tx += 4) {
       int d[16];
@@ -286,11 +344,11 @@ int64_t compound_satd(const Plane8& cur, const Plane8& r0, int mx0, int my0, con
   return total;
 }
 
-// inter_common.h:364-369.
+// inter_common.h:195-204.
 bool pair_allowed(int a, int b, int nrefs) { return a != b && std::min(a, b) <= std::min(nrefs - 2, 2); }
 int pair_bits(int a, int b) { return 20 + a + b; }
 
-// inter_common.h:225-243 — candidates: the block's 8x8 cells in raster order, then left, above,
+// inter_common.h:57-78 — candidates: the block's 8x8 cells in raster order, then left, above,
 // right, below, above-left, above-right, below-left, below-right; field coordinates clamped.
 int cells_covered(int x, int y, int w, int h) { return ((x + w - 1) / 8 - x / 8 + 1) * ((y + h - 1) / 8 - y / 8 + 1); }
 int cand_count(int x, int y, int w, int h) { return cells_covered(x, y, w, h) + 8; }
@@ -311,7 +369,7 @@ int cand_cell(int x, int y, int w, int h, int k, int cols, int rows) {
   cy = std::clamp(cy, 0, rows - 1);
   return cy * cols + cx;
 }
-// inter_common.h:247-254.
+// inter_common.h:80-89.
 bool cand_repeats(const int* fx, const int* fy, int cols, int rows, int x, int y, int w, int h, int k) {
   const int i = cand_cell(x, y, w, h, k, cols, rows);
   for (int j = 0; j < k; j++) {
@@ -321,7 +379,7 @@ bool cand_repeats(const int* fx, const int* fy, int cols, int rows, int x, int y
   return false;
 }
 
-// sb_blocks.h:172-197 — 85 squares (64, 32, 16, 8; raster within a level), each whole then its
+// sb_blocks.h:1-37 — 85 squares (64, 32, 16, 8; raster within a level), each whole then its
 // top, bottom, left and right halves: block b = square * 5 + kind.
 int level_first(int level) { return ((1 << (2 * level)) - 1) / 3; }
 int square_id(int level, int qx, int qy) { return level_first(level) + (qy << level) + qx; }
@@ -361,7 +419,7 @@ void round_plane(const uint16_t* in, uint8_t* out, size_t n, int bit_depth) {
     out[i] = shift <= 0 ? static_cast<uint8_t>(in[i]) : static_cast<uint8_t>(std::min(255, (in[i] + (1 << (shift - 1))) >> shift));
 }
 
-// kernels_ref.cpp:266-291 — level 0 the frame (a reference's followed by phases 1..15 at the
+// kernels_ref.cpp:213-238 — level 0 the frame (a reference's followed by phases 1..15 at the
 // clamped positions, one plane apart), levels 1..4 halved (floor) by half_sample.
 Pyramid build_pyramid(const Luma& f, bool phases) {
   Pyramid p;
@@ -369,12 +427,34 @@ Pyramid build_pyramid(const Luma& f, bool phases) {
   const size_t n = f.y.size();
   p.px[0] = f.y;
   if (phases) {
+    // qpel_sample at every position, computed separably: the horizontal 8-tap sum of each clamped
+    // row at each horizontal phase once (the same integers qpel_sample forms for that row), then the
+    // vertical taps over the clamped rows' sums and the one rounding.
     p.px[0].resize(n * kPhases);
     const Plane8 src{f.y.data(), f.pw, f.pw, f.ph, 0};
-    for (int k = 1; k < kPhases; k++)
+    std::vector<int> rows(n);
+    for (int qx = 0; qx < 4; qx++) {
       for (int y = 0; y < f.ph; y++)
-        for (int x = 0; x < f.pw; x++)
-          p.px[0][k * n + static_cast<size_t>(y) * f.pw + x] = static_cast<uint8_t>(qpel_sample(src, y, x, k / 4, k % 4));
+        for (int x = 0; x < f.pw; x++) {
+          int r = 0;
+          for (int j = 0; j < 8; j++) r += qpel_tap(qx, j) * src.at(y, x + j - 3);
+          rows[static_cast<size_t>(y) * f.pw + x] = r;
+        }
+      for (int qy = 0; qy < 4; qy++) {
+        const int k = qy * 4 + qx;
+        if (k == 0) continue;
+        uint8_t* out = p.px[0].data() + k * n;
+        for (int y = 0; y < f.ph; y++)
+          for (int x = 0; x < f.pw; x++) {
+            int acc = 0;
+            for (int i = 0; i < 8; i++) {
+              const int tv = qpel_tap(qy, i);
+              if (tv) acc += tv * rows[static_cast<size_t>(std::clamp(y + i - 3, 0, f.ph - 1)) * f.pw + x];
+            }
+            out[static_cast<size_t>(y) * f.pw + x] = static_cast<uint8_t>(std::clamp((acc + 8192) >> 14, 0, 255));
+          }
+      }
+    }
   }
   p.plane[0] = Plane8{p.px[0].data(), f.pw, f.pw, f.ph, phases ? n : 0};
   for (int l = 1; l < kLevels; l++) {
@@ -388,7 +468,7 @@ Pyramid build_pyramid(const Luma& f, bool phases) {
   return p;
 }
 
-// kernels_ref.cpp:293-304.
+// kernels_ref.cpp:240-251.
 void search_rows(const Pyramid& cur, const Pyramid& ref, int level, const Field* coarser, int row0, int row1,
                  Field& out) {
   for (int by = row0; by < row1; by++)
@@ -411,7 +491,7 @@ void search_all(const Pyramid& cur, const Pyramid& ref, std::vector<Field>& lv) 
   }
 }
 
-// ---- per-block choice and partition DP (kernels_ref.cpp:81-251) -------------------------------------
+// ---- per-block choice and partition DP (kernels_ref.cpp:26-198) --------------------------------------
 
 namespace {
 
@@ -428,7 +508,7 @@ struct Scores {
   int nrefs = 0;
 };
 
-// kernels_ref.cpp:96-132 — intra modes in order; each reference's candidates in order (repeats
+// kernels_ref.cpp:41-79 — intra modes in order; each reference's candidates in order (repeats
 // skipped); the best reference, the runner-up and their compound (first strict minimum throughout).
 Scores score_block(const Plane8& s, const RefSet& rs, int x, int y, int w, int h, int64_t lam, bool with_intra) {
   Scores sc;
@@ -464,7 +544,7 @@ Scores score_block(const Plane8& s, const RefSet& rs, int x, int y, int w, int h
   return sc;
 }
 
-// kernels_ref.cpp:142-157 — the block's choice (inter only when strictly cheaper) and, for 64x64
+// kernels_ref.cpp:87-104 — the block's choice (inter only when strictly cheaper) and, for 64x64
 // superblocks, its candidate record.
 Pick choose(const Plane8& s, const RefSet& rs, int x, int y, int w, int h, int64_t lam, std::vector<Cands>* rec,
             int ox, int oy, bool with_intra) {
@@ -485,7 +565,7 @@ Pick choose(const Plane8& s, const RefSet& rs, int x, int y, int w, int h, int64
   return sc.inter.cost < sc.intra.cost ? sc.inter : sc.intra;
 }
 
-// kernels_ref.cpp:159-175 — a leaf's BlockDecision (quarter pels to 1/8).
+// kernels_ref.cpp:106-122 — a leaf's BlockDecision (quarter pels to 1/8).
 void add_leaf(Sb& sb, int w, int h, const Pick& p) {
   sb.tree.push_back(kNone);
   Block b;
@@ -510,7 +590,7 @@ void append(Sb& out, const Sb& sub) {
 
 }  // namespace
 
-// kernels_ref.cpp:178-251 — bottom-up; NONE, HORZ, VERT, SPLIT with the first strict minimum.
+// kernels_ref.cpp:124-198 — bottom-up; NONE, HORZ, VERT, SPLIT with the first strict minimum.
 // Above 32x32 only inter blocks (inter frames, up to 64x64) compete with SPLIT; 8x8 never splits.
 int64_t decide_square(const Plane8& s, const RefSet& rs, int x, int y, int size, int64_t lam, Sb& out,
                       std::vector<Cands>* rec, int ox, int oy) {
@@ -547,7 +627,7 @@ int64_t decide_square(const Plane8& s, const RefSet& rs, int x, int y, int size,
   return best;
 }
 
-// kernels_ref.cpp:306-323 — candidates are recorded only for 64x64 superblocks, pre-filled with each
+// kernels_ref.cpp:253-270 — candidates are recorded only for 64x64 superblocks, pre-filled with each
 // block's size and the inter flag.
 Sb decide_sb(const Luma& src, const RefSet& rs, int sb_col, int sb_row, int sb_size, int64_t lam) {
   const Plane8 s{src.y.data(), src.pw, src.pw, src.ph, 0};
@@ -586,7 +666,7 @@ Map decide_frame(const Luma& src, const std::vector<const Luma*>& refs, int sb_s
   return m;
 }
 
-// ---- RowGate (decision_map.cpp:19-62) ---------------------------------------------------------------
+// ---- RowGate (decision_map.cpp:19-61) ---------------------------------------------------------------
 
 void RowGate::publish(int row) {
   {

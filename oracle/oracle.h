@@ -22,21 +22,21 @@
 
 namespace orc {
 
-constexpr int kLevels = 5;         // inter_common.h:209
-constexpr int kPhases = 16;        // inter_common.h:176
+constexpr int kLevels = 5;         // inter_common.h:44
+constexpr int kPhases = 16;        // inter_common.h:11
 constexpr int kModes = 13;         // intra_common.h:42
-constexpr int kMaxRefs = 7;        // inter_common.h:211
+constexpr int kMaxRefs = 7;        // inter_common.h:46
 constexpr int kSquares = 85;       // sb_blocks.h:9
 constexpr int kBlocks = 425;       // sb_blocks.h:10
-constexpr int kRange = 8;          // inter_common.h:200, 205 (both levels)
-constexpr int kPelPenalty = 4;     // inter_common.h:220
-constexpr int kQpelPenalty = 16;   // inter_common.h:305
+constexpr int kRange = 8;          // inter_common.h:35, 40 (both levels)
+constexpr int kPelPenalty = 4;     // inter_common.h:55
+constexpr int kQpelPenalty = 16;   // inter_common.h:140
 constexpr int kIntraMaxSize = 32;  // intra_common.h:39
 constexpr int kInterMaxSize = 64;  // intra_common.h:40
 constexpr int kSplitBits = 2;      // intra_common.h:33
-constexpr int kIntraOnInter = 32;  // inter_common.h:215
-constexpr int kNewmvBits = 3;      // inter_common.h:210
-constexpr int kCap = 256;          // sb_out.h:10
+constexpr int kIntraOnInter = 32;  // inter_common.h:50
+constexpr int kNewmvBits = 3;      // inter_common.h:45
+constexpr int kCap = 256;          // sb_out.h:11
 
 // A read-only 8-bit plane read through clamped coordinates (intra_common.h:11-26). A reference's
 // level 0 also carries its 16 quarter-pel phase planes `phase` bytes apart (0: none).
@@ -62,36 +62,36 @@ struct Plane8 {
 // ---- primitives (header-only rules of the reference) -------------------------------------------
 int mode_bits(int mode);                                         // intra_common.h:32
 int wht4_abs_sum(const int d[16]);                               // intra_common.h:131-146
-int qpel_tap(int q, int k);                                      // inter_common.h:178-184
-int qpel_sample(const Plane8& s, int y, int x, int qy, int qx);  // inter_common.h:187-198
-int half_sample(const Plane8& s, int y, int x);                  // inter_common.h:256-259
-int sad_8x8(const Plane8& cur, int x, int y, const Plane8& ref, int mvx, int mvy);   // :261-269
+int qpel_tap(int q, int k);                                      // inter_common.h:12-19
+int qpel_sample(const Plane8& s, int y, int x, int qy, int qx);  // inter_common.h:20-33
+int half_sample(const Plane8& s, int y, int x);                  // inter_common.h:91-94
+int sad_8x8(const Plane8& cur, int x, int y, const Plane8& ref, int mvx, int mvy);   // inter_common.h:96-104
 int full_search(const Plane8& cur, const Plane8& ref, int bx, int by, int px, int py, int range, int* mvx,
-                int* mvy);                                                           // :273-284
-int satd_8x8_q(const Plane8& cur, int x, int y, const Plane8& ref, int qmx, int qmy); // :287-298
-void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx, int* mvy);  // :306-319
+                int* mvy);                                                           // inter_common.h:106-119
+int satd_8x8_q(const Plane8& cur, int x, int y, const Plane8& ref, int qmx, int qmy); // inter_common.h:121-133
+void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx, int* mvy);  // inter_common.h:135-154
 // Returns the integer search's winning cost (SAD + distance penalty from its centre).
 int search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
-                int prows, int* mvx, int* mvy, bool level0);                          // :327-358
+                int prows, int* mvx, int* mvy, bool level0);                          // inter_common.h:156-193
 int edge_above(const Plane8& s, int x, int y, int k);                                 // intra_common.h:47-51
-int edge_left(const Plane8& s, int x, int y, int k);                                  // :52-56
-int directional(const Plane8& s, int x, int y, int w, int h, int angle, int i, int j); // :61-85
-int intra_predict(const Plane8& s, int x, int y, int w, int h, int mode, int i, int j); // :89-118
-int dc_mean(const Plane8& s, int x, int y, int w, int h);                             // :120-128
-int64_t intra_satd(const Plane8& s, int x, int y, int w, int h, int mode);             // :149-163
-int64_t inter_satd(const Plane8& cur, const Plane8& ref, int x, int y, int w, int h, int mvx, int mvy);  // :388-400
+int edge_left(const Plane8& s, int x, int y, int k);                                  // intra_common.h:52-56
+int directional(const Plane8& s, int x, int y, int w, int h, int angle, int i, int j); // intra_common.h:61-85
+int intra_predict(const Plane8& s, int x, int y, int w, int h, int mode, int i, int j); // intra_common.h:89-118
+int dc_mean(const Plane8& s, int x, int y, int w, int h);                             // intra_common.h:120-128
+int64_t intra_satd(const Plane8& s, int x, int y, int w, int h, int mode);             // intra_common.h:149-163
+int64_t inter_satd(const Plane8& cur, const Plane8& ref, int x, int y, int w, int h, int mvx, int mvy);  // inter_common.h:222-235
 int64_t compound_satd(const Plane8& cur, const Plane8& r0, int mx0, int my0, const Plane8& r1, int mx1, int my1,
-                      int x, int y, int w, int h);                                    // :370-385
-bool pair_allowed(int a, int b, int nrefs);                                           // :364-367
-int pair_bits(int a, int b);                                                          // :369
-int cells_covered(int x, int y, int w, int h);                                        // :225-227
-int cand_count(int x, int y, int w, int h);                                           // :228
-int cand_cell(int x, int y, int w, int h, int k, int cols, int rows);                 // :230-243
-bool cand_repeats(const int* fx, const int* fy, int cols, int rows, int x, int y, int w, int h, int k);  // :247-254
-int level_first(int level);                                                           // sb_blocks.h:172
-int square_id(int level, int qx, int qy);                                             // sb_blocks.h:174
-void block_geom(int b, int* x, int* y, int* w, int* h);                               // sb_blocks.h:177-187
-int block_index(int x, int y, int w, int h);                                          // sb_blocks.h:190-197
+                      int x, int y, int w, int h);                                    // inter_common.h:205-220
+bool pair_allowed(int a, int b, int nrefs);                                           // inter_common.h:199-202
+int pair_bits(int a, int b);                                                          // inter_common.h:204
+int cells_covered(int x, int y, int w, int h);                                        // inter_common.h:60-62
+int cand_count(int x, int y, int w, int h);                                           // inter_common.h:63
+int cand_cell(int x, int y, int w, int h, int k, int cols, int rows);                 // inter_common.h:65-78
+bool cand_repeats(const int* fx, const int* fy, int cols, int rows, int x, int y, int w, int h, int k);  // inter_common.h:82-89
+int level_first(int level);                                                           // sb_blocks.h:12
+int square_id(int level, int qx, int qy);                                             // sb_blocks.h:14
+void block_geom(int b, int* x, int* y, int* w, int* h);                               // sb_blocks.h:17-27
+int block_index(int x, int y, int w, int h);                                          // sb_blocks.h:30-37
 
 // ---- frames, pyramids, motion fields -----------------------------------------------------------
 // The padded 8-bit luma of a frame (Frame420::y, src/io/y4m.h:11-23): pw x ph, stride pw.
@@ -100,10 +100,10 @@ struct Luma {
   std::vector<uint8_t> y;
 };
 Luma make_luma(int width, int height, const uint8_t* y);  // copies a pw x ph plane
-// round_to_8bit for one plane (src/io/y4m.cpp:88-94).
+// round_to_8bit for one plane (src/io/y4m.cpp:28-34).
 void round_plane(const uint16_t* in, uint8_t* out, size_t n, int bit_depth);
 
-struct Pyramid {  // build_pyramid_ref (kernels_ref.cpp:266-291)
+struct Pyramid {  // build_pyramid_ref (kernels_ref.cpp:213-238)
   std::vector<std::vector<uint8_t>> px;
   Plane8 plane[kLevels];
   Pyramid() = default;
@@ -119,12 +119,12 @@ struct Field {  // RefMvLevel (kernels_ref.h:33-36)
   int cols = 0, rows = 0;
 };
 void search_rows(const Pyramid& cur, const Pyramid& ref, int level, const Field* coarser, int row0, int row1,
-                 Field& out);  // me_level_rows_ref (kernels_ref.cpp:293-304)
+                 Field& out);  // me_level_rows_ref (kernels_ref.cpp:240-251)
 void search_all(const Pyramid& cur, const Pyramid& ref, std::vector<Field>& levels);  // levels 4..0
 
 // ---- the decision map --------------------------------------------------------------------------
 enum Part : uint8_t { kNone = 0, kHorz = 1, kVert = 2, kSplit = 3 };
-struct Block {  // BlockDecision (decision_map.h:74-88)
+struct Block {  // BlockDecision (decision_map.h:22-36)
   uint8_t w4 = 0, h4 = 0;
   bool inter = false;
   uint8_t intra = 0;
@@ -132,14 +132,14 @@ struct Block {  // BlockDecision (decision_map.h:74-88)
   uint8_t ref = 0, ref2 = 255;
   int16_t mv2_row = 0, mv2_col = 0;
 };
-struct Cands {  // BlockCandidates (decision_map.h:93-102)
+struct Cands {  // BlockCandidates (decision_map.h:38-50)
   int64_t inter_cost = INT64_MAX, intra_cost = INT64_MAX;
   Block inter;
   uint8_t intra = 0;
   uint32_t ref_cost[kMaxRefs] = {UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX, UINT32_MAX};
   int16_t ref_mv_col[kMaxRefs] = {}, ref_mv_row[kMaxRefs] = {};
 };
-struct Sb {  // SbDecision (decision_map.h:106-124), the fields the engines fill
+struct Sb {  // SbDecision (decision_map.h:52-72), the fields the engines fill
   std::vector<uint8_t> tree;
   std::vector<Block> blocks;
   std::vector<Cands> cands;
@@ -156,16 +156,16 @@ struct RefSet {
   int size() const { return static_cast<int>(plane.size()); }
 };
 
-Sb decide_sb(const Luma& src, const RefSet& rs, int sb_col, int sb_row, int sb_size, int64_t lam);  // kernels_ref.cpp:306-323
-Map decide_frame(const Luma& src, const std::vector<const Luma*>& refs, int sb_size, int64_t lam);  // :253-262, 331-354
+Sb decide_sb(const Luma& src, const RefSet& rs, int sb_col, int sb_row, int sb_size, int64_t lam);  // kernels_ref.cpp:253-270
+Map decide_frame(const Luma& src, const std::vector<const Luma*>& refs, int sb_size, int64_t lam);  // kernels_ref.cpp:200-209, 282-301
 
-// One square's bottom-up decision (kernels_ref.cpp:178-251), exposed for the brute-force tests: the
+// One square's bottom-up decision (kernels_ref.cpp:124-198), exposed for the brute-force tests: the
 // cost it returns and the subtree it appends.
 int64_t decide_square(const Plane8& s, const RefSet& rs, int x, int y, int size, int64_t lam, Sb& out,
                       std::vector<Cands>* rec, int ox, int oy);
 
 // ---- row gate and the threaded engine ----------------------------------------------------------
-class RowGate {  // decision_map.h:87-108, decision_map.cpp:19-62
+class RowGate {  // decision_map.h:87-108, decision_map.cpp:19-61
  public:
   void publish(int row);
   void publish_one(int row);

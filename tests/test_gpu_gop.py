@@ -1,5 +1,5 @@
 """The HIP engine driven over a coding schedule (keyframe, hidden anchor, pyramid frames, B frames,
-a lone frame) the way Pipeline::code launches it (src/pipeline.cpp:124-146): each coded frame's
+a lone frame) the way Pipeline::code launches it (src/pipeline.cpp:125-177): each coded frame's
 ranked references with their display indices as keys (reused across frames), lambda of the frame's
 own qindex, the next coded frame prepared behind each launch. Every decision map must equal the
 oracle's CPU engine given the same references and lambda."""

@@ -1,4 +1,4 @@
-"""Pins the oracle's per-block choice and partition decision (kernels_ref.cpp:96-251 of CelV2, which
+"""Pins the oracle's per-block choice and partition decision (kernels_ref.cpp:41-198 of CelV2, which
 cannot be built here: it includes AVM's quant_common.h) by brute force over every partition tree of
 small squares, forced ties, invariants, and the known answers of CelV2's own engine tests."""
 import numpy as np
