@@ -479,6 +479,7 @@ int celv2hip_pruner_create(int device, int max_positions, unsigned seed, celv2hi
     check(hipSetDevice(device), "hipSetDevice");
     auto p = std::make_unique<celv2hip_pruner>();
     p->device = device;
+    check(cnn_scratch_init(device), "hipGetSymbolAddress pruner scratch");
     check(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipMalloc(&p->params, kParams * sizeof(float)), "hipMalloc params");
     check(hipMalloc(&p->pb, kParams * sizeof(uint16_t)), "hipMalloc params");
@@ -564,6 +565,28 @@ int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h
     cnn_infer(p->work.x, b, h, w, p->params, p->pb, p->work.z3, kMaxRefs, 0.5f, 0.f, nullptr, nullptr, p->stream);
     const size_t m = static_cast<size_t>(b) * h * w;
     check(hipMemcpyAsync(logits, p->work.z3, m * kOut * sizeof(float), hipMemcpyDeviceToHost, p->stream), "hipMemcpy logits");
+    check(hipStreamSynchronize(p->stream), "sync");
+  });
+}
+
+int celv2hip_pruner_forward_static(const celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits) {
+  if (!p || !feats || !logits || b <= 0 || h <= 0 || w <= 0) {
+    g_error = "pruner_forward_static: bad arguments";
+    return CELV2_ERR_ARG;
+  }
+  return guarded([&] {
+    const size_t m = static_cast<size_t>(b) * h * w;
+    if (m > static_cast<size_t>(kStaticPositions))
+      throw Failure(CELV2_ERR_SIZE, "pruner_forward_static: at most 16384 superblock positions (b x h x w) per call");
+    check(hipSetDevice(p->device), "hipSetDevice");
+    float* const s = cnn_scratch(p->device);
+    if (!s) throw Failure(CELV2_ERR_STATE, "pruner_forward_static: no scratch on this device");
+    float* const x = s;
+    float* const z = s + static_cast<size_t>(kStaticPositions) * kFeatures;
+    check(hipMemcpyAsync(x, feats, m * kFeatures * sizeof(float), hipMemcpyHostToDevice, p->stream), "hipMemcpy feats");
+    cnn_infer(x, b, h, w, p->params, p->pb, z, kMaxRefs, 0.5f, 0.f, nullptr, nullptr, p->stream);
+    check(hipGetLastError(), "pruner kernels");
+    check(hipMemcpyAsync(logits, z, m * kOut * sizeof(float), hipMemcpyDeviceToHost, p->stream), "hipMemcpy logits");
     check(hipStreamSynchronize(p->stream), "sync");
   });
 }

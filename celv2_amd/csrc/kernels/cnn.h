@@ -54,4 +54,12 @@ void cnn_limits(const float* logits, int m, int nrefs, float threshold, float ba
 void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
                float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s);
 
+// The static scratch of the no-allocation forward pass: features [kStaticPositions][kFeatures]
+// followed by logits [kStaticPositions][kOut] (1.3 MB), one copy per device. cnn_scratch_init takes
+// the current device's address once (call with `device` current); cnn_scratch returns it (null before).
+constexpr int kStaticPositions = 16384;
+constexpr int kScratchDevices = 64;
+hipError_t cnn_scratch_init(int device);
+float* cnn_scratch(int device);
+
 }  // namespace celv2hip

@@ -9,6 +9,7 @@
 // The backward pass reuses the GEMM with strides: dW = col^T dZ, dcol = dZ W^T (gathered back to
 // the grid), and reductions for the biases.
 #include <cmath>
+#include <mutex>
 
 #include "cnn.h"
 #include "common.h"
@@ -410,6 +411,34 @@ void cnn_infer(const float* x, int b, int h, int wd, const float* params, const 
   if (b <= 0 || h <= 0 || wd <= 0) return;
   const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band};
   k_infer<<<dim3((wd + kTileW - 1) / kTileW, (h + kTileH - 1) / kTileH, b), kInferThreads, 0, s>>>(a);
+}
+
+// Scratch of celv2hip_pruner_forward_static: the features and logits of up to kStaticPositions
+// positions in a device array of this file, so that entry point allocates nothing. Each device holds
+// its own copy until the process ends; its address is taken once per device (cnn_scratch_init, from
+// celv2hip_pruner_create with that device current) and the kernels receive that address, never the
+// array's name.
+static __device__ __align__(256) float scratch[static_cast<size_t>(kStaticPositions) * (kFeatures + kOut)];
+
+namespace {
+std::mutex g_scratch_mu;
+float* g_scratch[kScratchDevices] = {};
+}  // namespace
+
+hipError_t cnn_scratch_init(int device) {
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (rc == hipSuccess) g_scratch[device] = static_cast<float*>(p);
+  return rc;
+}
+
+float* cnn_scratch(int device) {
+  if (device < 0 || device >= kScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch[device];
 }
 
 void cnn_limits(const float* logits, int m, int nrefs, float threshold, float band, uint8_t* kmap,

@@ -60,6 +60,7 @@ def lib():
         L.celv2hip_pruner_get_params.argtypes = [P, P, I]
         L.celv2hip_pruner_set_params.argtypes = [P, P, I]
         L.celv2hip_pruner_forward.argtypes = [P, P, I, I, I, P]
+        L.celv2hip_pruner_forward_static.argtypes = [P, P, I, I, I, P]
         L.celv2hip_pruner_gradients.argtypes = [P, P, P, I, I, I, P, ctypes.POINTER(ctypes.c_float)]
         L.celv2hip_pruner_train_step.argtypes = [P, P, P, I, I, I, ctypes.c_float, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
         L.celv2hip_set_pruner.argtypes = [P, P, ctypes.c_float, ctypes.c_float]
@@ -387,6 +388,15 @@ class Pruner:
         b, h, w, _ = feats.shape
         out = np.zeros((b, h, w, 6), np.float32)
         if lib().celv2hip_pruner_forward(self.h, feats.ctypes.data, b, h, w, out.ctypes.data) != 0:
+            raise EngineError(last_error())
+        return out
+
+    def forward_static(self, feats):
+        """forward() through the entry point that allocates nothing (at most 16384 positions)."""
+        feats = np.ascontiguousarray(feats, dtype=np.float32)
+        b, h, w, _ = feats.shape
+        out = np.zeros((b, h, w, 6), np.float32)
+        if lib().celv2hip_pruner_forward_static(self.h, feats.ctypes.data, b, h, w, out.ctypes.data) != 0:
             raise EngineError(last_error())
         return out
 

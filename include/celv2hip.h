@@ -196,6 +196,13 @@ int celv2hip_pruner_get_params(celv2hip_pruner* p, float* out, int cap);
 int celv2hip_pruner_set_params(celv2hip_pruner* p, const float* in, int n);
 /* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 14 + i]: the 6 logits per superblock. */
 int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
+/* The same forward pass without any device allocation: features and logits are staged in a static
+ * device array of the library (one per device, taken at celv2hip_pruner_create), so a caller that
+ * cannot allocate (a serving loop, code around graph capture) may call it. At most 16384 positions
+ * (b * h * w) per call, else CELV2_ERR_SIZE and nothing runs. Calls that overlap on one device share
+ * that array and race; the array keeps the last call's contents. No counterpart in the reference: an
+ * entry point for callers that cannot allocate device memory. */
+int celv2hip_pruner_forward_static(const celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
 /* Mean binary cross-entropy against labels kstar (the references each superblock needs, 1..7) and its
  * gradient with respect to every parameter, without updating them. */
 int celv2hip_pruner_gradients(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w,
