@@ -493,6 +493,12 @@ void search_all(const Pyramid& cur, const Pyramid& ref, std::vector<Field>& lv) 
 
 // ---- per-block choice and partition DP (kernels_ref.cpp:26-198) --------------------------------------
 
+// ORC_PLANT (test builds only, tests/test_oracle_choice.py): a deliberate mistake in the per-block
+// choice below, 1..8 as listed there; 0, the default, is the rule itself.
+#ifndef ORC_PLANT
+#define ORC_PLANT 0
+#endif
+
 namespace {
 
 struct Pick {
@@ -515,7 +521,7 @@ Scores score_block(const Plane8& s, const RefSet& rs, int x, int y, int w, int h
   const int n = rs.size();
   if (with_intra)
     for (int m = 0; m < kModes; m++) {
-      const int64_t c = intra_satd(s, x, y, w, h, m) * 256 + lam * (mode_bits(m) + (n > 0 ? kIntraOnInter : 0));
+      const int64_t c = intra_satd(s, x, y, w, h, m) * 256 + lam * (mode_bits(m) + (n > 0 ? kIntraOnInter - (ORC_PLANT == 1) : 0));
       if (c < sc.intra.cost) sc.intra = Pick{c, m, 0, 0, 0};
     }
   int best = -1, runner = -1;
@@ -526,19 +532,22 @@ Scores score_block(const Plane8& s, const RefSet& rs, int x, int y, int w, int h
     for (int k = 0; k < count; k++) {
       if (cand_repeats(f.mvx.data(), f.mvy.data(), f.cols, f.rows, x, y, w, h, k)) continue;
       const int c = cand_cell(x, y, w, h, k, f.cols, f.rows);
-      const int64_t cost = inter_satd(s, *rs.plane[r], x, y, w, h, f.mvx[c], f.mvy[c]) * 256 + lam * (kNewmvBits + r);
-      if (cost < sc.per_ref[r].cost) sc.per_ref[r] = Pick{cost, kModes, f.mvx[c], f.mvy[c], r};
+      const int64_t cost = inter_satd(s, *rs.plane[r], x, y, w, h, f.mvx[c], f.mvy[c]) * 256 +
+                           lam * (kNewmvBits + (ORC_PLANT == 2 ? 0 : r) + (ORC_PLANT == 3));
+      if (cost < sc.per_ref[r].cost || (ORC_PLANT == 4 && cost == sc.per_ref[r].cost))
+        sc.per_ref[r] = Pick{cost, kModes, f.mvx[c], f.mvy[c], r};
     }
     if (best < 0 || sc.per_ref[r].cost < sc.per_ref[best].cost) best = r;
   }
   for (int r = 0; r < n; r++)
-    if (r != best && (runner < 0 || sc.per_ref[r].cost < sc.per_ref[runner].cost)) runner = r;
+    if (r != best && (runner < 0 || sc.per_ref[r].cost < sc.per_ref[runner].cost || (ORC_PLANT == 5 && sc.per_ref[r].cost == sc.per_ref[runner].cost)))
+      runner = r;
   if (best >= 0) sc.inter = sc.per_ref[best];
   sc.nrefs = n;
   if (runner >= 0 && pair_allowed(best, runner, n)) {
     const Pick &p = sc.per_ref[best], &q = sc.per_ref[runner];
     const int64_t c = compound_satd(s, *rs.plane[best], p.mvx, p.mvy, *rs.plane[runner], q.mvx, q.mvy, x, y, w, h) * 256 +
-                      lam * pair_bits(best, runner);
+                      lam * (pair_bits(best, runner) + (ORC_PLANT == 6 ? runner : 0));
     if (c < sc.inter.cost) sc.inter = Pick{c, kModes, p.mvx, p.mvy, best, runner, q.mvx, q.mvy};
   }
   return sc;
@@ -557,8 +566,8 @@ Pick choose(const Plane8& s, const RefSet& rs, int x, int y, int w, int h, int64
     c.inter.mv2_col = static_cast<int16_t>(2 * sc.inter.mvx2), c.inter.mv2_row = static_cast<int16_t>(2 * sc.inter.mvy2);
     for (int r = 0; r < sc.nrefs; r++)
       if (sc.per_ref[r].cost != INT64_MAX) {
-        c.ref_cost[r] = static_cast<uint32_t>(std::min<int64_t>(sc.per_ref[r].cost >> 8, UINT32_MAX - 1));
-        c.ref_mv_col[r] = static_cast<int16_t>(2 * sc.per_ref[r].mvx);
+        c.ref_cost[r] = static_cast<uint32_t>(std::min<int64_t>(sc.per_ref[r].cost >> (ORC_PLANT == 7 ? 9 : 8), UINT32_MAX - 1));
+        c.ref_mv_col[r] = static_cast<int16_t>((ORC_PLANT == 8 ? 1 : 2) * sc.per_ref[r].mvx);
         c.ref_mv_row[r] = static_cast<int16_t>(2 * sc.per_ref[r].mvy);
       }
   }

@@ -7,9 +7,15 @@
 //   src/gpu/cpu_engine.cpp, src/gpu/decision_map.cpp, src/io/y4m.cpp.
 // Pinning (see tests/test_oracle_*.py): the header-only primitives are compared with the
 // reference's own headers compiled from /root/reference (oracle/_ref, recipe oracle/Makefile) and
-// with fixtures under tests/golden/; the per-block choice and the partition DP, which live in
-// kernels_ref.cpp (unbuildable here: it includes AVM's quant_common.h), are pinned by brute force
-// over every partition tree of small squares, forced-tie cases and invariants.
+// with fixtures under tests/golden/. The per-block choice (kernels_ref.cpp:41-104), which lives in a
+// file that cannot be built here (it includes AVM's quant_common.h), is pinned by rebuilding every
+// candidate record of several superblocks from those pinned primitives, by known answers (identical
+// references, a source averaging two references, flat pictures, tied runners-up) and by deliberate
+// mistakes (ORC_PLANT in oracle.cpp) that each make one of those checks fail; the partition DP by
+// brute force over every partition tree of small squares, forced ties and invariants. lambda_q8 is
+// an input here: its parity with CelV2 is unpinned (AVM's quantiser table is absent).
+// Speed paths (direct addressing where a block lies inside its plane, two-row SSE2 SADs, separable
+// quarter-pel planes) compute the same integers as the clamped per-sample forms they sit beside.
 #pragma once
 #include <atomic>
 #include <condition_variable>
