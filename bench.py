@@ -3,14 +3,15 @@
 Workload (the reference's engine_bench method, tools/engine_bench.cpp:44-57): a 1920x1080 10-bit
 4:2:0 synthetic clip of 32 frames (pan, 12 moving objects, grain, fade to black at frames 24-27);
 frame i >= 7 is decided against the 7 source frames before it, nearest first, keys = frame indices,
-qindex 130, with the next frame prepare()d behind it. A step is one frame's decision. The 10-bit
-frames are resident in HBM before timing (the engine rounds them to the 8 bits the decisions read
-on the device); decision records stay in HBM (device-resident mode).
+qindex 130, with the next frame prepare()d behind it. A step is one frame's decision, timed as
+engine_bench times it: launch -> prepare -> collect through the C++ engine (celv2::HipEngine, CelV2's
+DecisionEngine), the frame's DecisionMap on the host when collect returns. The 10-bit frames are
+resident in HBM before timing (the engine rounds them to the 8 bits the decisions read on the device).
 
-Printed: one JSON line with frames/s, the dominant kernel's roofline fraction, stage times measured
-with HIP events on the engine's stream over the timed steps, the end-to-end rate through the C++
-HipEngine with host buffers (PCIe and DecisionMap conversion included), and the CPU engine (the
-oracle) timed on this host's cores on one frame, whose decisions are also checked against the GPU's.
+Printed: one JSON line with that rate (`value`), the rate with the records left in HBM
+(`kernel_only`), the rate with host frames uploaded by launch / prepare (`host_frames_fps`), the
+dominant kernel's roofline fraction, stage times measured with HIP events on the engine's stream, and
+the CPU engine (the oracle) timed on this host's cores, whose decisions are checked against the GPU's.
 
     python bench.py [--gpus 1] [--steps 50] [--warmup 5] [--no-cpu]
 """
@@ -132,6 +133,18 @@ def bench_batched(NR, lam, steps=20, warmup=3, nf=16):
         rates[name] = (steps * len(engines) / wall, steps * sbs / wall, wall / steps * 1e3)
         maps[name] = [e.fetch() for e in engines]
         log(f"mixed workload, {name}: {rates[name][0]:.1f} frames/s, {rates[name][1]:.0f} superblocks/s, {rates[name][2]:.3f} ms per step")
+    # The one-group batched form again with stage events on (first engine: the batch's stages; every
+    # engine counts its own centres): the motion search's pixel-SAD rate over the whole batch.
+    for e in engines:
+        e.set_timing(True)
+        e.stats_reset()
+    for s in range(warmup + steps, warmup + 2 * steps):
+        step_batch(s, [range(6)])
+    st = [e.stats() for e in engines]
+    for e in engines:
+        e.set_timing(False)
+    nb = max(1, st[0][1]["launches"])
+    roof = me_roofline(sum(c["me_centres"] for _, c in st) / nb, st[0][0]["me"] / nb)
     same = all(compare_maps(a, b, e.sb_cols) is None for f in maps if f != "plain"
                for a, b, e in zip(maps["plain"], maps[f], engines))
     best = max((f for f in rates if f != "plain"), key=lambda f: rates[f][0])
@@ -147,7 +160,8 @@ def bench_batched(NR, lam, steps=20, warmup=3, nf=16):
             "ms_per_step": round(rates[best][2], 4),
             "by_grouping": {f: {"frames_per_s": round(r[0], 2), "ms_per_step": round(r[2], 4)} for f, r in rates.items()},
             "plain_frames_per_s": round(rates["plain"][0], 2), "plain_superblocks_per_s": round(rates["plain"][1]),
-            "speedup": round(rates[best][0] / rates["plain"][0], 3), "decisions_equal_plain": same}
+            "speedup": round(rates[best][0] / rates["plain"][0], 3), "decisions_equal_plain": same,
+            "roofline_batched": roof}
 
 
 def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
@@ -185,6 +199,7 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
     wall = time.perf_counter() - t0
     fps = steps / wall
     per = {k: round(v, 4) for k, v in stage_times(E, step, warmup + steps, steps).items()}
+    roof = me_roofline(stage_times.counts["me_centres"], per["me"])
     # Untimed: inter candidates counted, regret, limits and fallbacks on every decidable frame.
     E.set_counting(True)
     E.stats_reset()
@@ -212,7 +227,7 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
             "cost_regret": round(regret, 6), "cost_regret_unit": "sum(J_learned - J_plain) / sum(J_plain) over every decidable frame (engine J = SATD*256 + lambda*bits)",
             "bd_rate": "pending (needs the full encoder and AVM)",
             "mean_reference_limit": round(float(ks.mean()), 3), "fallback_rate": round(c2["pruner_fallbacks"] / ks.size, 5),
-            "threshold": threshold, "band": band, "stage_ms_per_frame": per,
+            "threshold": threshold, "band": band, "stage_ms_per_frame": per, "roofline": roof,
             "inter_candidates_per_frame": round(inter_l), "compound_blocks_per_frame": round(c2["compound_blocks"] / nd),
             "network": f"conv3x3 11-32, conv3x3 32-32, conv1x1 32-6, BF16 on MFMA, trained on {cfg['train_superblocks']} superblocks of synthetic clips (celv2_amd/pruner/train.py)"}
 
@@ -229,7 +244,16 @@ def stage_times(E, step, first, steps):
     n = max(1, cnt["launches"])
     per = {k: v / n for k, v in ms.items() if k != "intra_prepare"}
     per["intra"] = ms["intra_prepare"] / n
+    stage_times.counts = {k: v / n for k, v in cnt.items() if k != "launches"}
     return per
+
+
+def me_roofline(centres_per_frame, me_ms):
+    """The motion search's pixel-SAD rate (289 x 64 per centre searched, as the kernel counts them)
+    as a fraction of the v_sad_u8 peak."""
+    achieved = centres_per_frame * 289 * 64 / (me_ms / 1e3) / 1e12
+    return {"bound": "valu", "kernel": "k_me_level", "achieved": round(achieved, 3), "peak": round(SAD_PEAK_TPX, 2),
+            "unit": "T pixel-SAD/s (v_sad_u8)", "frac": round(achieved / SAD_PEAK_TPX, 4), "ms_per_frame": round(me_ms, 4)}
 
 
 def _timed(step, engines, prepare, steps, warmup):
@@ -475,7 +499,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the qindex sweep of the learned form")
     ap.add_argument("--no-cel", action="store_true", help="skip the 4K cel workload (learned + batched)")
     ap.add_argument("--no-gop", action="store_true", help="skip the clip in the frame loop's coding order")
-    ap.add_argument("--e2e-steps", type=int, default=12)
+    ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU engine timed on all threads")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
     args = ap.parse_args()
@@ -573,23 +597,63 @@ def main():
     # + compound evaluations) per frame.
     cands = sbs * 420 * 13 + inter_cands + cnt["compound_blocks"] / n
 
-    # End to end through the C++ HipEngine (host frames, upload, DecisionMap); never the metric.
-    e2e = None
-    if args.e2e_steps > 0:
-        HE = engine.HipEngine(W, H, full_depth=True)
-        frames = decidable[: args.e2e_steps + 2]
-        ts = []
-        for j, i in enumerate(frames):
-            refs = [i - k for k in range(1, NR + 1)]
-            t = time.perf_counter()
-            HE.launch(y8[i], [y8[r] for r in refs], lam=lam, keys=refs, src_key=i, src16=y16[i],
-                      refs16=[y16[r] for r in refs], bit_depth=10)
-            if j + 1 < len(frames):
-                HE.prepare(y8[frames[j + 1]], frames[j + 1], y16=y16[frames[j + 1]], bit_depth=10)
-            HE.collect(export=False)  # the DecisionMap as an encoder receives it, not copied back out
-            ts.append(time.perf_counter() - t)
-        e2e = round(1.0 / (sum(ts[2:]) / len(ts[2:])), 2)
-        HE.close()
+    # The headline: engine_bench's step through the C++ engine (celv2::HipEngine): launch -> prepare ->
+    # collect, the frame's DecisionMap on the host when collect returns; frames resident in HBM.
+    HE = engine.HipEngine(W, H)
+
+    def bstep(s):
+        i = decidable[s % len(decidable)]
+        nxt = decidable[(s + 1) % len(decidable)]
+        refs = [i - k for k in range(1, NR + 1)]
+        HE.launch_device(lum[i], [lum[r] for r in refs], lam=lam, keys=refs, src_key=i)
+        HE.prepare_device(lum[nxt], nxt)
+        HE.collect(export=False)  # the DecisionMap as an encoder receives it
+        return i
+
+    HE.prepare_device(lum[decidable[0]], decidable[0])
+    for s in range(args.warmup):
+        bstep(s)
+    HE.sync()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, args.warmup + args.steps):
+        blast = bstep(s)
+    HE.sync()
+    wall_b = time.perf_counter() - t0
+    fps_b = args.steps / wall_b
+    # The last frame's DecisionMap exported: it must be the device-resident run's decisions of that frame.
+    refs_b = [blast - k for k in range(1, NR + 1)]
+    HE.launch_device(lum[blast], [lum[r] for r in refs_b], lam=lam, keys=refs_b, src_key=blast)
+    bmap, _ = HE.collect()
+    E.launch(lum[blast], [lum[r] for r in refs_b], lam, keys=refs_b, src_key=blast)
+    boundary_equal = compare_maps(bmap, E.fetch(), E.sb_cols) is None
+    HE.close()
+    log(f"boundary (launch -> prepare -> collect into the host DecisionMap, frames in HBM): {fps_b:.1f} frames/s; "
+        f"kernel only {fps:.1f}; DecisionMap equal to the resident run's: {boundary_equal}")
+
+    # The same steps with host frames (engine_bench's own inputs: 10-bit planes uploaded by launch and
+    # prepare; references reused by key).
+    HH = engine.HipEngine(W, H, full_depth=True)
+
+    def hstep(s):
+        i = decidable[s % len(decidable)]
+        nxt = decidable[(s + 1) % len(decidable)]
+        refs = [i - k for k in range(1, NR + 1)]
+        HH.launch(y8[i], [y8[r] for r in refs], lam=lam, keys=refs, src_key=i, src16=y16[i],
+                  refs16=[y16[r] for r in refs], bit_depth=10)
+        HH.prepare(y8[nxt], nxt, y16=y16[nxt], bit_depth=10)
+        HH.collect(export=False)
+
+    HH.prepare(y8[decidable[0]], decidable[0], y16=y16[decidable[0]], bit_depth=10)
+    for s in range(args.warmup):
+        hstep(s)
+    HH.sync()
+    t0 = time.perf_counter()
+    for s in range(args.warmup, args.warmup + args.steps):
+        hstep(s)
+    HH.sync()
+    fps_h = args.steps / (time.perf_counter() - t0)
+    HH.close()
+    log(f"host frames (10-bit uploads by launch / prepare, DecisionMap on the host): {fps_h:.1f} frames/s")
 
     batched = None if args.no_batched else bench_batched(NR, lam)
     learned = None
@@ -605,53 +669,67 @@ def main():
     cpu = None
     parity = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        refs = [last - k for k in range(1, NR + 1)]
-        dt, diff = cpu_baseline(y8, last, refs, lam, gpu_map, threads)
-        parity = diff is None
-        if diff:
-            log("PARITY FAILURE vs the CPU engine:", diff)
-        # One thread (as engine_bench runs it) on a bounded sample: the frame's top two superblock
-        # rows, scaled to the frame's rows.
-        crop = [np.ascontiguousarray(f[:128]) for f in y8]
-        dt1, _ = cpu_baseline(crop, last, refs, lam, None, 1)
-        one = dt1 * E.sb_rows / 2
-        cpu = {"value": round(1.0 / dt, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"1 frame (frame {last}) of the same 1080p 10-bit 7-reference workload, oracle CpuEngine with {threads} threads: {dt:.1f} s",
-               "one_thread": {"value": round(1.0 / one, 5), "unit": "frames/s", "cores": 1,
-                              "sample": f"rows 0-127 of frame {last} (2 of {E.sb_rows} superblock rows), 1 thread: {dt1:.1f} s, scaled x{E.sb_rows / 2:g}"}}
-        log(f"cpu engine: {dt:.2f}s for one frame ({threads} threads), 1 thread {dt1:.1f}s for 2 SB rows; parity {'ok' if parity else 'FAILED'}")
+        # This host's CPUs as nproc reports them (the process's affinity); the engine's workers are
+        # capped at 16, the CPU share a one-GPU box gives a process.
+        nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        threads = max(1, min(16, nproc))
+        k0 = decidable.index(last)
+        frames_cpu = [decidable[(k0 + k) % len(decidable)] for k in range(max(1, args.cpu_frames))]  # `last` first
+        dts = []
+        for f in frames_cpu:
+            refs = [f - k for k in range(1, NR + 1)]
+            want_gpu = gpu_map if f == last else None
+            dt, diff = cpu_baseline(y8, f, refs, lam, want_gpu, threads)
+            dts.append(dt)
+            if f == last:
+                parity = diff is None
+                if diff:
+                    log("PARITY FAILURE vs the CPU engine:", diff)
+        dt_all = sum(dts) / len(dts)
+        # One thread (as engine_bench runs the CPU engine) on one whole frame.
+        dt1, _ = cpu_baseline(y8, last, [last - k for k in range(1, NR + 1)], lam, None, 1)
+        cpu = {"value": round(1.0 / dt_all, 4), "unit": "frames/s", "cores": threads, "nproc": nproc, "kind": "port",
+               "sample": f"{len(dts)} frames ({', '.join(map(str, frames_cpu))}) of the same 1080p 10-bit 7-reference workload, "
+                         f"the oracle's CpuEngine on {threads} threads: {sum(dts):.1f} s",
+               "one_thread": {"value": round(1.0 / dt1, 5), "unit": "frames/s", "cores": 1,
+                              "sample": f"frame {last}, whole, 1 thread: {dt1:.1f} s"}}
+        log(f"cpu engine: {dt_all:.2f} s per frame on {threads} threads (nproc {nproc}), 1 thread {dt1:.1f} s; parity {'ok' if parity else 'FAILED'}")
     for b in dev:
         b.free()
 
     out = {
         "metric": METRIC,
-        "value": round(fps, 2),
+        "value": round(fps_b, 2),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "ms_per_step": round(wall_b / args.steps * 1e3, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
-        "data": "synthetic (seeded 10-bit generator, frames resident in HBM)",
+        "data": "synthetic (seeded 10-bit generator, frames resident in HBM; DecisionMap on the host)",
         "config": {"workload": "1920x1080 10-bit 4:2:0, 32-frame clip, 7 references (previous source frames), qindex 130, "
-                               "plain HIP engine, engine_bench method (frame i decided against i-1..i-7, next frame prepared)",
+                               "plain HIP engine, engine_bench method through celv2::HipEngine (frame i decided against i-1..i-7, "
+                               "next frame prepared, launch -> prepare -> collect into the host DecisionMap)",
                    "width": W, "height": H, "refs": NR, "qindex": args.qindex, "lambda_q8": lam, "form": "plain"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_vs_cpu_engine": parity,
+        "kernel_only": {"frames_per_s": round(fps, 2), "ms_per_step": round(wall / args.steps * 1e3, 4),
+                        "what": "the same steps with the records left in HBM (no download, no DecisionMap)"},
+        "host_frames_fps": round(fps_h, 2),
+        "boundary_map_equals_resident": boundary_equal,
         "stage_ms_per_frame": {k: round(v, 4) for k, v in per.items()},
         "counts_per_frame": {k: round(v / max(1, n), 1) for k, v in cnt.items() if k != "launches"},
         "scored_candidates_per_frame": round(cands),
-        "scored_candidates_per_s": round(cands * fps),
-        "me_positions_per_s": round(cnt["me_centres"] / n * 289 * fps),
+        "scored_candidates_per_s": round(cands * fps_b),
+        "me_positions_per_s": round(cnt["me_centres"] / n * 289 * fps_b),
         "int_ops_per_frame": total_ops,
         "all_kernels_tops": round(total_ops * fps / 1e12, 3),
-        "engine_path_fps_host_buffers": e2e,
-        "forms": {"plain": {"frames_per_s": round(fps, 2), "scored_candidates_per_s": round(cands * fps)},
+        "forms": {"plain": {"frames_per_s": round(fps_b, 2), "kernel_only_frames_per_s": round(fps, 2),
+                            "scored_candidates_per_s": round(cands * fps_b), "roofline": me_roofline(cnt["me_centres"] / n, per["me"])},
                   "batched": batched, "learned": learned, "learned_batched_cel_4k": cel, "gop_order": gop_form},
     }
     print(json.dumps(out), flush=True)

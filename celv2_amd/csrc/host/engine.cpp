@@ -57,6 +57,10 @@ int guarded(F&& f) {
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Row groups of a launch with one row stream: each group's records are downloaded while the next
+// groups compute.
+constexpr int kCopyGroups = 4;
+
 // A picture's pyramid in one allocation: level 0 (a reference's followed by its 15 phase planes,
 // phase_bytes apart) then levels 1..4, every plane bordered.
 struct Pyramid {
@@ -132,6 +136,16 @@ struct celv2hip_engine {
   // of levels 4..1 (only level 0's refinement, inter and compound scores read them): sub_start is
   // recorded on the launch stream after the import, sub_done on this stream after the phase planes.
   hipStream_t aux = nullptr;
+  // Record downloads (rec_dev -> rec_host) run on `aux` too, late in the launch, long after its phase
+  // planes: each group of rows is copied once its finalize is done, while the launch stream goes on
+  // with the next rows (a stream of their own would put every engine's launch stream on the same of
+  // the process's 4 hardware queues). group_done[g] is recorded on aux after group g's copy, and row
+  // r of the last launch waits for group_done[row_group[r]]; copy_done
+  // after a launch's last copy (the next launch waits for it before finalize overwrites rec_dev).
+  // rows_ev: one per group, recorded on the launch stream after the group's finalize.
+  std::vector<hipEvent_t> rows_ev;
+  hipEvent_t copy_done = nullptr;
+  bool copy_pending = false;
   hipEvent_t sub_start = nullptr, sub_done = nullptr;
   hipEvent_t order_ev = nullptr;  // orders this engine's stream around batched launches
   // Recorded before the last launch's level-0 motion search: prepare() starts the next source's
@@ -163,7 +177,8 @@ struct celv2hip_engine {
   void* rec_dev = nullptr;   // celv2_sb per superblock
   void* rec_host = nullptr;  // pinned
   std::vector<int> order;
-  std::vector<hipEvent_t> row_done;
+  std::vector<hipEvent_t> group_done;  // one per download group (at most one per row)
+  std::vector<int> row_group;          // the group that downloads row r in the last launch
   // Per-launch stage events in a ring (kRing launches): start, references built, motion search,
   // inter scores, finalize, end; and per prepare(): intra scores start / end.
   static constexpr int kRing = 512;
@@ -218,7 +233,9 @@ struct celv2hip_engine {
     if (rec_host) (void)hipHostFree(rec_host);
     if (counters) (void)hipFree(counters);
     if (counters_host) (void)hipHostFree(counters_host);
-    for (hipEvent_t e : row_done) (void)hipEventDestroy(e);
+    for (hipEvent_t e : group_done) (void)hipEventDestroy(e);
+    for (hipEvent_t e : rows_ev) (void)hipEventDestroy(e);
+    if (copy_done) (void)hipEventDestroy(copy_done);
     for (auto& a : ring)
       for (hipEvent_t e : a) (void)hipEventDestroy(e);
     for (auto& a : pring)
@@ -325,6 +342,15 @@ void upload_tables_once(int device) {
   });
 }
 
+// The checks of an entry point that only waits on events or reads host memory: no current device needed.
+int validate_host(celv2hip_engine* e) {
+  if (!e) {
+    g_error = "null engine";
+    return CELV2_ERR_ARG;
+  }
+  return CELV2_OK;
+}
+
 int validate(celv2hip_engine* e) {
   if (!e) {
     g_error = "null engine";
@@ -380,6 +406,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     check(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking), "hipStreamCreate");
     check(hipStreamCreateWithFlags(&e->aux, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipEventCreateWithFlags(&e->copy_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->sub_start, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->sub_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming), "hipEventCreate");
@@ -400,8 +427,11 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     check(hipHostMalloc(&e->rec_host, e->nsb() * sizeof(celv2_sb), hipHostMallocDefault), "hipHostMalloc records");
     check(hipMalloc(&e->counters, kCounterBytes), "hipMalloc counters");
     check(hipHostMalloc(reinterpret_cast<void**>(&e->counters_host), kCounterBytes, hipHostMallocDefault), "hipHostMalloc counters");
-    e->row_done.resize(e->sb_rows);
-    for (hipEvent_t& ev : e->row_done) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+    e->group_done.resize(e->sb_rows);
+    e->row_group.assign(e->sb_rows, 0);
+    e->rows_ev.resize(e->sb_rows);
+    for (hipEvent_t& ev : e->rows_ev) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+    for (hipEvent_t& ev : e->group_done) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
     e->ring.resize(celv2hip_engine::kRing);
     e->pring.resize(celv2hip_engine::kRing);
     for (auto& a : e->ring)
@@ -828,6 +858,10 @@ void launch_batch(const LaunchIn* in, int n) {
   run_intra(intra_now, S);
   for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[1], S), "hipEventRecord");
 
+  // One engine with one row stream whose records go to the host: level 0 of the motion search runs in
+  // row groups interleaved with the inter scores, finalize and downloads of the groups above (below).
+  const bool wavefront = n == 1 && !e0->resident && std::max(1, std::min(e0->row_streams, e0->sb_rows)) == 1;
+  Batch<MeLevelArgs> level0;  // the level-0 search of the wavefront form, launched by row groups
   // Motion search, top level first, every reference of every engine per launch.
   for (int l = kLevels - 1; l >= 0; l--) {
     Batch<MeLevelArgs> b;
@@ -860,6 +894,10 @@ void launch_batch(const LaunchIn* in, int n) {
         in[i].e->deep_pending = true;
       }
       if (phases_pending) check(hipStreamWaitEvent(S, e0->sub_done, 0), "hipStreamWaitEvent");  // level 0 refines on them
+      if (wavefront) {
+        level0 = b;
+        continue;
+      }
     }
     launch_me_level(b, S);
     check(hipGetLastError(), "me kernel");
@@ -923,9 +961,24 @@ void launch_batch(const LaunchIn* in, int n) {
       for (int t = 0; t < streams; t++)
         if (t * run + k < e->sb_rows) e->order.push_back(t * run + k);
   }
-  const bool per_row = n == 1 && !e0->resident && std::max(1, std::min(e0->row_streams, e0->sb_rows)) > 1;
-  if (!per_row) {
-    // All rows in one launch per kernel.
+  // Records of rows r0 .. r0 + nrows - 1 of engine e to the host once the launch stream has finished
+  // them (group g's event), on the engine's aux stream, group_done[g] behind the copy.
+  auto queue_copy = [&](celv2hip_engine* e, int g, int r0, int nrows) {
+    const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
+    check(hipEventRecord(e->rows_ev[g], S), "hipEventRecord");
+    check(hipStreamWaitEvent(e->aux, e->rows_ev[g], 0), "hipStreamWaitEvent");
+    const size_t off = static_cast<size_t>(r0) * row_bytes;
+    check(hipMemcpyAsync(static_cast<char*>(e->rec_host) + off, static_cast<char*>(e->rec_dev) + off, row_bytes * nrows,
+                         hipMemcpyDeviceToHost, e->aux), "hipMemcpyAsync records");
+    check(hipEventRecord(e->group_done[g], e->aux), "hipEventRecord");
+    for (int r = r0; r < r0 + nrows; r++) e->row_group[r] = g;
+  };
+  // The previous launch's downloads read rec_dev: this launch's finalize writes it after them.
+  for (int i = 0; i < n; i++)
+    if (in[i].e->copy_pending) check(hipStreamWaitEvent(S, in[i].e->copy_done, 0), "hipStreamWaitEvent");
+  const bool grouped = n == 1 && !e0->resident;
+  if (!grouped) {
+    // All rows in one launch per kernel (batches, and resident mode).
     Batch<InterArgs> bi;
     for (int i = 0; i < n; i++) {
       if (!in[i].nrefs) continue;
@@ -948,41 +1001,68 @@ void launch_batch(const LaunchIn* in, int n) {
     for (int i = 0; i < n; i++) {
       celv2hip_engine* e = in[i].e;
       if (e->timing) check(hipEventRecord(e->ev[4], S), "hipEventRecord");
-      if (!e->resident) {
-        const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
-        check(hipMemcpyAsync(e->rec_host, e->rec_dev, row_bytes * e->sb_rows, hipMemcpyDeviceToHost, S), "hipMemcpyAsync records");
-        for (int r = 0; r < e->sb_rows; r++) check(hipEventRecord(e->row_done[r], S), "hipEventRecord");
-      }
+      if (!e->resident) queue_copy(e, 0, 0, e->sb_rows);
     }
   } else {
-    // One engine, rows interleaved: each row's inter scores, finalize and record copy in turn.
+    // One engine: its rows in groups, each group's inter scores and finalize in turn, each group's
+    // download on the aux stream while the next group computes. One row stream (the wavefront form):
+    // groups of about a quarter of the rows, level 0 of the search by the same groups one ahead, so
+    // each download overlaps the next group's search; several row streams: one row at a time in
+    // row-stream order after the whole search.
     celv2hip_engine* e = e0;
-    const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
-    for (int r : e->order) {
+    std::vector<std::pair<int, int>> groups;  // (first row, rows)
+    if (!wavefront) {
+      for (int r : e->order) groups.push_back({r, 1});
+    } else {
+      const int rows = std::max(1, (e->sb_rows + kCopyGroups - 1) / kCopyGroups);
+      for (int r0 = 0; r0 < e->sb_rows; r0 += rows) groups.push_back({r0, std::min(rows, e->sb_rows - r0)});
+    }
+    // The level-0 search of group g's block rows. A superblock row's inter scores read the level-0
+    // field one cell beyond its own 8 block rows, so group g's rows are scored once group g + 1's
+    // search is done: search(0), search(1), rows(0), search(2), rows(1), ..., rows(last).
+    auto search_rows = [&](size_t g) {
+      if (!wavefront || g >= groups.size() || level0.n == 0) return;
+      Batch<MeLevelArgs> b0;
+      MeLevelArgs a = level0.item[0];
+      a.row0 = 8 * groups[g].first;
+      a.nrun = std::min(8 * groups[g].second, e->frows[0] - a.row0);
+      if (a.nrun <= 0) return;
+      add_me_level(b0, a);
+      launch_me_level(b0, S);
+      check(hipGetLastError(), "me kernel");
+    };
+    search_rows(0);
+    for (size_t g = 0; g < groups.size(); g++) {
+      search_rows(g + 1);
+      if (wavefront && g + 1 == groups.size() && e->timing) check(hipEventRecord(e->ev[2], S), "hipEventRecord");
+      const auto [r0, nr] = groups[g];
       if (in[0].nrefs) {
         Batch<InterArgs> bi;
         InterArgs ia = inter_args(0);
-        ia.row0 = r, ia.nrows = 1;
+        ia.row0 = r0, ia.nrows = nr;
         add_inter(bi, ia);
         launch_inter(bi, S);
       }
       Batch<FinalizeArgs> bf;
       FinalizeArgs fa = finalize_args(0);
-      fa.row0 = r, fa.nrows = 1;
+      fa.row0 = r0, fa.nrows = nr;
       add_finalize(bf, fa);
       launch_finalize(bf, S);
       check(hipGetLastError(), "row kernels");
-      const size_t off = static_cast<size_t>(r) * row_bytes;
-      check(hipMemcpyAsync(static_cast<char*>(e->rec_host) + off, static_cast<char*>(e->rec_dev) + off, row_bytes,
-                           hipMemcpyDeviceToHost, S), "hipMemcpyAsync row");
-      check(hipEventRecord(e->row_done[r], S), "hipEventRecord");
+      queue_copy(e, static_cast<int>(g), r0, nr);
     }
     if (e->timing) check(hipEventRecord(e->ev[3], S), "hipEventRecord");
     if (e->timing) check(hipEventRecord(e->ev[4], S), "hipEventRecord");
   }
   for (int i = 0; i < n; i++) {
     celv2hip_engine* e = in[i].e;
-    if (e->timing) check(hipEventRecord(e->ev[5], S), "hipEventRecord");
+    if (e->resident) continue;
+    check(hipEventRecord(e->copy_done, e->aux), "hipEventRecord");
+    e->copy_pending = true;
+  }
+  for (int i = 0; i < n; i++) {
+    celv2hip_engine* e = in[i].e;
+    if (e->timing) check(hipEventRecord(e->ev[5], e->resident ? S : e->aux), "hipEventRecord");
     e->release(*cur[i], S);
     e->nrefs_last = in[i].nrefs;
     e->limits_last = use_kmap[i] != 0;
@@ -1034,24 +1114,24 @@ int celv2hip_row_order(const celv2hip_engine* e, int* rows, int cap) {
 }
 
 int celv2hip_wait_row(celv2hip_engine* e, int row, celv2_sb* out) {
-  if (int rc = validate(e)) return rc;
+  if (int rc = validate_host(e)) return rc;
   return guarded([&] {
     if (!e->launched) throw Failure(CELV2_ERR_STATE, "HipEngine: no launch to collect");
     if (e->resident) throw Failure(CELV2_ERR_STATE, "HipEngine: resident mode keeps results on the device");
     if (row < 0 || row >= e->sb_rows) throw Failure(CELV2_ERR_ARG, "HipEngine: no such row");
-    check(hipEventSynchronize(e->row_done[row]), "row wait");
+    check(hipEventSynchronize(e->group_done[e->row_group[row]]), "row wait");
     if (out) std::memcpy(out, static_cast<char*>(e->rec_host) + static_cast<size_t>(row) * e->sb_cols * sizeof(celv2_sb), e->sb_cols * sizeof(celv2_sb));
   });
 }
 
 int celv2hip_row_records(celv2hip_engine* e, int row, const celv2_sb** out) {
-  if (int rc = validate(e)) return rc;
+  if (int rc = validate_host(e)) return rc;
   return guarded([&] {
     if (!out) throw Failure(CELV2_ERR_ARG, "HipEngine: no output pointer");
     if (!e->launched) throw Failure(CELV2_ERR_STATE, "HipEngine: no launch to collect");
     if (e->resident) throw Failure(CELV2_ERR_STATE, "HipEngine: resident mode keeps results on the device");
     if (row < 0 || row >= e->sb_rows) throw Failure(CELV2_ERR_ARG, "HipEngine: no such row");
-    check(hipEventSynchronize(e->row_done[row]), "row wait");
+    check(hipEventSynchronize(e->group_done[e->row_group[row]]), "row wait");
     *out = reinterpret_cast<const celv2_sb*>(static_cast<const char*>(e->rec_host) + static_cast<size_t>(row) * e->sb_cols * sizeof(celv2_sb));
   });
 }

@@ -92,6 +92,39 @@ int celv2_engine_launch(void* p, const uint8_t* src8, const uint16_t* src16, int
   }
 }
 
+// Frames in device memory (celv2_luma with on_device set), through HipEngine::launch_device.
+int celv2_engine_launch_device(void* p, const celv2_luma* src, const celv2_luma* refs, const long long* keys, int nrefs,
+                               int qindex, long long lam, long long src_key) {
+  Handle* h = static_cast<Handle*>(p);
+  try {
+    if (!src || nrefs < 0 || (nrefs > 0 && !refs)) throw std::invalid_argument("launch_device: bad arguments");
+    std::vector<celv2_luma> r(refs, refs + nrefs);
+    std::vector<long long> k;
+    if (keys) k.assign(keys, keys + nrefs);
+    h->e->set_lambda(lam);
+    h->e->launch_device(*src, r, qindex, h->dm, k, src_key);
+    return 0;
+  } catch (const std::invalid_argument& ex) {
+    g_err = ex.what();
+    return 1;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 2;
+  }
+}
+
+int celv2_engine_prepare_device(void* p, const celv2_luma* next, long long key) {
+  Handle* h = static_cast<Handle*>(p);
+  try {
+    if (!next) throw std::invalid_argument("prepare_device: no frame");
+    h->e->prepare_device(*next, key);
+    return 0;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 2;
+  }
+}
+
 int celv2_engine_prepare(void* p, const uint8_t* y8, const uint16_t* y16, int bit_depth, long long key) {
   Handle* h = static_cast<Handle*>(p);
   try {
@@ -122,6 +155,16 @@ int celv2_engine_collect(void* p, celv2_sb* out, int* rows_ready) {
     return 0;
   } catch (const std::exception& ex) {
     if (rows_ready) *rows_ready = gate.ready_rows();
+    g_err = ex.what();
+    return 2;
+  }
+}
+
+int celv2_engine_sync(void* p) {
+  try {
+    static_cast<Handle*>(p)->e->synchronize();
+    return 0;
+  } catch (const std::exception& ex) {
     g_err = ex.what();
     return 2;
   }

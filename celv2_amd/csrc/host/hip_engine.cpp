@@ -5,7 +5,10 @@
 #include <atomic>
 #include <cmath>
 #include <cstddef>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <iterator>
 #include <stdexcept>
 #include <string>
 #include <mutex>
@@ -74,15 +77,110 @@ constexpr bool kSameLayout =
     offsetof(BlockCandidates, ref_mv_col) == offsetof(celv2_cands, ref_mv_col) && offsetof(BlockCandidates, ref_mv_row) == offsetof(celv2_cands, ref_mv_row) &&
     BlockCandidates::kRefs == CELV2_MAX_REFS;
 
+namespace {
+
+// A forward iterator over records that yields each one as the DecisionMap type with the same layout
+// (byte copy), so a vector is built from records in one pass: assign() constructs each element from
+// its record, where resize() + memcpy would write every element twice.
+template <class To, class From>
+struct AsIter {
+  using iterator_category = std::forward_iterator_tag;
+  using value_type = To;
+  using difference_type = std::ptrdiff_t;
+  using pointer = const To*;
+  using reference = To;
+  const From* p;
+  To operator*() const {
+    To t;
+    std::memcpy(static_cast<void*>(&t), p, sizeof(To));
+    return t;
+  }
+  AsIter& operator++() {
+    ++p;
+    return *this;
+  }
+  AsIter operator++(int) {
+    AsIter o = *this;
+    ++p;
+    return o;
+  }
+  bool operator==(const AsIter& o) const { return p == o.p; }
+  bool operator!=(const AsIter& o) const { return p != o.p; }
+};
+
+template <class To, class From>
+void assign_records(std::vector<To>& v, const From* first, int n) {
+  static_assert(sizeof(To) == sizeof(From) && std::is_trivially_copyable_v<To>, "record layouts differ");
+  v.assign(AsIter<To, From>{first}, AsIter<To, From>{first + n});
+}
+
+}  // namespace
+
+// Worker threads that run one job at a time beside the calling thread: run(job) starts job on every
+// worker, runs it on the caller, and returns once every worker has finished it (so no worker touches
+// a job, or what it refers to, after run returns). The job divides its work itself.
+class CollectPool {
+ public:
+  explicit CollectPool(int workers) {
+    for (int k = 0; k < workers; k++) threads_.emplace_back([this] { loop(); });
+  }
+  ~CollectPool() {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      stop_ = true;
+    }
+    start_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  void run(const std::function<void()>& job) {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      job_ = &job;
+      finished_ = 0;
+      gen_++;
+    }
+    start_.notify_all();
+    job();
+    std::unique_lock<std::mutex> lock(mu_);
+    done_.wait(lock, [&] { return finished_ == static_cast<int>(threads_.size()); });
+    job_ = nullptr;
+  }
+
+ private:
+  void loop() {
+    long long seen = 0;
+    for (;;) {
+      const std::function<void()>* job;
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        start_.wait(lock, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        job = job_;
+      }
+      (*job)();
+      {
+        std::lock_guard<std::mutex> lock(mu_);
+        finished_++;
+      }
+      done_.notify_all();
+    }
+  }
+  std::vector<std::thread> threads_;
+  std::mutex mu_;
+  std::condition_variable start_, done_;
+  const std::function<void()>* job_ = nullptr;
+  long long gen_ = 0;
+  int finished_ = 0;
+  bool stop_ = false;
+};
+
 void sb_record_to_decision(const celv2_sb& rec, SbDecision& d) {
   if constexpr (kSameLayout) {
     // inter / intra bytes of the records are 0 or 1 and a mode index: valid values of bool and IntraMode.
-    d.tree.resize(rec.ntree);
-    std::memcpy(static_cast<void*>(d.tree.data()), rec.tree, static_cast<size_t>(rec.ntree));
-    d.blocks.resize(rec.nblocks);
-    std::memcpy(static_cast<void*>(d.blocks.data()), rec.blocks, static_cast<size_t>(rec.nblocks) * sizeof(celv2_block));
-    d.cands.resize(rec.ncands);
-    std::memcpy(static_cast<void*>(d.cands.data()), rec.cands, static_cast<size_t>(rec.ncands) * sizeof(celv2_cands));
+    assign_records(d.tree, reinterpret_cast<const Part*>(rec.tree), rec.ntree);
+    assign_records(d.blocks, rec.blocks, rec.nblocks);
+    assign_records(d.cands, rec.cands, rec.ncands);
     return;
   }
   d.tree.resize(rec.ntree);
@@ -132,9 +230,17 @@ HipEngine::HipEngine(int width, int height, int sb_size, int device)
   name_ = name;
 }
 
-HipEngine::~HipEngine() { celv2hip_destroy(h_); }
+HipEngine::~HipEngine() {
+  pool_.reset();
+  celv2hip_destroy(h_);
+}
 
 void HipEngine::set_row_streams(int n) { celv2hip_set_row_streams(h_, n); }
+
+void HipEngine::synchronize() {
+  const int rc = celv2hip_sync(h_);
+  if (rc != CELV2_OK) raise(rc, "HipEngine::synchronize");
+}
 
 void HipEngine::prepare(const Frame420& next, long long key) {
   if (key < 0 || next.width != width_ || next.height != height_) return;
@@ -153,9 +259,30 @@ void HipEngine::launch(const Frame420& src, const std::vector<const Frame420*>& 
   const celv2_luma s = luma_of(src, full_depth_);
   std::vector<celv2_luma> r;
   for (const Frame420* f : refs) r.push_back(luma_of(*f, full_depth_));
+  launch_luma(s, r.data(), static_cast<int>(r.size()), qindex, dm, ref_keys, src_key);
+}
+
+void HipEngine::launch_device(const celv2_luma& src, const std::vector<celv2_luma>& refs, int qindex, DecisionMap& dm,
+                              const std::vector<long long>& ref_keys, long long src_key) {
+  if (refs.size() > 7) throw std::runtime_error("HipEngine: too many references");
+  if (!ref_keys.empty() && ref_keys.size() != refs.size()) throw std::invalid_argument("HipEngine: one key per reference");
+  if (!src.on_device) throw std::invalid_argument("HipEngine::launch_device: the source is not in device memory");
+  for (const celv2_luma& l : refs)
+    if (!l.on_device) throw std::invalid_argument("HipEngine::launch_device: a reference is not in device memory");
+  launch_luma(src, refs.data(), static_cast<int>(refs.size()), qindex, dm, ref_keys, src_key);
+}
+
+void HipEngine::prepare_device(const celv2_luma& next, long long key) {
+  if (key < 0) return;
+  if (!next.on_device) throw std::invalid_argument("HipEngine::prepare_device: the frame is not in device memory");
+  const int rc = celv2hip_prepare(h_, &next, key);
+  if (rc != CELV2_OK) raise(rc, "HipEngine::prepare");
+}
+
+void HipEngine::launch_luma(const celv2_luma& src, const celv2_luma* refs, int nrefs, int qindex, DecisionMap& dm,
+                            const std::vector<long long>& ref_keys, long long src_key) {
   const int64_t lam = lambda_ >= 0 ? lambda_ : lambda_q8(qindex);
-  const int rc = celv2hip_launch(h_, &s, src_key, r.data(), ref_keys.empty() ? nullptr : ref_keys.data(),
-                                 static_cast<int>(refs.size()), lam);
+  const int rc = celv2hip_launch(h_, &src, src_key, refs, ref_keys.empty() ? nullptr : ref_keys.data(), nrefs, lam);
   if (rc != CELV2_OK) raise(rc, "HipEngine::launch");
   dm.sb_size = 64, dm.sb_cols = sb_cols_, dm.sb_rows = sb_rows_;
   dm.sbs.assign(static_cast<size_t>(sb_cols_) * sb_rows_, SbDecision{});
@@ -170,21 +297,35 @@ void HipEngine::collect(DecisionMap& dm, RowGate& gate) {
     celv2hip_sync(h_);
     std::rethrow_exception(e);
   }
-  // Rows are taken in queue order by up to 4 threads (this one included): each waits for its row,
-  // converts it straight from the engine's pinned buffer and publishes it. The first failure stops
-  // the others taking rows, fails the gate and is rethrown once every thread has finished.
+  // Rows are converted in chunks of up to 8 superblocks, taken in queue order by up to 8 threads (this
+  // one included): each waits for its row's records, converts them straight from the engine's pinned
+  // buffer, and the thread that converts a row's last chunk publishes the row. The first failure stops
+  // the others taking chunks, fails the gate and is rethrown once every thread has finished.
+  constexpr int kChunk = 8, kThreads = 8;
+  struct Chunk {
+    int row, sb0, sb1;
+  };
+  std::vector<Chunk> chunks;
+  std::vector<std::atomic<int>> left(static_cast<size_t>(sb_rows_));
+  for (int i = 0; i < n; i++) {
+    int k = 0;
+    for (int c = 0; c < sb_cols_; c += kChunk, k++) chunks.push_back({order[i], c, std::min(sb_cols_, c + kChunk)});
+    left[order[i]] = k;
+  }
   std::atomic<int> next{0};
   std::atomic<bool> failed{false};
   std::exception_ptr err;
   std::mutex err_mu;
-  auto work = [&] {
-    for (int i; !failed.load() && (i = next.fetch_add(1)) < n;) {
+  const int total = static_cast<int>(chunks.size());
+  const std::function<void()> work = [&] {
+    for (int i; !failed.load() && (i = next.fetch_add(1)) < total;) {
       try {
+        const Chunk& ch = chunks[i];
         const celv2_sb* row = nullptr;
-        const int rc = celv2hip_row_records(h_, order[i], &row);
+        const int rc = celv2hip_row_records(h_, ch.row, &row);
         if (rc != CELV2_OK) raise(rc, "HipEngine::collect");
-        for (int c = 0; c < sb_cols_; c++) sb_record_to_decision(row[c], dm.sbs[static_cast<size_t>(order[i]) * sb_cols_ + c]);
-        gate.publish_one(order[i]);
+        for (int c = ch.sb0; c < ch.sb1; c++) sb_record_to_decision(row[c], dm.sbs[static_cast<size_t>(ch.row) * sb_cols_ + c]);
+        if (left[ch.row].fetch_sub(1) == 1) gate.publish_one(ch.row);
       } catch (...) {
         std::lock_guard<std::mutex> lock(err_mu);
         if (!err) err = std::current_exception();
@@ -192,12 +333,11 @@ void HipEngine::collect(DecisionMap& dm, RowGate& gate) {
       }
     }
   };
-  const unsigned hw = std::thread::hardware_concurrency();
-  const int threads = std::max(1, std::min({4, n, static_cast<int>(hw ? hw : 1)}));
-  std::vector<std::thread> pool;
-  for (int k = 1; k < threads; k++) pool.emplace_back(work);
-  work();
-  for (auto& t : pool) t.join();
+  if (!pool_) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    pool_ = std::make_unique<CollectPool>(std::max(0, std::min(kThreads, static_cast<int>(hw ? hw : 1)) - 1));
+  }
+  pool_->run(work);
   if (err) {
     gate.fail(err);
     celv2hip_sync(h_);  // returns with no work pending

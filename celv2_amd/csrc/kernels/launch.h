@@ -46,6 +46,9 @@ struct MeLevelArgs {
   const uint8_t* kmap;
   int sb_cols, sb_rows, level;
   uint32_t* cost_out[kMaxRefs];  // the searched best cost (SAD + distance penalty) per block (may be null)
+  // The block rows this launch searches: [row0, row0 + nrun) (nrun 0: every row). rows stays the
+  // level's row count (neighbour clamps, the coarser level's rows).
+  int row0, nrun;
   float inv_nrefs, inv_cols;     // 1 / nrefs, 1 / cols (set by add_me_level)
 };
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a);

@@ -14,6 +14,8 @@
 //      columns of three 16x16x16 Hadamard products on the matrix cores, accumulated onto the block's
 //      own transform (hadamard.h); the key (cost << 4 | order) makes the centre win ties as in
 //      me_refine.
+#include <stdexcept>
+
 #include "common.h"
 #include "hadamard.h"
 #include "launch.h"
@@ -324,8 +326,8 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
   const int bi = batch_item(batch, g);
   const MeLevelArgs& a = batch.item[bi];
   const int u = g - batch.first[bi], grp = div_small(u, a.nrefs, a.inv_nrefs), r = u - grp * a.nrefs;
-  const int block = 4 * grp + wave;
-  if (block >= a.rows * a.cols) return;  // whole wave
+  const int block = a.row0 * a.cols + 4 * grp + wave;
+  if (block >= (a.row0 + a.nrun) * a.cols) return;  // whole wave
   const int by = div_small(block, a.cols, a.inv_cols);
   search_block(a, BlockPos{r, block - by * a.cols, by}, lane, wins[wave][0], wins[wave][1]);
 }
@@ -333,9 +335,11 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
 }  // namespace
 
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a) {
-  const int units = a.nrefs * ((a.rows * a.cols + 3) / 4);
-  if (units <= 0) return;
   MeLevelArgs x = a;
+  if (x.nrun <= 0) x.row0 = 0, x.nrun = a.rows;
+  if (x.row0 < 0 || x.row0 + x.nrun > a.rows) throw std::logic_error("motion search rows outside the level");
+  const int units = a.nrefs * ((x.nrun * a.cols + 3) / 4);
+  if (units <= 0) return;
   x.inv_nrefs = 1.f / static_cast<float>(a.nrefs), x.inv_cols = 1.f / static_cast<float>(a.cols);
   b.add(x, units);
 }

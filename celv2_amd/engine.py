@@ -83,8 +83,11 @@ def lib():
         L.celv2_engine_destroy.argtypes = [P]
         L.celv2_engine_launch.argtypes = [P, P, P, I, P, P, ctypes.POINTER(LL), I, I, LL, LL]
         L.celv2_engine_prepare.argtypes = [P, P, P, I, LL]
+        L.celv2_engine_launch_device.argtypes = [P, ctypes.POINTER(Luma), ctypes.POINTER(Luma), ctypes.POINTER(LL), I, I, LL, LL]
+        L.celv2_engine_prepare_device.argtypes = [P, ctypes.POINTER(Luma), LL]
         L.celv2_engine_collect.argtypes = [P, P, ctypes.POINTER(I)]
         L.celv2_engine_description.argtypes = [P, ctypes.c_char_p, I]
+        L.celv2_engine_sync.argtypes = [P]
         L.celv2_lambda_q8.argtypes = [I]
         L.celv2_lambda_q8.restype = ctypes.c_int64
         if L.celv2_sizeof_sb() != SB_DTYPE.itemsize:
@@ -178,6 +181,20 @@ class HipEngine:
         if rc != 0:
             raise EngineError(lib().celv2_engine_last_error().decode())
 
+    def launch_device(self, src, refs, qindex=120, lam=-1, keys=None, src_key=-1):
+        """launch() for frames in device memory: src / refs are RawEngine.luma(dev8=... or dev16=...)."""
+        arr = (Luma * max(1, len(refs)))(*refs)
+        k = (LL * max(1, len(refs)))(*keys) if keys is not None else None
+        rc = lib().celv2_engine_launch_device(self.h, ctypes.byref(src), arr, k, len(refs), qindex, lam, src_key)
+        if rc == 1:
+            raise ValueError(lib().celv2_engine_last_error().decode())
+        if rc != 0:
+            raise EngineError(lib().celv2_engine_last_error().decode())
+
+    def prepare_device(self, luma, key):
+        if lib().celv2_engine_prepare_device(self.h, ctypes.byref(luma), key) != 0:
+            raise EngineError(lib().celv2_engine_last_error().decode())
+
     def collect(self, export=True):
         """(records, ready rows); export False: the DecisionMap stays in the engine (timing), records None."""
         out = np.zeros(self.sb_cols * self.sb_rows, dtype=SB_DTYPE) if export else None
@@ -186,6 +203,10 @@ class HipEngine:
         if rc != 0:
             raise EngineError(lib().celv2_engine_last_error().decode())
         return out, ready.value
+
+    def sync(self):
+        if lib().celv2_engine_sync(self.h) != 0:
+            raise EngineError(lib().celv2_engine_last_error().decode())
 
     def decide(self, src8, refs8, lam, **kw):
         self.launch(src8, refs8, lam=lam, **kw)

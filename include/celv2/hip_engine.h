@@ -7,6 +7,7 @@
 // include/celv2hip.h.
 #pragma once
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -20,8 +21,11 @@
 
 struct celv2hip_engine;
 struct celv2_sb;
+struct celv2_luma;
 
 namespace celv2 {
+
+class CollectPool;  // collect()'s worker threads (hip_engine.cpp)
 
 class HipEngine : public DecisionEngine {
  public:
@@ -38,10 +42,19 @@ class HipEngine : public DecisionEngine {
   void launch(const Frame420& src, const std::vector<const Frame420*>& refs, int qindex, DecisionMap& dm,
               const std::vector<long long>& ref_keys = {}, long long src_key = -1) override;
   void prepare(const Frame420& next, long long key) override;
+  // launch / prepare for frames already in this engine's device memory (celv2_luma with on_device
+  // set: the padded luma, 8-bit or deeper, include/celv2hip.h): no copy from the host, the same
+  // decisions and the same collect. No counterpart in CelV2's interface: for callers that keep their
+  // frames in HBM.
+  void launch_device(const celv2_luma& src, const std::vector<celv2_luma>& refs, int qindex, DecisionMap& dm,
+                     const std::vector<long long>& ref_keys = {}, long long src_key = -1);
+  void prepare_device(const celv2_luma& next, long long key);
   // Publishes rows in the order they were queued (publish_one); on failure gate.fail + rethrow.
   void collect(DecisionMap& dm, RowGate& gate) override;
   void set_row_streams(int n) override;
 
+  // Waits for all work queued on the engine's streams (prepare()'s included); for timing.
+  void synchronize();
   const std::string& device_name() const { return name_; }
   std::string description() const { return "decisions on HIP (" + name_ + ")"; }
   // Frames above 8 bits: send the full-depth luma (hy) and round it on the device instead of
@@ -51,8 +64,11 @@ class HipEngine : public DecisionEngine {
   void set_lambda(int64_t lam) { lambda_ = lam; }
 
  private:
+  void launch_luma(const celv2_luma& src, const celv2_luma* refs, int nrefs, int qindex, DecisionMap& dm,
+                   const std::vector<long long>& ref_keys, long long src_key);
   int width_, height_, sb_cols_, sb_rows_;
   celv2hip_engine* h_ = nullptr;
+  std::unique_ptr<CollectPool> pool_;  // created by the first collect
   std::string name_;
   bool full_depth_ = false;
   int64_t lambda_ = -1;

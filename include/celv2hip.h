@@ -266,8 +266,15 @@ int celv2_engine_launch(void* engine, const uint8_t* src8, const uint16_t* src16
                         const uint8_t* const* refs8, const uint16_t* const* refs16, const long long* keys, int nrefs,
                         int qindex, long long lam, long long src_key);
 int celv2_engine_prepare(void* engine, const uint8_t* y8, const uint16_t* y16, int bit_depth, long long key);
+/* The same launch / prepare for frames already in device memory (celv2_luma with on_device set):
+ * HipEngine::launch_device / prepare_device. */
+int celv2_engine_launch_device(void* engine, const celv2_luma* src, const celv2_luma* refs, const long long* keys,
+                               int nrefs, int qindex, long long lam, long long src_key);
+int celv2_engine_prepare_device(void* engine, const celv2_luma* next, long long key);
 int celv2_engine_collect(void* engine, celv2_sb* out, int* rows_ready);
 int celv2_engine_description(void* engine, char* buf, int cap);
+/* Waits for all work queued on the engine's streams (timing). */
+int celv2_engine_sync(void* engine);
 int celv2_sizeof_sb(void);
 
 /* Digest of the native sources the library was built from (the build's own check against a stale

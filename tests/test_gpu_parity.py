@@ -168,3 +168,31 @@ def test_reference_limits_decide_as_fewer_references(gpu):
     e.launch(gpu.RawEngine.luma(y8[f]), [gpu.RawEngine.luma(y8[r]) for r in refs], lam, keys=refs, src_key=f)
     assert compare_maps(e.collect(), maps[7], e.sb_cols) is None
     e.close()
+
+
+def test_device_frames_through_the_cpp_engine(gpu):
+    """celv2::HipEngine::launch_device / prepare_device: 10-bit frames resident in HBM, three frames in a
+    row (each next frame prepared, references reused by key), every DecisionMap equal to the oracle's;
+    a host frame given as a device one is refused."""
+    w, h = 352, 288
+    _, y16, y8 = clip(1, 23, 6, w, h)
+    lam = gpu.lambda_q8(125)
+    bufs = [gpu.DeviceBuffer(f) for f in y16]
+    lum = [gpu.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in bufs]
+    e = gpu.HipEngine(w, h)
+    try:
+        e.prepare_device(lum[3], 3)
+        for i in (3, 4, 5):
+            refs = [i - 1, i - 2, i - 3]
+            e.launch_device(lum[i], [lum[r] for r in refs], lam=lam, keys=refs, src_key=i)
+            if i + 1 < 6:
+                e.prepare_device(lum[i + 1], i + 1)
+            got, ready = e.collect()
+            assert ready == e.sb_rows
+            check(got, ol.decide_frame(y8[i], [y8[r] for r in refs], lam, w, h), e.sb_cols)
+        with pytest.raises(ValueError):
+            e.launch_device(gpu.RawEngine.luma(y8[5]), [lum[4]], lam=lam)
+    finally:
+        e.close()
+        for b in bufs:
+            b.free()
