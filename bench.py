@@ -500,6 +500,7 @@ def main():
     ap.add_argument("--no-cel", action="store_true", help="skip the 4K cel workload (learned + batched)")
     ap.add_argument("--no-gop", action="store_true", help="skip the clip in the frame loop's coding order")
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU engine timed on all threads")
+    ap.add_argument("--row-groups", type=int, default=0, help="the launch's row groups (0: tuned by timing)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
     args = ap.parse_args()
@@ -610,6 +611,17 @@ def main():
         HE.collect(export=False)  # the DecisionMap as an encoder receives it
         return i
 
+    # The launch's row groups: tuned on the first decidable frame (steady-state launch -> prepare ->
+    # collect at each candidate count), or as given.
+    if args.row_groups > 0:
+        HE.set_row_groups(args.row_groups)
+        tuning = {"chosen": args.row_groups, "tuned": False}
+    else:
+        f0 = decidable[0]
+        r0 = [f0 - k for k in range(1, NR + 1)]
+        g, timed = HE.autotune_device(lum[f0], [lum[r] for r in r0], lam=lam, keys=r0, src_key=f0, frames=5)
+        tuning = {"chosen": g, "tuned": True, "median_ms_per_frame": {str(k): round(v, 3) for k, v in sorted(timed.items())}}
+        log(f"row groups: {g} (median ms per frame by count: {tuning['median_ms_per_frame']})")
     HE.prepare_device(lum[decidable[0]], decidable[0])
     for s in range(args.warmup):
         bstep(s)
@@ -633,6 +645,7 @@ def main():
     # The same steps with host frames (engine_bench's own inputs: 10-bit planes uploaded by launch and
     # prepare; references reused by key).
     HH = engine.HipEngine(W, H, full_depth=True)
+    HH.set_row_groups(tuning["chosen"])
 
     def hstep(s):
         i = decidable[s % len(decidable)]
@@ -721,6 +734,7 @@ def main():
                         "what": "the same steps with the records left in HBM (no download, no DecisionMap)"},
         "host_frames_fps": round(fps_h, 2),
         "boundary_map_equals_resident": boundary_equal,
+        "row_groups": tuning,
         "stage_ms_per_frame": {k: round(v, 4) for k, v in per.items()},
         "counts_per_frame": {k: round(v / max(1, n), 1) for k, v in cnt.items() if k != "launches"},
         "scored_candidates_per_frame": round(cands),

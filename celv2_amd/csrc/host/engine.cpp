@@ -57,9 +57,10 @@ int guarded(F&& f) {
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// Row groups of a launch with one row stream: each group's records are downloaded while the next
-// groups compute.
-constexpr int kCopyGroups = 4;
+// Row groups of a launch with one row stream (the wavefront form) unless set otherwise
+// (celv2hip_set_row_groups, chosen by the autotuner): each group's records are downloaded while the
+// next groups compute.
+constexpr int kDefaultRowGroups = 4;
 
 // A picture's pyramid in one allocation: level 0 (a reference's followed by its 15 phase planes,
 // phase_bytes apart) then levels 1..4, every plane bordered.
@@ -124,6 +125,10 @@ struct SrcSlot {
   // waits for the other's last work on the slot before touching it.
   hipEvent_t side_done = nullptr, main_done = nullptr;
   bool side_pending = false, main_pending = false;
+  // prepare()'s pyramid done (before its intra scores): a launch of the wavefront form reads only the
+  // pyramid on the launch stream (its finalize, on the side stream, follows the intra scores there).
+  hipEvent_t pyr_done = nullptr;
+  bool pyr_pending = false;
 };
 
 }  // namespace
@@ -144,6 +149,7 @@ struct celv2hip_engine {
   // after a launch's last copy (the next launch waits for it before finalize overwrites rec_dev).
   // rows_ev: one per group, recorded on the launch stream after the group's finalize.
   std::vector<hipEvent_t> rows_ev;
+  std::vector<hipEvent_t> inter_ev;  // wavefront form: group g's inter scores done (launch stream)
   hipEvent_t copy_done = nullptr;
   bool copy_pending = false;
   hipEvent_t sub_start = nullptr, sub_done = nullptr;
@@ -155,6 +161,7 @@ struct celv2hip_engine {
   bool deep_pending = false;
   std::string name;
   int row_streams = 1;
+  int row_groups = kDefaultRowGroups;  // the wavefront form's row groups
   bool resident = false;
   bool counting = false;  // count non-repeated inter candidates (costs the inter kernel LDS atomics)
   SrcSlot srcs[2];
@@ -214,6 +221,7 @@ struct celv2hip_engine {
       if (s.intra) (void)hipFree(s.intra);
       if (s.side_done) (void)hipEventDestroy(s.side_done);
       if (s.main_done) (void)hipEventDestroy(s.main_done);
+      if (s.pyr_done) (void)hipEventDestroy(s.pyr_done);
       free_up(s.up);
     }
     for (auto& s : slots) {
@@ -235,6 +243,7 @@ struct celv2hip_engine {
     if (counters_host) (void)hipHostFree(counters_host);
     for (hipEvent_t e : group_done) (void)hipEventDestroy(e);
     for (hipEvent_t e : rows_ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : inter_ev) (void)hipEventDestroy(e);
     if (copy_done) (void)hipEventDestroy(copy_done);
     for (auto& a : ring)
       for (hipEvent_t e : a) (void)hipEventDestroy(e);
@@ -414,6 +423,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     for (SrcSlot& s : e->srcs) {
       check(hipEventCreateWithFlags(&s.side_done, hipEventDisableTiming), "hipEventCreate");
       check(hipEventCreateWithFlags(&s.main_done, hipEventDisableTiming), "hipEventCreate");
+      check(hipEventCreateWithFlags(&s.pyr_done, hipEventDisableTiming), "hipEventCreate");
     }
     int w = e->pw, h = e->ph;
     for (int l = 0; l < kLevels; l++) {
@@ -431,6 +441,8 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     e->row_group.assign(e->sb_rows, 0);
     e->rows_ev.resize(e->sb_rows);
     for (hipEvent_t& ev : e->rows_ev) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+    e->inter_ev.resize(e->sb_rows);
+    for (hipEvent_t& ev : e->inter_ev) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
     for (hipEvent_t& ev : e->group_done) check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
     e->ring.resize(celv2hip_engine::kRing);
     e->pring.resize(celv2hip_engine::kRing);
@@ -461,6 +473,18 @@ int celv2hip_set_row_streams(celv2hip_engine* e, int n) {
   e->row_streams = n;
   return CELV2_OK;
 }
+
+int celv2hip_set_row_groups(celv2hip_engine* e, int groups) {
+  if (!e) return CELV2_ERR_ARG;
+  if (groups < 0 || groups > 64) {
+    g_error = "row groups are 1..64 (0: the default)";
+    return CELV2_ERR_ARG;
+  }
+  e->row_groups = groups == 0 ? kDefaultRowGroups : groups;
+  return CELV2_OK;
+}
+
+int celv2hip_row_groups(const celv2hip_engine* e) { return e ? e->row_groups : -CELV2_ERR_ARG; }
 
 int celv2hip_set_resident(celv2hip_engine* e, int on) {
   if (!e) return CELV2_ERR_ARG;
@@ -766,6 +790,7 @@ int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key) 
     s.key = -1;
     e->claim(s, e->side);
     run_builds({e->build_job(*next, s.p, s.up, false, e->side)}, e->side);
+    check(hipEventRecord(s.pyr_done, e->side), "hipEventRecord"), s.pyr_pending = true;
     if (e->deep_pending) check(hipStreamWaitEvent(e->side, e->deep_ev, 0), "hipStreamWaitEvent");
     run_intra({{e, &s}}, e->side);
     e->release(s, e->side);
@@ -809,6 +834,10 @@ void launch_batch(const LaunchIn* in, int n) {
   std::vector<std::vector<RefSlot*>> used(n);
   std::vector<char> use_kmap(n);
   for (int i = 0; i < n; i++) use_kmap[i] = in[i].e->kmap_on;
+  // One engine with one row stream whose records go to the host (the wavefront form): level 0 of the
+  // motion search runs in row groups interleaved with the inter scores of the groups above them; each
+  // group's finalize runs on the side stream, its record download on the aux stream.
+  const bool wavefront = n == 1 && !e0->resident && std::max(1, std::min(e0->row_streams, e0->sb_rows)) == 1;
   for (int i = 0; i < n; i++) {
     const LaunchIn& x = in[i];
     celv2hip_engine* e = x.e;
@@ -818,7 +847,14 @@ void launch_batch(const LaunchIn* in, int n) {
     int use = -1;
     for (int k = 0; k < 2; k++)
       if (x.src_key >= 0 && e->srcs[k].key == x.src_key) use = k;
-    e->claim(e->srcs[use < 0 ? 1 - e->cur_set : use], S);
+    SrcSlot& claimed = e->srcs[use < 0 ? 1 - e->cur_set : use];
+    if (wavefront && use >= 0) {
+      // A prepared source: the launch stream reads its pyramid; finalize, on the side stream, comes
+      // after its intra scores there.
+      if (claimed.pyr_pending) check(hipStreamWaitEvent(S, claimed.pyr_done, 0), "hipStreamWaitEvent");
+    } else {
+      e->claim(claimed, S);
+    }
     if (use < 0) {
       use = 1 - e->cur_set;
       SrcSlot& s = e->srcs[use];
@@ -858,9 +894,6 @@ void launch_batch(const LaunchIn* in, int n) {
   run_intra(intra_now, S);
   for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[1], S), "hipEventRecord");
 
-  // One engine with one row stream whose records go to the host: level 0 of the motion search runs in
-  // row groups interleaved with the inter scores, finalize and downloads of the groups above (below).
-  const bool wavefront = n == 1 && !e0->resident && std::max(1, std::min(e0->row_streams, e0->sb_rows)) == 1;
   Batch<MeLevelArgs> level0;  // the level-0 search of the wavefront form, launched by row groups
   // Motion search, top level first, every reference of every engine per launch.
   for (int l = kLevels - 1; l >= 0; l--) {
@@ -963,11 +996,13 @@ void launch_batch(const LaunchIn* in, int n) {
   }
   // Records of rows r0 .. r0 + nrows - 1 of engine e to the host once the launch stream has finished
   // them (group g's event), on the engine's aux stream, group_done[g] behind the copy.
-  auto queue_copy = [&](celv2hip_engine* e, int g, int r0, int nrows) {
+  auto queue_copy = [&](celv2hip_engine* e, int g, int r0, int nrows, hipStream_t done_on) {
     const size_t row_bytes = static_cast<size_t>(e->sb_cols) * sizeof(celv2_sb);
-    check(hipEventRecord(e->rows_ev[g], S), "hipEventRecord");
+    check(hipEventRecord(e->rows_ev[g], done_on), "hipEventRecord");
     check(hipStreamWaitEvent(e->aux, e->rows_ev[g], 0), "hipStreamWaitEvent");
     const size_t off = static_cast<size_t>(r0) * row_bytes;
+    // The copy engine, not a copy kernel: a kernel writing to host memory beside the inter scores
+    // slowed them by 60-100 % (measured), the copy engine's transfers barely.
     check(hipMemcpyAsync(static_cast<char*>(e->rec_host) + off, static_cast<char*>(e->rec_dev) + off, row_bytes * nrows,
                          hipMemcpyDeviceToHost, e->aux), "hipMemcpyAsync records");
     check(hipEventRecord(e->group_done[g], e->aux), "hipEventRecord");
@@ -1001,7 +1036,7 @@ void launch_batch(const LaunchIn* in, int n) {
     for (int i = 0; i < n; i++) {
       celv2hip_engine* e = in[i].e;
       if (e->timing) check(hipEventRecord(e->ev[4], S), "hipEventRecord");
-      if (!e->resident) queue_copy(e, 0, 0, e->sb_rows);
+      if (!e->resident) queue_copy(e, 0, 0, e->sb_rows, S);
     }
   } else {
     // One engine: its rows in groups, each group's inter scores and finalize in turn, each group's
@@ -1014,7 +1049,7 @@ void launch_batch(const LaunchIn* in, int n) {
     if (!wavefront) {
       for (int r : e->order) groups.push_back({r, 1});
     } else {
-      const int rows = std::max(1, (e->sb_rows + kCopyGroups - 1) / kCopyGroups);
+      const int rows = std::max(1, (e->sb_rows + e->row_groups - 1) / e->row_groups);
       for (int r0 = 0; r0 < e->sb_rows; r0 += rows) groups.push_back({r0, std::min(rows, e->sb_rows - r0)});
     }
     // The level-0 search of group g's block rows. A superblock row's inter scores read the level-0
@@ -1043,13 +1078,19 @@ void launch_batch(const LaunchIn* in, int n) {
         add_inter(bi, ia);
         launch_inter(bi, S);
       }
+      // Wavefront form: the group's finalize on the side stream once its inter scores are done.
+      const hipStream_t F = wavefront ? e->side : S;
+      if (wavefront) {
+        check(hipEventRecord(e->inter_ev[g], S), "hipEventRecord");
+        check(hipStreamWaitEvent(F, e->inter_ev[g], 0), "hipStreamWaitEvent");
+      }
       Batch<FinalizeArgs> bf;
       FinalizeArgs fa = finalize_args(0);
       fa.row0 = r0, fa.nrows = nr;
       add_finalize(bf, fa);
-      launch_finalize(bf, S);
+      launch_finalize(bf, F);
       check(hipGetLastError(), "row kernels");
-      queue_copy(e, static_cast<int>(g), r0, nr);
+      queue_copy(e, static_cast<int>(g), r0, nr, F);
     }
     if (e->timing) check(hipEventRecord(e->ev[3], S), "hipEventRecord");
     if (e->timing) check(hipEventRecord(e->ev[4], S), "hipEventRecord");

@@ -160,6 +160,50 @@ int celv2_engine_collect(void* p, celv2_sb* out, int* rows_ready) {
   }
 }
 
+int celv2_engine_set_row_groups(void* p, int groups) {
+  try {
+    static_cast<Handle*>(p)->e->set_row_groups(groups);
+    return 0;
+  } catch (const std::invalid_argument& ex) {
+    g_err = ex.what();
+    return 1;
+  }
+}
+
+int celv2_engine_row_groups(void* p) { return static_cast<Handle*>(p)->e->row_groups(); }
+
+// HipEngine::autotune_device; writes the chosen count, then (row groups, median ms) per candidate timed
+// (none when the count came from the process's earlier tuning) up to cap, and their number.
+int celv2_engine_autotune_device(void* p, const celv2_luma* src, const celv2_luma* refs, const long long* keys, int nrefs,
+                                 int qindex, long long lam, long long src_key, int frames, int force, int* groups,
+                                 int* cand_groups, double* cand_ms, int cap, int* ncand) {
+  Handle* h = static_cast<Handle*>(p);
+  try {
+    if (!src || nrefs < 0 || (nrefs > 0 && !refs)) throw std::invalid_argument("autotune_device: bad arguments");
+    std::vector<celv2_luma> r(refs, refs + nrefs);
+    std::vector<long long> k;
+    if (keys) k.assign(keys, keys + nrefs);
+    h->e->set_lambda(lam);
+    const celv2::HipEngine::Tuning t = h->e->autotune_device(*src, r, qindex, k, src_key, frames, force != 0);
+    if (groups) *groups = t.groups;
+    int n = 0;
+    for (const auto& [g, ms] : t.ms) {
+      if (n >= cap) break;
+      if (cand_groups) cand_groups[n] = g;
+      if (cand_ms) cand_ms[n] = ms;
+      n++;
+    }
+    if (ncand) *ncand = n;
+    return 0;
+  } catch (const std::invalid_argument& ex) {
+    g_err = ex.what();
+    return 1;
+  } catch (const std::exception& ex) {
+    g_err = ex.what();
+    return 2;
+  }
+}
+
 int celv2_engine_sync(void* p) {
   try {
     static_cast<Handle*>(p)->e->synchronize();

@@ -3,16 +3,19 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
 #include <condition_variable>
 #include <cstring>
 #include <functional>
 #include <iterator>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -220,7 +223,7 @@ void decision_to_sb_record(const SbDecision& d, celv2_sb& rec) {
 }
 
 HipEngine::HipEngine(int width, int height, int sb_size, int device)
-    : width_(width), height_(height), sb_cols_((width + 63) / 64), sb_rows_((height + 63) / 64) {
+    : width_(width), height_(height), sb_cols_((width + 63) / 64), sb_rows_((height + 63) / 64), device_(device) {
   if (width <= 0 || height <= 0) throw std::runtime_error("HipEngine: frame size must be positive");
   if (sb_size != 64) throw std::runtime_error("HipEngine: only 64x64 superblocks are supported");
   const int rc = celv2hip_create(width, height, sb_size, device, &h_);
@@ -235,7 +238,105 @@ HipEngine::~HipEngine() {
   celv2hip_destroy(h_);
 }
 
-void HipEngine::set_row_streams(int n) { celv2hip_set_row_streams(h_, n); }
+void HipEngine::set_row_streams(int n) {
+  celv2hip_set_row_streams(h_, n);
+  row_streams_ = n;
+}
+
+void HipEngine::set_row_groups(int groups) {
+  const int rc = celv2hip_set_row_groups(h_, groups);
+  if (rc != CELV2_OK) throw std::invalid_argument(std::string("HipEngine::set_row_groups: ") + celv2hip_last_error());
+}
+
+int HipEngine::row_groups() const { return celv2hip_row_groups(h_); }
+
+HipEngine::Tuning HipEngine::autotune(const Frame420& src, const std::vector<const Frame420*>& refs, int qindex,
+                                      const std::vector<long long>& ref_keys, long long src_key, int frames, bool force) {
+  return tune(
+      static_cast<int>(refs.size()), frames, force,
+      [&](DecisionMap& dm) { launch(src, refs, qindex, dm, ref_keys, src_key); },
+      [&] { prepare(src, src_key); });
+}
+
+HipEngine::Tuning HipEngine::autotune_device(const celv2_luma& src, const std::vector<celv2_luma>& refs, int qindex,
+                                             const std::vector<long long>& ref_keys, long long src_key, int frames,
+                                             bool force) {
+  return tune(
+      static_cast<int>(refs.size()), frames, force,
+      [&](DecisionMap& dm) { launch_device(src, refs, qindex, dm, ref_keys, src_key); },
+      [&] { prepare_device(src, src_key); });
+}
+
+HipEngine::Tuning HipEngine::tune(int nrefs, int frames, bool force, const std::function<void(DecisionMap&)>& launch_one,
+                                  const std::function<void()>& prepare_one) {
+  // Best counts by (width, height, references, device), shared by the process's engines.
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, int> best;
+  const auto key = std::make_tuple(width_, height_, nrefs, device_);
+  Tuning t;
+  if (std::max(1, std::min(row_streams_, sb_rows_)) != 1) {
+    t.groups = row_groups();
+    return t;
+  }
+  if (!force) {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = best.find(key);
+    if (it != best.end()) {
+      set_row_groups(it->second);
+      t.groups = it->second, t.cached = true;
+      return t;
+    }
+  }
+  // Candidates: distinct rows per group (the launch splits the rows into runs of ceil(rows / groups)).
+  std::vector<int> cands;
+  std::vector<int> seen_rows;
+  for (int g : {1, 2, 3, 4, 5, 6, 8, 12, 16}) {
+    if (g > sb_rows_) break;
+    const int rows = (sb_rows_ + g - 1) / g;
+    if (std::find(seen_rows.begin(), seen_rows.end(), rows) != seen_rows.end()) continue;
+    seen_rows.push_back(rows);
+    cands.push_back((sb_rows_ + rows - 1) / rows);
+  }
+  const int was = row_groups();
+  DecisionMap dm;
+  RowGate gate;
+  auto frame = [&] {
+    launch_one(dm);
+    prepare_one();
+    collect(dm, gate);
+  };
+  std::vector<std::vector<double>> ms(cands.size());
+  try {
+    frames = std::max(1, frames);
+    prepare_one();
+    for (size_t c = 0; c < cands.size(); c++) {  // one untimed frame at each count
+      set_row_groups(cands[c]);
+      frame();
+    }
+    for (int f = 0; f < frames; f++)
+      for (size_t c = 0; c < cands.size(); c++) {
+        set_row_groups(cands[c]);
+        const auto t0 = std::chrono::steady_clock::now();
+        frame();
+        ms[c].push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+      }
+    synchronize();
+  } catch (...) {
+    set_row_groups(was);
+    throw;
+  }
+  size_t pick = 0;
+  for (size_t c = 0; c < cands.size(); c++) {
+    std::sort(ms[c].begin(), ms[c].end());
+    t.ms.emplace_back(cands[c], ms[c][ms[c].size() / 2]);
+    if (t.ms[c].second < t.ms[pick].second) pick = c;
+  }
+  t.groups = cands[pick];
+  set_row_groups(t.groups);
+  std::lock_guard<std::mutex> lock(mu);
+  best[key] = t.groups;
+  return t;
+}
 
 void HipEngine::synchronize() {
   const int rc = celv2hip_sync(h_);

@@ -39,6 +39,8 @@ def lib():
         L.celv2hip_destroy.argtypes = [P]
         L.celv2hip_device_name.argtypes = [P, ctypes.c_char_p, I]
         L.celv2hip_set_row_streams.argtypes = [P, I]
+        L.celv2hip_set_row_groups.argtypes = [P, I]
+        L.celv2hip_row_groups.argtypes = [P]
         L.celv2hip_set_resident.argtypes = [P, I]
         L.celv2hip_prepare.argtypes = [P, ctypes.POINTER(Luma), LL]
         L.celv2hip_launch.argtypes = [P, ctypes.POINTER(Luma), LL, ctypes.POINTER(Luma), ctypes.POINTER(LL), I, ctypes.c_int64]
@@ -88,6 +90,11 @@ def lib():
         L.celv2_engine_collect.argtypes = [P, P, ctypes.POINTER(I)]
         L.celv2_engine_description.argtypes = [P, ctypes.c_char_p, I]
         L.celv2_engine_sync.argtypes = [P]
+        L.celv2_engine_set_row_groups.argtypes = [P, I]
+        L.celv2_engine_row_groups.argtypes = [P]
+        L.celv2_engine_autotune_device.argtypes = [P, ctypes.POINTER(Luma), ctypes.POINTER(Luma), ctypes.POINTER(LL), I, I, LL,
+                                                   LL, I, I, ctypes.POINTER(I), ctypes.POINTER(I),
+                                                   ctypes.POINTER(ctypes.c_double), I, ctypes.POINTER(I)]
         L.celv2_lambda_q8.argtypes = [I]
         L.celv2_lambda_q8.restype = ctypes.c_int64
         if L.celv2_sizeof_sb() != SB_DTYPE.itemsize:
@@ -208,6 +215,30 @@ class HipEngine:
         if lib().celv2_engine_sync(self.h) != 0:
             raise EngineError(lib().celv2_engine_last_error().decode())
 
+    def set_row_groups(self, groups):
+        """Row groups of the one-row-stream launch (0: the default); the decisions do not depend on them."""
+        if lib().celv2_engine_set_row_groups(self.h, groups) != 0:
+            raise ValueError(lib().celv2_engine_last_error().decode())
+
+    def row_groups(self):
+        return lib().celv2_engine_row_groups(self.h)
+
+    def autotune_device(self, src, refs, qindex=120, lam=-1, keys=None, src_key=-1, frames=5, force=False):
+        """Times steady-state frames (launch -> prepare -> collect) of these device frames at each
+        candidate row-group count and keeps the fastest: (chosen count, {count: median ms}); the dict is
+        empty when the process had tuned this size, reference count and device already."""
+        arr = (Luma * max(1, len(refs)))(*refs)
+        k = (LL * max(1, len(refs)))(*keys) if keys is not None else None
+        g, n = I(0), I(0)
+        cg, cm = (I * 16)(), (ctypes.c_double * 16)()
+        rc = lib().celv2_engine_autotune_device(self.h, ctypes.byref(src), arr, k, len(refs), qindex, lam, src_key, frames,
+                                                int(force), ctypes.byref(g), cg, cm, 16, ctypes.byref(n))
+        if rc == 1:
+            raise ValueError(lib().celv2_engine_last_error().decode())
+        if rc != 0:
+            raise EngineError(lib().celv2_engine_last_error().decode())
+        return g.value, {cg[i]: cm[i] for i in range(n.value)}
+
     def decide(self, src8, refs8, lam, **kw):
         self.launch(src8, refs8, lam=lam, **kw)
         return self.collect()[0]
@@ -241,6 +272,9 @@ class RawEngine:
 
     def set_row_streams(self, n):
         self._check(lib().celv2hip_set_row_streams(self.h, n), "set_row_streams")
+
+    def set_row_groups(self, groups):
+        self._check(lib().celv2hip_set_row_groups(self.h, groups), "set_row_groups")
 
     @staticmethod
     def luma(y8=None, y16=None, bit_depth=8, dev8=None, dev16=None):

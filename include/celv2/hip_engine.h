@@ -7,8 +7,10 @@
 // include/celv2hip.h.
 #pragma once
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #ifdef CELV2_IN_TREE
@@ -55,6 +57,26 @@ class HipEngine : public DecisionEngine {
 
   // Waits for all work queued on the engine's streams (prepare()'s included); for timing.
   void synchronize();
+
+  // Row groups of the one-row-stream launch (celv2hip_set_row_groups; 0: the default). The decisions
+  // do not depend on them.
+  void set_row_groups(int groups);
+  int row_groups() const;
+  // Picks the row-group count by timing: `frames` steady-state frames (launch with the source
+  // prepared -> prepare of the same source under the same key -> collect, after one untimed frame) at
+  // each candidate count, round-robin; keeps the count with the fastest median frame and remembers it
+  // for every engine of this frame size, reference count and device in the process (a later call
+  // finds it there and times nothing; force re-times). With more than one row stream there is
+  // nothing to pick. No counterpart in CelV2: its launch shapes are hand-set (src/vk/vk_kernels.cpp:193-209).
+  struct Tuning {
+    int groups = 0;
+    bool cached = false;
+    std::vector<std::pair<int, double>> ms;  // (row groups, median ms per frame) per candidate
+  };
+  Tuning autotune(const Frame420& src, const std::vector<const Frame420*>& refs, int qindex,
+                  const std::vector<long long>& ref_keys, long long src_key, int frames = 5, bool force = false);
+  Tuning autotune_device(const celv2_luma& src, const std::vector<celv2_luma>& refs, int qindex,
+                         const std::vector<long long>& ref_keys, long long src_key, int frames = 5, bool force = false);
   const std::string& device_name() const { return name_; }
   std::string description() const { return "decisions on HIP (" + name_ + ")"; }
   // Frames above 8 bits: send the full-depth luma (hy) and round it on the device instead of
@@ -66,7 +88,10 @@ class HipEngine : public DecisionEngine {
  private:
   void launch_luma(const celv2_luma& src, const celv2_luma* refs, int nrefs, int qindex, DecisionMap& dm,
                    const std::vector<long long>& ref_keys, long long src_key);
-  int width_, height_, sb_cols_, sb_rows_;
+  Tuning tune(int nrefs, int frames, bool force, const std::function<void(DecisionMap&)>& launch,
+              const std::function<void()>& prepare);
+  int width_, height_, sb_cols_, sb_rows_, device_;
+  int row_streams_ = 1;
   celv2hip_engine* h_ = nullptr;
   std::unique_ptr<CollectPool> pool_;  // created by the first collect
   std::string name_;

@@ -105,6 +105,14 @@ int celv2hip_device_name(const celv2hip_engine* e, char* buf, int cap);
 /* DecisionEngine::set_row_streams (src/gpu/decision_engine.h:30-33). */
 int celv2hip_set_row_streams(celv2hip_engine* e, int n);
 
+/* Launch shape of a single engine with one row stream whose records go to the host: the superblock
+ * rows fall into `groups` equal runs; level 0 of the motion search runs run by run, and each run's
+ * inter scores, finalize and record download overlap the next run's search (0: the default, 4). The
+ * decisions do not depend on it; celv2::HipEngine::autotune picks it by timing. No counterpart in the
+ * reference (its hand-set launch rules, src/vk/vk_kernels.cpp:193-209, are the closest). */
+int celv2hip_set_row_groups(celv2hip_engine* e, int groups);
+int celv2hip_row_groups(const celv2hip_engine* e);
+
 /* DecisionEngine::prepare (src/gpu/decision_engine.h:19-22; src/vk/vk_engine.cpp:154-171): queues the
  * next source's import, pyramid and intra scores behind the current work. key < 0: nothing. */
 int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key);
@@ -275,6 +283,15 @@ int celv2_engine_collect(void* engine, celv2_sb* out, int* rows_ready);
 int celv2_engine_description(void* engine, char* buf, int cap);
 /* Waits for all work queued on the engine's streams (timing). */
 int celv2_engine_sync(void* engine);
+/* HipEngine::set_row_groups / row_groups / autotune_device: the one-row-stream launch's row groups, and
+ * the tuner that picks them by timing steady-state frames. The tuner writes the chosen count and
+ * (row groups, median ms per frame) for each candidate it timed, up to cap (none when the process
+ * had tuned this size, reference count and device already). No counterpart in the reference. */
+int celv2_engine_set_row_groups(void* engine, int groups);
+int celv2_engine_row_groups(void* engine);
+int celv2_engine_autotune_device(void* engine, const celv2_luma* src, const celv2_luma* refs, const long long* keys,
+                                 int nrefs, int qindex, long long lam, long long src_key, int frames, int force,
+                                 int* groups, int* cand_groups, double* cand_ms, int cap, int* ncand);
 int celv2_sizeof_sb(void);
 
 /* Digest of the native sources the library was built from (the build's own check against a stale

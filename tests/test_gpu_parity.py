@@ -196,3 +196,40 @@ def test_device_frames_through_the_cpp_engine(gpu):
         e.close()
         for b in bufs:
             b.free()
+
+
+def test_row_groups_and_their_tuner_leave_the_decisions_alone(gpu):
+    """The one-row-stream launch split into 1, 2, 3, 11 (one per row) and the default number of row
+    groups gives the oracle's DecisionMap every time; the tuner picks one of the counts it timed,
+    leaves the engine at it, and a second engine of the same size finds it without timing."""
+    w, h = 448, 704  # 7 x 11 superblocks
+    _, _, y8 = clip(1, 31, 4, w, h)
+    lam = gpu.lambda_q8(110)
+    bufs = [gpu.DeviceBuffer(f) for f in y8]
+    lum = [gpu.RawEngine.luma(dev8=b.ptr) for b in bufs]
+    want = ol.decide_frame(y8[3], [y8[2], y8[1], y8[0]], lam, w, h)
+    engines = [gpu.HipEngine(w, h), gpu.HipEngine(w, h)]
+    try:
+        e = engines[0]
+        with pytest.raises(ValueError):
+            e.set_row_groups(-1)
+        with pytest.raises(ValueError):
+            e.set_row_groups(65)
+        for g in (1, 2, 3, 11, 0):
+            e.set_row_groups(g)
+            e.launch_device(lum[3], [lum[2], lum[1], lum[0]], lam=lam, keys=[2, 1, 0], src_key=3)
+            got, ready = e.collect()
+            assert ready == e.sb_rows
+            check(got, want, e.sb_cols)
+        g, timed = e.autotune_device(lum[3], [lum[2], lum[1], lum[0]], lam=lam, keys=[2, 1, 0], src_key=3, frames=2, force=True)
+        assert g in timed and len(timed) >= 4 and all(ms > 0 for ms in timed.values())
+        assert e.row_groups() == g
+        e.launch_device(lum[3], [lum[2], lum[1], lum[0]], lam=lam, keys=[2, 1, 0], src_key=3)
+        check(e.collect()[0], want, e.sb_cols)
+        g2, timed2 = engines[1].autotune_device(lum[3], [lum[2], lum[1], lum[0]], lam=lam, keys=[2, 1, 0], src_key=3)
+        assert (g2, timed2) == (g, {}) and engines[1].row_groups() == g
+    finally:
+        for x in engines:
+            x.close()
+        for b in bufs:
+            b.free()
