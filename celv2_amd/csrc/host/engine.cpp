@@ -154,6 +154,12 @@ struct celv2hip_engine {
   bool copy_pending = false;
   hipEvent_t sub_start = nullptr, sub_done = nullptr;
   hipEvent_t order_ev = nullptr;  // orders this engine's stream around batched launches
+  // The last launch ran in a batch on another engine's stream (batch_on), batch_done recorded there
+  // behind it: later work of this engine that is not ordered by that stream waits for it then. (A
+  // wait put on this engine's own stream at once would block the hardware queue that stream shares
+  // with other streams, and so their work, until the whole batch is done.)
+  hipEvent_t batch_done = nullptr;
+  hipStream_t batch_on = nullptr;
   // Recorded before the last launch's level-0 motion search: prepare() starts the next source's
   // intra scores there, beside the long latency-bound launches (level 0, inter, finalize) rather than
   // beside the short coarse levels, which wait for free CU slots while intra holds them.
@@ -208,6 +214,7 @@ struct celv2hip_engine {
 
   ~celv2hip_engine() {
     if (device >= 0) (void)hipSetDevice(device);
+    if (batch_on) (void)hipEventSynchronize(batch_done);
     if (stream) (void)hipStreamSynchronize(stream);
     if (side) (void)hipStreamSynchronize(side);
     if (aux) (void)hipStreamSynchronize(aux);
@@ -255,10 +262,12 @@ struct celv2hip_engine {
     if (sub_start) (void)hipEventDestroy(sub_start);
     if (sub_done) (void)hipEventDestroy(sub_done);
     if (order_ev) (void)hipEventDestroy(order_ev);
+    if (batch_done) (void)hipEventDestroy(batch_done);
     if (deep_ev) (void)hipEventDestroy(deep_ev);
   }
 
   void sync_all() {
+    if (batch_on) check(hipEventSynchronize(batch_done), "hipEventSynchronize batch");
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     check(hipStreamSynchronize(side), "hipStreamSynchronize side");
     check(hipStreamSynchronize(aux), "hipStreamSynchronize aux");
@@ -419,6 +428,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     check(hipEventCreateWithFlags(&e->sub_start, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->sub_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&e->batch_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->deep_ev, hipEventDisableTiming), "hipEventCreate");
     for (SrcSlot& s : e->srcs) {
       check(hipEventCreateWithFlags(&s.side_done, hipEventDisableTiming), "hipEventCreate");
@@ -824,9 +834,13 @@ void launch_batch(const LaunchIn* in, int n) {
     for (int k = 0; k < i; k++)
       if (in[k].e == x.e) throw Failure(CELV2_ERR_ARG, "an engine appears twice in one batch");
   }
-  for (int i = 1; i < n; i++) {
-    check(hipEventRecord(in[i].e->order_ev, in[i].e->stream), "hipEventRecord");
-    check(hipStreamWaitEvent(S, in[i].e->order_ev, 0), "hipStreamWaitEvent");
+  for (int i = 0; i < n; i++) {
+    celv2hip_engine* e = in[i].e;
+    if (i > 0) {  // the engine's own earlier launches
+      check(hipEventRecord(e->order_ev, e->stream), "hipEventRecord");
+      check(hipStreamWaitEvent(S, e->order_ev, 0), "hipStreamWaitEvent");
+    }
+    if (e->batch_on && e->batch_on != S) check(hipStreamWaitEvent(S, e->batch_done, 0), "hipStreamWaitEvent");
   }
   std::vector<celv2hip_engine::BuildJob> jobs;
   std::vector<std::pair<celv2hip_engine*, SrcSlot*>> intra_now;
@@ -834,6 +848,8 @@ void launch_batch(const LaunchIn* in, int n) {
   std::vector<std::vector<RefSlot*>> used(n);
   std::vector<char> use_kmap(n);
   for (int i = 0; i < n; i++) use_kmap[i] = in[i].e->kmap_on;
+  // Prepared sources whose intra scores the launch stream waits for only before finalize.
+  std::vector<char> late_claim(n);
   // One engine with one row stream whose records go to the host (the wavefront form): level 0 of the
   // motion search runs in row groups interleaved with the inter scores of the groups above them; each
   // group's finalize runs on the side stream, its record download on the aux stream.
@@ -848,10 +864,11 @@ void launch_batch(const LaunchIn* in, int n) {
     for (int k = 0; k < 2; k++)
       if (x.src_key >= 0 && e->srcs[k].key == x.src_key) use = k;
     SrcSlot& claimed = e->srcs[use < 0 ? 1 - e->cur_set : use];
-    if (wavefront && use >= 0) {
-      // A prepared source: the launch stream reads its pyramid; finalize, on the side stream, comes
-      // after its intra scores there.
+    if (use >= 0 && (wavefront || n > 1)) {
+      // A prepared source: the motion search reads its pyramid; finalize waits for its intra scores
+      // (on the side stream in the wavefront form, which computes them there first).
       if (claimed.pyr_pending) check(hipStreamWaitEvent(S, claimed.pyr_done, 0), "hipStreamWaitEvent");
+      late_claim[i] = !wavefront;
     } else {
       e->claim(claimed, S);
     }
@@ -1024,6 +1041,7 @@ void launch_batch(const LaunchIn* in, int n) {
     }
     launch_inter(bi, S);
     for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[3], S), "hipEventRecord");
+    for (int i = 0; i < n; i++) if (late_claim[i]) in[i].e->claim(*cur[i], S);
     Batch<FinalizeArgs> bf;
     for (int i = 0; i < n; i++) {
       FinalizeArgs fa = finalize_args(i);
@@ -1110,11 +1128,11 @@ void launch_batch(const LaunchIn* in, int n) {
     e->last_refs = used[i];
     e->launched = true;
   }
-  // One event on S orders every other engine's later work after the batch (each record between
-  // kernels costs S a few microseconds).
-  if (n > 1) {
-    check(hipEventRecord(e0->order_ev, S), "hipEventRecord");
-    for (int i = 1; i < n; i++) check(hipStreamWaitEvent(in[i].e->stream, e0->order_ev, 0), "hipStreamWaitEvent");
+  // The other engines' later work waits for the batch when it comes (batch_done).
+  e0->batch_on = nullptr;
+  for (int i = 1; i < n; i++) {
+    check(hipEventRecord(in[i].e->batch_done, S), "hipEventRecord");
+    in[i].e->batch_on = S;
   }
 }
 

@@ -121,7 +121,8 @@ def test_resident_device_frames(gpu):
 
 def test_batched_launch_matches_single_launches(gpu):
     """Three engines of different sizes in shared launches (celv2hip_launch_batch) decide exactly as
-    each alone; a second batched frame reuses the cached references by key."""
+    each alone; a second batched frame reuses the cached references by key; launched alone after the
+    batches, the engines still decide as the ones that never batched."""
     sizes = [(352, 288), (202, 122), (160, 96)]
     clips = [synth.clip_frames(synth.params(0, 11 + i, w, h), 4)[1] for i, (w, h) in enumerate(sizes)]
     lam = gpu.lambda_q8(120)
@@ -138,6 +139,16 @@ def test_batched_launch_matches_single_launches(gpu):
             want = e.collect()
             diff = compare_maps(got[i], want, e.sb_cols)
             assert diff is None, (f, sizes[i], diff)
+    # The batch's engines launched alone afterwards (their own streams, after the batch on another's):
+    # the next frame decides as the engines that never batched.
+    srcs = [gpu.RawEngine.luma(c[1]) for c in clips]
+    refs = [[gpu.RawEngine.luma(c[0])] for c in clips]
+    for k in (2, 1, 0):
+        batch[k].launch(srcs[k], refs[k], lam, keys=[0])
+        batch[k].prepare(srcs[k], 9)
+    for k in range(3):
+        alone[k].launch(srcs[k], refs[k], lam, keys=[0])
+        assert compare_maps(batch[k].collect(), alone[k].collect(), alone[k].sb_cols) is None, sizes[k]
     for e in batch + alone:
         e.close()
 
