@@ -164,6 +164,74 @@ def bench_batched(NR, lam, steps=20, warmup=3, nf=16):
             "roofline_batched": roof}
 
 
+def bench_transforms(y8, frame, steps, warmup, cpu_seconds=0.0):
+    """The transform step of the RD pass's trial scoring (celv2hip_fwd_tx) over every candidate block
+    of one 1080p frame: the 425 blocks of each of the 510 superblocks, DCT-II at the block's own size
+    (8x4 to 64x64), residuals frame `frame` less frame `frame` - 1 (int16, zero below the picture),
+    resident in HBM. Per call: the job list uploaded, one kernel, the coefficients (AVM's layout, low
+    32x32) and 64-point dropped energies written to HBM. Returns calls/s, the kernel's own time and its
+    HBM roofline; cpu_seconds > 0: the oracle's transform on a bounded sample of the same jobs."""
+    from celv2_amd import transform as tx
+
+    H8, W8 = y8[frame].shape
+    sb_cols, sb_rows = (W8 + 63) // 64, (H8 + 63) // 64
+    plane = np.zeros((sb_rows * 64, sb_cols * 64), np.int16)
+    plane[:H8, :W8] = y8[frame].astype(np.int16) - y8[frame - 1].astype(np.int16)
+    res = engine.DeviceBuffer(plane)
+    probe, off = tx.frame_jobs(0, plane.shape[1], sb_cols, sb_rows, 0)
+    coeff = engine.DeviceBuffer(np.zeros(int(off[-1]), np.int32))
+    dropped = engine.DeviceBuffer(np.zeros(len(probe), np.int64))
+    jobs, off = tx.frame_jobs(res.ptr, plane.shape[1], sb_cols, sb_rows, coeff.ptr, dropped.ptr, scale=1.0)
+    n = len(jobs)
+    sides = np.stack([1 << jobs["w_log2"].astype(np.int64), 1 << jobs["h_log2"].astype(np.int64)])
+    res_bytes = int(np.sum(sides[0] * sides[1]) * 2)
+    out_bytes = int(off[-1]) * 4 + 8 * n
+    list_bytes = n * 48  # the device job records the kernel reads
+    for _ in range(warmup):
+        tx.fwd_tx(jobs)
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kms.append(tx.fwd_tx(jobs))
+    wall = (time.perf_counter() - t0) / steps
+    kernel_s = float(np.median(kms)) / 1e3
+    alg = res_bytes + out_bytes + list_bytes
+    out = {"workload": f"every candidate block of one 1080p frame ({sb_cols * sb_rows} superblocks x 425 blocks, 8x4 to 64x64, "
+                       "DCT-II at the block's size, scale 1.0), residuals of two clip frames resident in HBM",
+           "jobs_per_call": n, "calls_per_s": round(1.0 / wall, 2), "jobs_per_s": round(n / wall),
+           "ms_per_call": round(wall * 1e3, 4), "kernel_ms": round(kernel_s * 1e3, 4),
+           "kernel_jobs_per_s": round(n / kernel_s),
+           "roofline": {"bound": "hbm", "kernel": "k_fwd_tx", "achieved": round(alg / kernel_s / 1e9, 1), "peak": 8000.0,
+                        "unit": "GB/s", "frac": round(alg / kernel_s / 8e12, 4), "traffic": None,
+                        "algorithmic_bytes_per_call": alg,
+                        "bytes": {"residuals": res_bytes, "coefficients_and_dropped": out_bytes, "job_records": list_bytes}}}
+    if cpu_seconds > 0:
+        # The oracle's transform (test infrastructure, timed here only) on one thread over the same
+        # jobs from the first, a run sized to about cpu_seconds; its coefficients also check the GPU's.
+        O = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+        O.orc_fwd_tx_frame.restype = ctypes.c_longlong
+        O.orc_fwd_tx_frame.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_longlong,
+                                       ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p]
+        want = np.zeros(int(off[-1]), np.int32)
+        t0 = time.perf_counter()
+        O.orc_fwd_tx_frame(plane.ctypes.data, plane.shape[1], sb_cols, sb_rows, 0, 4250, 1.0, want.ctypes.data)
+        per = (time.perf_counter() - t0) / 4250
+        m = int(min(n, max(4250, cpu_seconds / per)))
+        t0 = time.perf_counter()
+        O.orc_fwd_tx_frame(plane.ctypes.data, plane.shape[1], sb_cols, sb_rows, 0, m, 1.0, want.ctypes.data)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(m / dt), "unit": "jobs/s", "cores": 1, "kind": "port",
+                               "sample": f"the first {m} jobs ({m // 425} superblocks) of the same list, the oracle's "
+                                         f"transform on one thread in {dt:.1f} s"}
+        flat = coeff.read(np.int32, int(off[m]))
+        out["sample_equals_oracle"] = bool(np.array_equal(flat, want[:int(off[m])]))
+    for b in (res, coeff, dropped):
+        b.free()
+    log(f"transforms: {n} jobs a call, {out['ms_per_call']} ms a call ({out['kernel_ms']} ms kernel, "
+        f"{out['roofline']['achieved']} GB/s algorithmic)")
+    return out
+
+
 def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
     """The learned-pruned form on the same workload: the trained network (celv2_amd/pruner) runs inside
     every launch on the engine's coarse search and limits each superblock's references; frames/s over
@@ -501,6 +569,7 @@ def main():
     ap.add_argument("--no-gop", action="store_true", help="skip the clip in the frame loop's coding order")
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU engine timed on all threads")
     ap.add_argument("--row-groups", type=int, default=0, help="the launch's row groups (0: tuned by timing)")
+    ap.add_argument("--no-transforms", action="store_true", help="skip the RD pass's forward transforms")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
     args = ap.parse_args()
@@ -678,6 +747,7 @@ def main():
             learned["qindex_sweep"] = bench_qindex_sweep(lum, decidable, NR, args.steps, args.warmup)
     cel = None if args.no_cel or args.no_learned or NR != 7 else bench_cel(NR, args.qindex)
     gop_form = None if args.no_gop or NR != 7 else bench_gop(lum, args.qindex)
+    transforms = None if args.no_transforms else bench_transforms(y8, last, 20, 3, 0.0 if args.no_cpu else 10.0)
 
     cpu = None
     parity = None
@@ -744,7 +814,8 @@ def main():
         "all_kernels_tops": round(total_ops * fps / 1e12, 3),
         "forms": {"plain": {"frames_per_s": round(fps_b, 2), "kernel_only_frames_per_s": round(fps, 2),
                             "scored_candidates_per_s": round(cands * fps_b), "roofline": me_roofline(cnt["me_centres"] / n, per["me"])},
-                  "batched": batched, "learned": learned, "learned_batched_cel_4k": cel, "gop_order": gop_form},
+                  "batched": batched, "learned": learned, "learned_batched_cel_4k": cel, "gop_order": gop_form,
+                  "transforms": transforms},
     }
     print(json.dumps(out), flush=True)
 

@@ -41,8 +41,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{os.path.join(ROOT, 'include')}"]
 
-KERNELS = ["planes", "me", "intra", "inter", "finalize", "pruner", "cnn"]
-HOST = ["engine", "hip_engine", "row_gate", "engine_capi", "gop", "y4m"]
+KERNELS = ["planes", "me", "intra", "inter", "finalize", "pruner", "cnn", "transform"]
+HOST = ["engine", "hip_engine", "row_gate", "engine_capi", "gop", "y4m", "tx_capi"]
 
 
 def _run(cmd):

@@ -25,35 +25,12 @@
 #include "celv2hip.h"
 #include "../kernels/launch.h"
 #include "pruner.h"
+#include "status.h"
 
 using namespace celv2hip;
+using namespace celv2host;
 
 namespace {
-
-thread_local std::string g_error;
-
-struct Failure : std::runtime_error {
-  int code;
-  Failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void check(hipError_t e, const char* what) {
-  if (e != hipSuccess) throw Failure(e == hipErrorOutOfMemory ? CELV2_ERR_MEMORY : CELV2_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <typename F>
-int guarded(F&& f) {
-  try {
-    f();
-    return CELV2_OK;
-  } catch (const Failure& e) {
-    g_error = e.what();
-    return e.code;
-  } catch (const std::exception& e) {
-    g_error = e.what();
-    return CELV2_ERR_STATE;
-  }
-}
 
 size_t round_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -1321,6 +1298,13 @@ int celv2hip_copy_to_device(int device, void* dst, const void* src, long long by
   return guarded([&] {
     check(hipSetDevice(device), "hipSetDevice");
     check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyHostToDevice), "hipMemcpy");
+  });
+}
+
+int celv2hip_copy_from_device(int device, void* dst, const void* src, long long bytes) {
+  return guarded([&] {
+    check(hipSetDevice(device), "hipSetDevice");
+    check(hipMemcpy(dst, src, static_cast<size_t>(bytes), hipMemcpyDeviceToHost), "hipMemcpy D2H");
   });
 }
 

@@ -114,5 +114,26 @@ void launch_features(const FeatureArgs& a, hipStream_t s);
 hipError_t upload_intra_tiles();
 hipError_t upload_inter_tables();
 hipError_t upload_final_tiles();
+hipError_t upload_tx_tables();
+
+// transform.hip: forward transforms of residual blocks. A job is one block (res: h rows of w int16
+// samples, stride apart; coeff: min(h, 32) rows of min(w, 32) coefficients; dropped may be null);
+// kv0 / kv: the vertical factor of frequency 0 / the others with the scale in it (as floats; the
+// scale itself for a vertical kind other than DCT-II). A unit is `count` consecutive jobs of one size
+// and kind pair, count <= tx_jobs_per_unit(hl); one wave per unit.
+struct TxJob {
+  const int16_t* res;
+  int32_t* coeff;
+  long long* dropped;
+  double scale;
+  int stride;
+  float kv0, kv;
+};
+struct TxUnit {
+  int first, count;
+  int wl, hl, vk, hk;  // log2 w, log2 h (2..6); vertical and horizontal kinds
+};
+int tx_jobs_per_unit(int hl);
+void launch_fwd_tx(const TxJob* jobs, const TxUnit* units, int nunits, hipStream_t s);
 
 }  // namespace celv2hip

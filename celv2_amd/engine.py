@@ -79,6 +79,8 @@ def lib():
         L.celv2hip_device_free.argtypes = [I, P]
         L.celv2hip_copy_to_device.argtypes = [I, P, P, LL]
         L.celv2hip_copy_on_device.argtypes = [I, P, P, LL]
+        L.celv2hip_copy_from_device.argtypes = [I, P, P, LL]
+        L.celv2hip_fwd_tx.argtypes = [I, P, I, ctypes.POINTER(ctypes.c_float)]
         L.celv2_engine_last_error.restype = ctypes.c_char_p
         L.celv2_engine_create.argtypes = [I, I, I, I, I]
         L.celv2_engine_create.restype = P
@@ -487,6 +489,15 @@ class DeviceBuffer:
         self.ptr = p.value
         if lib().celv2hip_copy_to_device(device, self.ptr, np.ascontiguousarray(array).ctypes.data, array.nbytes) != 0:
             raise EngineError(last_error())
+
+    def read(self, dtype, count=-1, offset=0):
+        """The buffer's bytes from `offset` as a numpy array of dtype (count items; -1: to the end)."""
+        dt = np.dtype(dtype)
+        n = (self.nbytes - offset) // dt.itemsize if count < 0 else count
+        out = np.empty(n, dt)
+        if n and lib().celv2hip_copy_from_device(self.device, out.ctypes.data, self.ptr + offset, out.nbytes) != 0:
+            raise EngineError(last_error())
+        return out
 
     def copy_from(self, other):
         """Device-to-device copy of other's bytes into this buffer (a copy kernel of known size)."""

@@ -254,10 +254,41 @@ int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, i
  * (superblocks x 425 x 13) or a negative status. */
 long long celv2hip_fetch_intra(celv2hip_engine* e, uint32_t* out, long long cap);
 
+/* ---- Forward transforms of the RD pass's trial scoring ------------------------------------------
+ * The transform step of tx_cost (src/core/rd_pass.cpp:231-300): per residual block, fwd_dct's
+ * coefficients (src/core/transform.h:12, transform.cpp:170-214: a separable DCT-II in float, rows then
+ * the kept columns, orthonormal times `scale`, rounded half away from zero; AVM's layout, row v =
+ * vertical frequency, only min(w, 32) x min(h, 32) kept) and, for a 64-point side, dropped_energy
+ * (rd_pass.cpp:268-276); 0 below 64 points. Kinds per direction as fwd_transform
+ * (transform.cpp:250-262) takes them from a TX_TYPE (g_ver_tx_type, g_hor_tx_type): DCT-II at every
+ * size, DST-VII and its flip up to 16 points and the identity up to 32, these three as orthonormal
+ * float forms (AVM's integer kernels and their tables are not part of this library). */
+enum {
+  CELV2_TX_DCT = 0,     /* DCT-II (fwd_dct's values) */
+  CELV2_TX_DST = 1,     /* DST-VII */
+  CELV2_TX_FLIPDST = 2, /* DST-VII of the reversed samples */
+  CELV2_TX_IDT = 3      /* identity */
+};
+typedef struct celv2_tx_job {
+  const int16_t* res; /* device memory: the block's first residual sample */
+  int32_t* coeff;     /* device memory: min(h, 32) rows of min(w, 32) coefficients */
+  int64_t* dropped;   /* device memory or NULL */
+  double scale;       /* > 0 */
+  int32_t stride;     /* samples from one residual row to the next, >= w */
+  uint8_t w_log2, h_log2; /* 2..6 each (4..64 samples) */
+  uint8_t vkind, hkind;   /* vertical and horizontal kind (CELV2_TX_*) */
+} celv2_tx_job;
+/* Transforms every job on `device` and returns when they are done; kernel_ms (may be NULL) gets the
+ * kernel's own duration (HIP events around it, the job list's upload excluded). Jobs may come in any
+ * order and mix sizes and kinds; one job's result does not depend on the others. CELV2_ERR_ARG
+ * (nothing runs) for a job with a size, kind, stride, scale or pointer outside the above. */
+int celv2hip_fwd_tx(int device, const celv2_tx_job* jobs, int njobs, float* kernel_ms);
+
 /* ---- Device memory helpers for callers without another allocator --------------------------- */
 int celv2hip_device_alloc(int device, long long bytes, void** out);
 int celv2hip_device_free(int device, void* p);
 int celv2hip_copy_to_device(int device, void* dst, const void* src, long long bytes);
+int celv2hip_copy_from_device(int device, void* dst, const void* src, long long bytes);
 /* Device-to-device copy of `bytes` (a plain copy kernel of known size, for calibrating traffic counters). */
 int celv2hip_copy_on_device(int device, void* dst, const void* src, long long bytes);
 

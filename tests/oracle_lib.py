@@ -43,6 +43,25 @@ def _prims(lib, prefix):
     return lib
 
 
+def tx_api(lib):
+    lib.orc_fwd_tx.argtypes = [P, I, I, I, I, I, ctypes.c_double, P, P]
+    return lib
+
+
+def fwd_tx(res, vkind=0, hkind=0, scale=1.0, lib=None):
+    """The oracle's forward transform of one residual block (int16, h x w): (coefficients, min(h, 32)
+    x min(w, 32) int32, row = vertical frequency; dropped energy)."""
+    lib = lib or oracle()
+    res = np.ascontiguousarray(res, dtype=np.int16)
+    h, w = res.shape
+    out = np.zeros((min(h, 32), min(w, 32)), np.int32)
+    dropped = np.zeros(1, np.int64)
+    rc = lib.orc_fwd_tx(res.ctypes.data, w, w, h, vkind, hkind, scale, out.ctypes.data, dropped.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"no {w}x{h} transform of kinds {vkind}/{hkind}")
+    return out, int(dropped[0])
+
+
 def oracle():
     global _oracle
     if _oracle is None:
@@ -62,6 +81,7 @@ def oracle():
         lib.orc_pyramid.restype = LL
         lib.orc_decide_square.argtypes = [I, I, P, P, P, P, I, I, I, I64, I, I, I, I, I, P, P, P, P]
         lib.orc_decide_square.restype = LL
+        tx_api(lib)
         assert lib.orc_sizeof_sb() == SB_DTYPE.itemsize, "celv2_sb layout mismatch"
         _oracle = lib
     return _oracle
