@@ -570,6 +570,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=3, help="frames of the CPU engine timed on all threads")
     ap.add_argument("--row-groups", type=int, default=0, help="the launch's row groups (0: tuned by timing)")
     ap.add_argument("--no-transforms", action="store_true", help="skip the RD pass's forward transforms")
+    ap.add_argument("--no-graph", action="store_true", help="skip the resident steps replayed as HIP graphs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a known-size kernel for scaling traffic counters)")
     args = ap.parse_args()
@@ -635,6 +636,36 @@ def main():
     _, cnt4 = E.stats()
     E.set_counting(False)
     inter_cands = cnt4["inter_candidates"] / max(1, cnt4["launches"])
+
+    # The same resident steps eager and as replayed HIP graphs (celv2hip_set_graph): frames/s and the
+    # host thread's CPU time to queue a frame (launch + prepare), then the last frame's decisions of
+    # both against each other.
+    def host_loop(graph, base):
+        E.set_graph(graph)
+        for s_ in range(base, base + args.warmup):
+            step(s_)
+        E.sync()
+        cpu, t0 = 0.0, time.perf_counter()
+        for s_ in range(base + args.warmup, base + args.warmup + args.steps):
+            c0 = time.thread_time()
+            f_ = step(s_)
+            cpu += time.thread_time() - c0
+        E.sync()
+        return args.steps / (time.perf_counter() - t0), cpu / args.steps * 1e3, f_
+
+    graph_form = None
+    if not args.no_graph:
+        base = args.warmup + args.steps + 4
+        fps_e, cpu_e, f_e = host_loop(False, base)
+        map_e = E.fetch()
+        fps_g, cpu_g, f_g = host_loop(True, base)
+        map_g = E.fetch()
+        graph_form = {"frames_per_s": round(fps_g, 2), "host_cpu_ms_per_frame": round(cpu_g, 4),
+                      "eager_frames_per_s": round(fps_e, 2), "eager_host_cpu_ms_per_frame": round(cpu_e, 4),
+                      "graph_launches_and_builds": list(E.graph_stats()),
+                      "decisions_equal_eager": f_e == f_g and compare_maps(map_e, map_g, E.sb_cols) is None}
+        E.set_graph(False)
+        log(f"graph form: {fps_g:.1f} frames/s, host {cpu_g:.3f} ms a frame; eager {fps_e:.1f} frames/s, host {cpu_e:.3f} ms")
 
     # Algorithmic integer work per frame (weights of the reference's cost rules, counted as executed).
     sbs = E.sb_cols * E.sb_rows
@@ -815,7 +846,7 @@ def main():
         "forms": {"plain": {"frames_per_s": round(fps_b, 2), "kernel_only_frames_per_s": round(fps, 2),
                             "scored_candidates_per_s": round(cands * fps_b), "roofline": me_roofline(cnt["me_centres"] / n, per["me"])},
                   "batched": batched, "learned": learned, "learned_batched_cel_4k": cel, "gop_order": gop_form,
-                  "transforms": transforms},
+                  "transforms": transforms, "graph": graph_form},
     }
     print(json.dumps(out), flush=True)
 

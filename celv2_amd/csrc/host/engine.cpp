@@ -146,6 +146,12 @@ struct celv2hip_engine {
   int row_streams = 1;
   int row_groups = kDefaultRowGroups;  // the wavefront form's row groups
   bool resident = false;
+  // Launches of resident engines captured as a HIP graph and replayed (celv2hip_set_graph): the
+  // executable graph of the last launch run on this engine's stream, updated in place when the next
+  // launch has the same shape, instantiated again when it has not.
+  bool graph_on = false;
+  hipGraphExec_t gexec = nullptr;
+  long long graph_launches = 0, graph_builds = 0;
   bool counting = false;  // count non-repeated inter candidates (costs the inter kernel LDS atomics)
   SrcSlot srcs[2];
   int cur_set = 0;
@@ -195,6 +201,7 @@ struct celv2hip_engine {
     if (stream) (void)hipStreamSynchronize(stream);
     if (side) (void)hipStreamSynchronize(side);
     if (aux) (void)hipStreamSynchronize(aux);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     auto free_up = [](Upload& u) {
       if (u.host) (void)hipHostFree(u.host);
       if (u.dev) (void)hipFree(u.dev);
@@ -476,6 +483,19 @@ int celv2hip_row_groups(const celv2hip_engine* e) { return e ? e->row_groups : -
 int celv2hip_set_resident(celv2hip_engine* e, int on) {
   if (!e) return CELV2_ERR_ARG;
   e->resident = on != 0;
+  return CELV2_OK;
+}
+
+int celv2hip_set_graph(celv2hip_engine* e, int on) {
+  if (!e) return CELV2_ERR_ARG;
+  e->graph_on = on != 0;
+  return CELV2_OK;
+}
+
+int celv2hip_graph_stats(const celv2hip_engine* e, long long* launches, long long* builds) {
+  if (!e) return CELV2_ERR_ARG;
+  if (launches) *launches = e->graph_launches;
+  if (builds) *builds = e->graph_builds;
   return CELV2_OK;
 }
 
@@ -797,6 +817,17 @@ struct LaunchIn {
   int64_t lam;
 };
 
+// Ends a stream capture that a throw left open (the captured graph is dropped).
+struct CaptureGuard {
+  hipStream_t s = nullptr;
+  ~CaptureGuard() {
+    if (!s) return;
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(s, &g);
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
 // One frame of each engine: every stage is one launch (or one per kMaxItems engines) on the first
 // engine's stream; each engine's own stream orders its earlier work before and its later work after.
 void launch_batch(const LaunchIn* in, int n) {
@@ -831,6 +862,12 @@ void launch_batch(const LaunchIn* in, int n) {
   // motion search runs in row groups interleaved with the inter scores of the groups above them; each
   // group's finalize runs on the side stream, its record download on the aux stream.
   const bool wavefront = n == 1 && !e0->resident && std::max(1, std::min(e0->row_streams, e0->sb_rows)) == 1;
+  // The graph form: every engine resident with its graph on and no stage events. The launch's kernels
+  // are captured on S (the aux stream's phase planes joined back in), and the graph replayed; every
+  // wait on work outside the launch is placed on S before the capture, every event that work outside
+  // waits for is recorded after the replay.
+  bool use_graph = true;
+  for (int i = 0; i < n; i++) use_graph = use_graph && in[i].e->graph_on && in[i].e->resident && !in[i].e->timing;
   for (int i = 0; i < n; i++) {
     const LaunchIn& x = in[i];
     celv2hip_engine* e = x.e;
@@ -841,7 +878,7 @@ void launch_batch(const LaunchIn* in, int n) {
     for (int k = 0; k < 2; k++)
       if (x.src_key >= 0 && e->srcs[k].key == x.src_key) use = k;
     SrcSlot& claimed = e->srcs[use < 0 ? 1 - e->cur_set : use];
-    if (use >= 0 && (wavefront || n > 1)) {
+    if (use >= 0 && (wavefront || n > 1) && !use_graph) {
       // A prepared source: the motion search reads its pyramid; finalize waits for its intra scores
       // (on the side stream in the wavefront form, which computes them there first).
       if (claimed.pyr_pending) check(hipStreamWaitEvent(S, claimed.pyr_done, 0), "hipStreamWaitEvent");
@@ -883,8 +920,29 @@ void launch_batch(const LaunchIn* in, int n) {
       used[i][ri] = slot;
     }
   }
-  // Only one batch of jobs per aux stream use: more than kMaxItems jobs build on S itself.
-  const bool phases_pending = jobs.size() <= static_cast<size_t>(kMaxItems) ? run_builds(jobs, S, e0) : run_builds(jobs, S);
+  CaptureGuard capture;
+  if (use_graph) {
+    // Buffers a launch allocates on first use, allocated now (no allocation inside a capture), and
+    // the waits on earlier downloads.
+    for (int i = 0; i < n; i++) {
+      celv2hip_engine* e = in[i].e;
+      const bool feats = e->pruner || e->want_features;
+      for (int r = 0; r < in[i].nrefs; r++) {
+        for (int l = 0; l < kLevels; l++)
+          if (!e->fields[r][l]) check(hipMalloc(&e->fields[r][l], static_cast<size_t>(e->fcols[l]) * e->frows[l] * sizeof(uint32_t)), "hipMalloc field");
+        if (feats && !e->cost2[r]) check(hipMalloc(&e->cost2[r], static_cast<size_t>(e->fcols[2]) * e->frows[2] * sizeof(uint32_t)), "hipMalloc cost2");
+      }
+      if (feats && in[i].nrefs >= 1 && !e->infer.cap) cnn_alloc(e->infer, e->nsb());
+      if (e->copy_pending) check(hipStreamWaitEvent(S, e->copy_done, 0), "hipStreamWaitEvent"), e->copy_pending = false;
+    }
+    for (auto& it : intra_now) (void)it.first->intra_item(*it.second);
+    check(hipStreamBeginCapture(S, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    capture.s = S;
+  }
+  // Only one batch of jobs per aux stream use: more than kMaxItems jobs build on S itself, and so does
+  // the graph form (one chain of kernels: a branch on another stream makes the graph's replay take
+  // more of the process's hardware queues, which the engines' own streams then share).
+  const bool phases_pending = jobs.size() <= static_cast<size_t>(kMaxItems) && !use_graph ? run_builds(jobs, S, e0) : run_builds(jobs, S);
   run_intra(intra_now, S);
   for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[1], S), "hipEventRecord");
 
@@ -916,7 +974,7 @@ void launch_batch(const LaunchIn* in, int n) {
       if (b.n == kMaxItems) launch_me_level(b, S), b = Batch<MeLevelArgs>();
     }
     if (l == 0) {
-      for (int i = 0; i < n; i++) {
+      for (int i = 0; i < n && !use_graph; i++) {
         check(hipEventRecord(in[i].e->deep_ev, S), "hipEventRecord");
         in[i].e->deep_pending = true;
       }
@@ -1002,9 +1060,10 @@ void launch_batch(const LaunchIn* in, int n) {
     check(hipEventRecord(e->group_done[g], e->aux), "hipEventRecord");
     for (int r = r0; r < r0 + nrows; r++) e->row_group[r] = g;
   };
-  // The previous launch's downloads read rec_dev: this launch's finalize writes it after them.
+  // The previous launch's downloads read rec_dev: this launch's finalize writes it after them (the
+  // graph form waited before its capture).
   for (int i = 0; i < n; i++)
-    if (in[i].e->copy_pending) check(hipStreamWaitEvent(S, in[i].e->copy_done, 0), "hipStreamWaitEvent");
+    if (in[i].e->copy_pending && !use_graph) check(hipStreamWaitEvent(S, in[i].e->copy_done, 0), "hipStreamWaitEvent");
   const bool grouped = n == 1 && !e0->resident;
   if (!grouped) {
     // All rows in one launch per kernel (batches, and resident mode).
@@ -1089,6 +1148,36 @@ void launch_batch(const LaunchIn* in, int n) {
     }
     if (e->timing) check(hipEventRecord(e->ev[3], S), "hipEventRecord");
     if (e->timing) check(hipEventRecord(e->ev[4], S), "hipEventRecord");
+  }
+  if (use_graph) {
+    hipGraph_t g = nullptr;
+    capture.s = nullptr;
+    check(hipStreamEndCapture(S, &g), "hipStreamEndCapture");
+    hipError_t rc = hipErrorUnknown;
+    if (e0->gexec) {
+      hipGraphNode_t bad = nullptr;
+      hipGraphExecUpdateResult why;
+      rc = hipGraphExecUpdate(e0->gexec, g, &bad, &why);
+      if (rc != hipSuccess || why != hipGraphExecUpdateSuccess) {
+        (void)hipGetLastError();
+        (void)hipGraphExecDestroy(e0->gexec);
+        e0->gexec = nullptr;
+        rc = hipErrorUnknown;
+      }
+    }
+    if (rc != hipSuccess) {
+      const hipError_t irc = hipGraphInstantiate(&e0->gexec, g, nullptr, nullptr, 0);
+      if (irc != hipSuccess) (void)hipGraphDestroy(g), e0->gexec = nullptr;
+      check(irc, "hipGraphInstantiate");
+      e0->graph_builds++;
+    }
+    (void)hipGraphDestroy(g);
+    check(hipGraphLaunch(e0->gexec, S), "hipGraphLaunch");
+    e0->graph_launches++;
+    for (int i = 0; i < n; i++) {
+      check(hipEventRecord(in[i].e->deep_ev, S), "hipEventRecord");
+      in[i].e->deep_pending = true;
+    }
   }
   for (int i = 0; i < n; i++) {
     celv2hip_engine* e = in[i].e;

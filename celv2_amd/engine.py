@@ -42,6 +42,8 @@ def lib():
         L.celv2hip_set_row_groups.argtypes = [P, I]
         L.celv2hip_row_groups.argtypes = [P]
         L.celv2hip_set_resident.argtypes = [P, I]
+        L.celv2hip_set_graph.argtypes = [P, I]
+        L.celv2hip_graph_stats.argtypes = [P, ctypes.POINTER(LL), ctypes.POINTER(LL)]
         L.celv2hip_prepare.argtypes = [P, ctypes.POINTER(Luma), LL]
         L.celv2hip_launch.argtypes = [P, ctypes.POINTER(Luma), LL, ctypes.POINTER(Luma), ctypes.POINTER(LL), I, ctypes.c_int64]
         L.celv2hip_launch_batch.argtypes = [ctypes.POINTER(P), I, ctypes.POINTER(Luma), ctypes.POINTER(LL),
@@ -271,6 +273,15 @@ class RawEngine:
 
     def set_resident(self, on):
         self._check(lib().celv2hip_set_resident(self.h, 1 if on else 0), "set_resident")
+
+    def set_graph(self, on):
+        self._check(lib().celv2hip_set_graph(self.h, 1 if on else 0), "set_graph")
+
+    def graph_stats(self):
+        """(launches replayed as graphs, of them the ones that instantiated a graph)."""
+        a, b = LL(0), LL(0)
+        self._check(lib().celv2hip_graph_stats(self.h, ctypes.byref(a), ctypes.byref(b)), "graph_stats")
+        return a.value, b.value
 
     def set_row_streams(self, n):
         self._check(lib().celv2hip_set_row_streams(self.h, n), "set_row_streams")

@@ -169,6 +169,14 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
 /* Keeps results on the device: wait_row is not available and no device-to-host copy is queued.
  * on != 0: device-resident mode; 0: normal. */
 int celv2hip_set_resident(celv2hip_engine* e, int on);
+/* on != 0: a launch whose engines are all resident (and not timing stages) is captured as a HIP graph
+ * on the launching engine's stream and replayed; the next launch of the same shape updates that graph
+ * in place instead of instantiating it again. The same kernels and decisions; the host queues one
+ * graph instead of each kernel. No counterpart in the reference (its command batches,
+ * src/vk/vk_engine.cpp:261-268, are recorded anew for every frame). graph_stats: launches replayed as
+ * graphs, and of them the ones that instantiated a graph. */
+int celv2hip_set_graph(celv2hip_engine* e, int on);
+int celv2hip_graph_stats(const celv2hip_engine* e, long long* launches, long long* builds);
 
 /* Per-superblock reference limits for later launches (the pruned form): superblock s (row-major) is
  * decided exactly as by a launch with only its nearest k[s] references (1..7; references beyond

@@ -244,3 +244,53 @@ def test_row_groups_and_their_tuner_leave_the_decisions_alone(gpu):
             x.close()
         for b in bufs:
             b.free()
+
+
+def test_graph_replays_decide_as_eager_launches(gpu):
+    """Resident launches captured as HIP graphs and replayed (celv2hip_set_graph): eight frames in a
+    row, each with one new reference and the next source prepared, alone and three engines batched,
+    every replay's decisions equal to an eager engine's on the same frame, with the eager engines'
+    own launches and synchronisations between the replays; the graph is updated in place once the
+    launches' shape repeats."""
+    sizes = [(352, 288), (202, 122), (160, 96)]
+    clips = [synth.clip_frames(synth.params(1, 40 + i, w, h), 12)[1] for i, (w, h) in enumerate(sizes)]
+    lam = gpu.lambda_q8(125)
+
+    def engines(graph):
+        out = [gpu.RawEngine(w, h) for (w, h) in sizes]
+        for e in out:
+            e.set_resident(True)
+            e.set_graph(graph)
+        return out
+
+    alone_g, batch_g, eager = engines(True), engines(True), engines(False)
+    try:
+        for k in range(3):
+            for e in (alone_g[k], batch_g[k], eager[k]):
+                e.prepare(gpu.RawEngine.luma(clips[k][3]), 3)
+        for f in range(3, 11):
+            refs = [f - 1, f - 2, f - 3]
+            srcs = [gpu.RawEngine.luma(c[f]) for c in clips]
+            rl = [[gpu.RawEngine.luma(c[r]) for r in refs] for c in clips]
+            want = []
+            for k, e in enumerate(eager):
+                e.launch(srcs[k], rl[k], lam, keys=refs, src_key=f)
+                e.prepare(gpu.RawEngine.luma(clips[k][f + 1]), f + 1)
+                want.append(e.fetch())
+            for k, e in enumerate(alone_g):
+                e.launch(srcs[k], rl[k], lam, keys=refs, src_key=f)
+                e.prepare(gpu.RawEngine.luma(clips[k][f + 1]), f + 1)
+            gpu.RawEngine.launch_batch(batch_g, srcs, rl, [lam] * 3, keys=[refs] * 3, src_keys=[f] * 3)
+            for k, e in enumerate(batch_g):
+                e.prepare(gpu.RawEngine.luma(clips[k][f + 1]), f + 1)
+            for k in range(3):
+                assert compare_maps(alone_g[k].fetch(), want[k], eager[k].sb_cols) is None, (f, sizes[k], "alone")
+                assert compare_maps(batch_g[k].fetch(), want[k], eager[k].sb_cols) is None, (f, sizes[k], "batched")
+        for e in alone_g:
+            launches, builds = e.graph_stats()
+            assert launches == 8 and builds <= 3, (launches, builds)
+        assert batch_g[0].graph_stats()[0] == 8 and batch_g[0].graph_stats()[1] <= 3
+        assert eager[0].graph_stats() == (0, 0)
+    finally:
+        for e in alone_g + batch_g + eager:
+            e.close()
