@@ -5,11 +5,13 @@
 //   1. the six candidate centres' SADs, 8 lanes per centre (one block row each), summed by DPP;
 //      the centre ranking is then the reference's sequential rule, wave-uniform (scalar code);
 //   2. each searched centre's 24 x 24 window is read from the plane once (144 coalesced word pairs)
-//      into the wave's LDS area as 4 byte-shifted copies; lane = (dx, third of the dy range), 51
-//      lanes, takes its 13 rows as aligned words of the copy its offset selects and scores its 5-6
-//      positions with v_sad_u8 against the block held in SGPRs (scalar loads), the distance
-//      penalty folded into the first v_sad_u8's accumulator; the first least cost in dy-major order
-//      is the minimum of the key cost << 9 | position index (DPP + 4 readlanes);
+//      into the wave's LDS area as 4 byte-shifted copies; both windows' 2 x 289 positions are then
+//      scored in one pass by all 64 lanes: lane (window, half of the dy range, dx < 16) takes 16 rows
+//      as aligned words of the copy its offset selects and scores 9 positions with v_sad_u8 against
+//      the block held in SGPRs (scalar loads); lanes 0..33 also score the two windows' column 16, one
+//      position each. A position's key is (SAD << 10) + (penalty << 10 | window << 9 | dy * 17 + dx),
+//      its lane constant from a compile-time table, so the wave minimum (DPP + 4 readlanes) is the
+//      first least cost in dy-major order, window 1 before window 2 at equal cost;
 //   3. level 0: the half-pel then quarter-pel refinement: per step the 9 positions x 4 tiles are 36
 //      columns of three 16x16x16 Hadamard products on the matrix cores, accumulated onto the block's
 //      own transform (hadamard.h); the key (cost << 4 | order) makes the centre win ties as in
@@ -25,13 +27,23 @@ namespace celv2hip {
 namespace {
 
 constexpr int kWin = 2 * kRange + 8;  // 24: a search window's side
-// A window is staged as 4 copies of its rows, copy s holding the aligned row's bytes from s on, so a
-// scoring lane reads its 8 samples at any byte offset o (0..19) as 2 aligned words of copy o & 3.
-constexpr int kCopyWords = 6;  // words per copied row: words 0..5 cover offsets up to 19 + 8
-// 25 rows: the last dy group's sixth position (past the range, its key forced to the maximum) reads
-// row 24, which stays inside the array.
-constexpr int kCopyLds = (kWin + 1) * kCopyWords;  // one copy
-constexpr int kWinLds = 4 * kCopyLds;
+// A window is staged as 4 copies of its 24 rows, copy s holding the aligned row's bytes from s on, so
+// a scoring lane reads its 8 samples at any byte offset o (0..19) as 2 aligned words of copy o & 3.
+// The copies are interleaved by word: word q of copy s of row r at r * kRowWords + 4 q + s, so the
+// lane's two words sit at o and o + 4 from its row's start, and 16 lanes with consecutive offsets
+// read 16 consecutive words (16 LDS banks). Rows are an odd number of words apart, so lanes reading
+// one column down 17 rows meet 17 different banks; the second window starts 16 banks after the
+// first, so the two windows' 16-lane groups of one half-wave read disjoint banks (up to 3 shared when
+// the first window's byte shift is the larger).
+constexpr int kCopyWords = 6;               // words per copied row: words 0..5 cover offsets up to 19 + 8
+constexpr int kRowWords = 4 * kCopyWords + 1;  // 25
+constexpr int kWinWords = kWin * kRowWords;    // 600
+constexpr int kWinB = 624;                     // the second window's start: 624 = 16 (mod 32)
+// A wave's area: both windows; a position past the range (key forced to the maximum) reads row 24,
+// past its window's rows: inside the gap before the second window, or up to 24 words into the next
+// wave's area (the workgroup's array has that much to spare after the last wave's).
+constexpr int kWaveLds = kWinB + kWinWords;    // 1224
+constexpr int kLdsSpare = 32;
 
 // Stages a window: item k = lane + 64 i (k < 24 x 6) is word d of row r of every copy, from the
 // aligned row's words d and d + 1 (rows from the window's first byte rounded down).
@@ -43,34 +55,61 @@ __device__ __forceinline__ void stage_window(const BPlane& rb, int X, int Y, con
     if (kdst[i] >= 0) {
       uint32_t lo, hi;
       bload8(rb, base + koff[i], 0, lo, hi);
-      win[kdst[i]] = lo;
-      win[kCopyLds + kdst[i]] = __builtin_amdgcn_alignbyte(hi, lo, 1);
-      win[2 * kCopyLds + kdst[i]] = __builtin_amdgcn_alignbyte(hi, lo, 2);
-      win[3 * kCopyLds + kdst[i]] = __builtin_amdgcn_alignbyte(hi, lo, 3);
+      uint32_t* o = win + kdst[i];
+      o[0] = lo;
+      o[1] = __builtin_amdgcn_alignbyte(hi, lo, 1);
+      o[2] = __builtin_amdgcn_alignbyte(hi, lo, 2);
+      o[3] = __builtin_amdgcn_alignbyte(hi, lo, 3);
     }
 }
 
-// The lane's best key over its positions from a staged window (byte shift sh of the window's first
-// sample): pen[k] / idx[k] hold each position's penalty and index (kill5: the lane has 5 positions).
-__device__ __forceinline__ uint32_t score_window(const uint32_t* win, int sh, int dxi, int dy0, const uint32_t c[16],
-                                                 const uint32_t pen[6], const uint32_t idx[6], uint32_t kill5) {
-  const int o = sh + dxi;
-  const uint32_t* copy = win + (o & 3) * kCopyLds + (o >> 2);
-  uint32_t w[13][2];
+// Lane constants of the search, computed per block (a table of them cost more in loads than the
+// arithmetic: the vector-memory path is the busier one):
+//   t[k], k < 9: the keys of the lane's main positions (dy group g = lane >> 5, window w =
+//     (lane >> 4) & 1, dx = lane & 15, dy = 9 g + k): (kPelPenalty (|dx - 8| + |dy - 8|)) << 10 |
+//     w << 9 | dy * 17 + dx, or kNoPosition past the range (dy 17). Along k the penalty falls by
+//     kPelPenalty while dy < 8 (group 0) and rises after (group 1): each key is the last plus a step;
+//   t[9]: the leftover position of lanes 0..33 (dx 16, window lane >= 17, dy lane % 17), else none;
+//   t[10..14]: dx, w, the first row's word offset of the main positions; w and the first row's word
+//     offset of the leftover one.
+constexpr uint32_t kNoPosition = 0x7fffffffu;  // above any key (SAD << 10 < 2^24)
+__device__ __forceinline__ void search_lanes(int lane, uint32_t (&t)[16]) {
+  const int g = lane >> 5, w = (lane >> 4) & 1, dx = lane & 15;
+  const uint32_t pen0 = static_cast<uint32_t>(kPelPenalty * (abs(dx - kRange) + (g ? 1 : kRange)));
+  t[0] = pen0 << 10 | static_cast<uint32_t>(w) << 9 | static_cast<uint32_t>(153 * g + dx);
+  const uint32_t step = g ? (static_cast<uint32_t>(kPelPenalty) << 10) + 17u : 17u - (static_cast<uint32_t>(kPelPenalty) << 10);
 #pragma unroll
-  for (int rr = 0; rr < 13; rr++) {
-    const uint32_t* row = copy + (dy0 + rr) * kCopyWords;
-    w[rr][0] = row[0], w[rr][1] = row[1];
-  }
-  uint32_t best = 0xffffffffu;
+  for (int k = 1; k < 9; k++) t[k] = t[k - 1] + step;
+  if (g) t[8] = kNoPosition;
+  const int lw = lane >= 17 ? 1 : 0, ldy = lane - 17 * lw;
+  const uint32_t lpen = static_cast<uint32_t>(kPelPenalty * (kRange + abs(ldy - kRange)));
+  t[9] = lane < 34 ? (lpen << 10 | static_cast<uint32_t>(lw) << 9 | static_cast<uint32_t>(17 * ldy + 16)) : kNoPosition;
+  t[10] = static_cast<uint32_t>(dx), t[11] = static_cast<uint32_t>(w), t[12] = static_cast<uint32_t>(9 * kRowWords * g);
+  t[13] = static_cast<uint32_t>(lane < 34 ? lw : 0), t[14] = static_cast<uint32_t>(lane < 34 ? kRowWords * ldy : 0), t[15] = 0;
+}
+
+// The lane's least key over its positions of the two staged windows A and B (LDS word addresses,
+// byte shifts shA / shB of their first samples; B may be A).
+__device__ __forceinline__ uint32_t score_pair(const uint32_t (&t)[16], const uint32_t* winA, const uint32_t* winB, int shA, int shB,
+                                               const uint32_t c[16], int lane) {
+  const uint32_t* row = (t[11] ? winB : winA) + ((t[11] ? shB : shA) + static_cast<int>(t[10])) + t[12];
+  uint32_t w[16][2];
 #pragma unroll
-  for (int k = 0; k < 6; k++) {
-    uint32_t acc = pen[k];
+  for (int rr = 0; rr < 16; rr++) w[rr][0] = row[rr * kRowWords], w[rr][1] = row[rr * kRowWords + 4];
+  uint32_t best = kNoPosition;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    uint32_t acc = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) acc = sad4(c[2 * i + 1], w[k + i][1], sad4(c[2 * i], w[k + i][0], acc));
-    uint32_t key = (acc << 9) | idx[k];
-    if (k == 5) key |= kill5;
-    best = min(best, key);
+    best = min(best, (acc << 10) + t[k]);
+  }
+  if (lane < 34) {  // column 16 of both windows
+    const uint32_t* lrow = (t[13] ? winB : winA) + ((t[13] ? shB : shA) + 16) + t[14];
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc = sad4(c[2 * i + 1], lrow[i * kRowWords + 4], sad4(c[2 * i], lrow[i * kRowWords], acc));
+    best = min(best, (acc << 10) + t[9]);
   }
   return best;
 }
@@ -184,12 +223,14 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   }
 
   // Full searches (me_search, src/gpu/inter_common.h:106-119) from windows staged in LDS.
+  // Stage item k = lane + 64 i is word d of row r of a window (r = k / 6 = (171 k) >> 10 for k < 192,
+  // d = k - 6 r); item 2 exists for lanes < 16 only.
   int koff[3], kdst[3];
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    const int k = lane + 64 * i, row = k / kCopyWords, d = k - row * kCopyWords;
-    koff[i] = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(row))) + 4 * d;
-    kdst[i] = k < kWin * kCopyWords ? row * kCopyWords + d : -1;
+    const int k = lane + 64 * i, r = (171 * k) >> 10, d = k - kCopyWords * r;
+    koff[i] = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(r))) + 4 * d;
+    kdst[i] = i < 2 || lane < kWin * kCopyWords - 128 ? r * kRowWords + 4 * d : -1;
   }
   const int X1 = clamp_origin_uniform(8 * bx + p1x - kRange, ref.w, kWin), Y1 = clamp_origin_uniform(8 * by + p1y - kRange, ref.h, kWin);
   const int X2 = clamp_origin_uniform(8 * bx + p2x - kRange, ref.w, kWin), Y2 = clamp_origin_uniform(8 * by + p2y - kRange, ref.h, kWin);
@@ -199,31 +240,16 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
-  const bool act = lane < 51;
-  const int li = act ? lane : 0;
-  const int dxi = li % 17, g3 = li / 17, dy0 = 6 * g3;
-  uint32_t pen[6], idx[6];
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    pen[k] = kPelPenalty * (abs(dxi - kRange) + abs(dy0 + k - kRange));
-    idx[k] = static_cast<uint32_t>((dy0 + k) * 17 + dxi);
-  }
-  const uint32_t kill5 = g3 == 2 ? 0xffffffffu : 0u;
-  uint32_t best_cost;
   // A window's first sample sits at byte X & 3 of its aligned row (rows are a multiple of 64 bytes
-  // apart, the border 64 samples wide).
-  const uint32_t k1 = wave_min_u32(act ? score_window(win1, X1 & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
-  const int i1 = static_cast<int>(k1 & 511);
-  int mvx = p1x + i1 % 17 - kRange, mvy = p1y + i1 / 17 - kRange;
-  best_cost = k1 >> 9;
-  if (two) {
-    const uint32_t k2 = wave_min_u32(act ? score_window(win2, X2 & 3, dxi, dy0, c, pen, idx, kill5) : 0xffffffffu);
-    if ((k2 >> 9) < (k1 >> 9)) {
-      best_cost = k2 >> 9;
-      const int i2 = static_cast<int>(k2 & 511);
-      mvx = p2x + i2 % 17 - kRange, mvy = p2y + i2 / 17 - kRange;
-    }
-  }
+  // apart, the border 64 samples wide). With one centre, the second window's lanes score the first
+  // window again (its keys lose every tie to the first window's).
+  uint32_t tk[16];
+  search_lanes(lane, tk);
+  const uint32_t key = wave_min_u32(score_pair(tk, win1, two ? win2 : win1, X1 & 3, two ? X2 & 3 : X1 & 3, c, lane));
+  const uint32_t best_cost = key >> 10;
+  const int ip = static_cast<int>(key & 511);
+  const bool second = ((key >> 9) & 1) != 0;
+  int mvx = (second ? p2x : p1x) + ip % 17 - kRange, mvy = (second ? p2y : p1y) + ip / 17 - kRange;
 
   if (a.level0) {
     // me_refine (src/gpu/inter_common.h:135-154) from the integer MV, in quarter pels. Column
@@ -312,9 +338,11 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   }
 }
 
-// One workgroup per unit: 4 blocks of one reference, one per wave.
+// One workgroup per unit: 4 blocks of one reference, one per wave. kOne: the batch has one item (its
+// fields then sit at constant kernel-argument offsets: no per-field address arithmetic).
+template <bool kOne>
 __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
-  __shared__ uint32_t wins[4][2][kWinLds];
+  __shared__ uint32_t wins[4 * kWaveLds + kLdsSpare];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addressing
   const int total = batch.total();
@@ -323,13 +351,13 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
   // Units are (group of 4 consecutive blocks of a reference, reference), the reference cycling
   // fastest: the XCD-major numbering then gives every XCD all references of its blocks, however the
   // reference limits thin out the far ones, and a workgroup's 4 waves search neighbouring blocks.
-  const int bi = batch_item(batch, g);
+  const int bi = kOne ? 0 : batch_item(batch, g);
   const MeLevelArgs& a = batch.item[bi];
   const int u = g - batch.first[bi], grp = div_small(u, a.nrefs, a.inv_nrefs), r = u - grp * a.nrefs;
   const int block = a.row0 * a.cols + 4 * grp + wave;
   if (block >= (a.row0 + a.nrun) * a.cols) return;  // whole wave
   const int by = div_small(block, a.cols, a.inv_cols);
-  search_block(a, BlockPos{r, block - by * a.cols, by}, lane, wins[wave][0], wins[wave][1]);
+  search_block(a, BlockPos{r, block - by * a.cols, by}, lane, wins + wave * kWaveLds, wins + wave * kWaveLds + kWinB);
 }
 
 }  // namespace
@@ -345,7 +373,10 @@ void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a) {
 }
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s) {
   if (b.total() <= 0) return;
-  k_me_level<<<((b.total() + 7) / 8) * 8, 256, 0, s>>>(b);
+  if (b.n == 1)
+    k_me_level<true><<<((b.total() + 7) / 8) * 8, 256, 0, s>>>(b);
+  else
+    k_me_level<false><<<((b.total() + 7) / 8) * 8, 256, 0, s>>>(b);
 }
 
 }  // namespace celv2hip
