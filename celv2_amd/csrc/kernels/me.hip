@@ -206,19 +206,24 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     const uint32_t c0 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row, 64));
     const uint32_t c1 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row + 1, 64));
     const uint32_t s = oct_sum(sad4(c1, w1, sad4(c0, w0, 0)));
-    // me_cell's ranking (src/gpu/inter_common.h:164-186): each distinct centre once, best and runner-up.
-    int s1 = 0x7fffffff, s2 = 0x7fffffff;
+    // me_cell's ranking (src/gpu/inter_common.h:164-186) walks the centres in order, skipping one equal
+    // to the current best or runner-up, and keeps the best and the runner-up, each the first strict
+    // minimum. A centre equal to an earlier one that is neither has that one's SAD, which lost to
+    // both, so it changes nothing either: the two are the two least (SAD, k) over the centres that
+    // differ from every earlier one. Here in the lanes: a group's key SAD << 3 | k, or none when an
+    // earlier centre is the same, and two wave minima.
+    bool dup = false;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      const bool dup = (k > 0 && cx[k] == p1x && cy[k] == p1y) || (two && cx[k] == p2x && cy[k] == p2y);
-      if (dup) continue;
-      const int sk = __builtin_amdgcn_readlane(static_cast<int>(s), 8 * k);
-      if (sk < s1) {
-        if (k > 0) s2 = s1, p2x = p1x, p2y = p1y, two = true;
-        s1 = sk, p1x = cx[k], p1y = cy[k];
-      } else if (sk < s2) {
-        s2 = sk, p2x = cx[k], p2y = cy[k], two = true;
-      }
+    for (int j = 0; j < 5; j++) dup = dup || (j < kk && cx[j] == mx && cy[j] == my);
+    const uint32_t key = dup ? 0xffffffffu : (s << 3 | static_cast<uint32_t>(kk));
+    const uint32_t k1 = wave_min_u32(key);
+    const uint32_t k2 = wave_min_u32(key == k1 ? 0xffffffffu : key);
+    const int l1 = 8 * static_cast<int>(k1 & 7);
+    p1x = __builtin_amdgcn_readlane(mx, l1), p1y = __builtin_amdgcn_readlane(my, l1);
+    two = k2 != 0xffffffffu;
+    if (two) {
+      const int l2 = 8 * static_cast<int>(k2 & 7);
+      p2x = __builtin_amdgcn_readlane(mx, l2), p2y = __builtin_amdgcn_readlane(my, l2);
     }
   }
 
