@@ -39,7 +39,10 @@ SAD_OPS, SATD_OPS = 3, 7  # int ops per pixel (sub, abs, add; diff, 2 butterfly 
 SAD_PEAK_TPX, SAD_MEASURED_TPX = 256 * 64 * 4 * 2.4e9 / 1e12, 125.4
 # HBM bytes the five motion-search launches of one frame move (rocprofv3 FETCH_SIZE and WRITE_SIZE
 # per dispatch, scaled by a 64 MiB device copy: profiles/me_traffic_1080p10_7refs.md).
-ME_TRAFFIC_BYTES = 351.2e6
+ME_TRAFFIC_BYTES = 366.7e6
+# HBM bytes of one forward-transform call over a 1080p frame's candidate blocks (FETCH_SIZE + WRITE_SIZE,
+# scaled the same way: profiles/me_traffic_1080p10_7refs.md).
+TX_TRAFFIC_BYTES = 142.0e6
 
 
 def log(*a):
@@ -202,7 +205,7 @@ def bench_transforms(y8, frame, steps, warmup, cpu_seconds=0.0):
            "ms_per_call": round(wall * 1e3, 4), "kernel_ms": round(kernel_s * 1e3, 4),
            "kernel_jobs_per_s": round(n / kernel_s),
            "roofline": {"bound": "hbm", "kernel": "k_fwd_tx", "achieved": round(alg / kernel_s / 1e9, 1), "peak": 8000.0,
-                        "unit": "GB/s", "frac": round(alg / kernel_s / 8e12, 4), "traffic": None,
+                        "unit": "GB/s", "frac": round(alg / kernel_s / 8e12, 4), "traffic": TX_TRAFFIC_BYTES,
                         "algorithmic_bytes_per_call": alg,
                         "bytes": {"residuals": res_bytes, "coefficients_and_dropped": out_bytes, "job_records": list_bytes}}}
     if cpu_seconds > 0:
