@@ -670,6 +670,44 @@ def main():
         E.set_graph(False)
         log(f"graph form: {fps_g:.1f} frames/s, host {cpu_g:.3f} ms a frame; eager {fps_e:.1f} frames/s, host {cpu_e:.3f} ms")
 
+    # The intra dimension of the pruned form at its extreme: every superblock scoring DC, V and H only
+    # (celv2hip_set_intra_masks), against every mode: frames/s of the resident steps, the intra stage,
+    # and the decision cost it adds (sum of the superblocks' costs over 4 frames).
+    intra_groups = None
+    if not args.no_learned:
+        def cost_sum(masks):
+            E.set_intra_masks(masks)
+            tot = 0
+            for f_ in decidable[-4:]:
+                refs = [f_ - k for k in range(1, NR + 1)]
+                E.launch(lum[f_], [lum[r] for r in refs], lam, keys=refs)
+                E.sync()
+                tot += int(E.fetch_sb_costs().sum())
+            return tot
+        nsb = E.sb_cols * E.sb_rows
+        out_ig = {}
+        for name, m in (("all_groups", None), ("dc_v_h_only", np.zeros(nsb, np.uint8))):
+            E.set_intra_masks(m)
+            E.prepare(lum[decidable[0]], decidable[0])
+            b0 = args.warmup + 2 * args.steps + 8
+            for s_ in range(b0, b0 + args.warmup):
+                step(s_)
+            E.sync()
+            t0 = time.perf_counter()
+            for s_ in range(b0 + args.warmup, b0 + args.warmup + args.steps):
+                step(s_)
+            E.sync()
+            fps_m = args.steps / (time.perf_counter() - t0)
+            st = stage_times(E, step, b0 + args.warmup + args.steps, args.steps)
+            out_ig[name] = {"frames_per_s": round(fps_m, 2), "intra_ms_per_frame": round(st["intra"], 4)}
+        c_all, c_g0 = cost_sum(None), cost_sum(np.zeros(nsb, np.uint8))
+        E.set_intra_masks(None)
+        intra_groups = {**out_ig, "cost_regret": round((c_g0 - c_all) / c_all, 5),
+                        "what": "every superblock restricted to DC, V and H against every intra mode, resident steps"}
+        log(f"intra groups: DC/V/H only {out_ig['dc_v_h_only']['frames_per_s']} frames/s (intra "
+            f"{out_ig['dc_v_h_only']['intra_ms_per_frame']} ms) against {out_ig['all_groups']['frames_per_s']} "
+            f"({out_ig['all_groups']['intra_ms_per_frame']} ms), cost +{100 * intra_groups['cost_regret']:.3f} %")
+
     # Algorithmic integer work per frame (weights of the reference's cost rules, counted as executed).
     sbs = E.sb_cols * E.sb_rows
     cells0 = ((W + 7) // 8) * ((H + 7) // 8)
@@ -849,7 +887,7 @@ def main():
         "forms": {"plain": {"frames_per_s": round(fps_b, 2), "kernel_only_frames_per_s": round(fps, 2),
                             "scored_candidates_per_s": round(cands * fps_b), "roofline": me_roofline(cnt["me_centres"] / n, per["me"])},
                   "batched": batched, "learned": learned, "learned_batched_cel_4k": cel, "gop_order": gop_form,
-                  "transforms": transforms, "graph": graph_form},
+                  "transforms": transforms, "graph": graph_form, "intra_groups": intra_groups},
     }
     print(json.dumps(out), flush=True)
 

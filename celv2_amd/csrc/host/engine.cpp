@@ -98,6 +98,7 @@ struct SrcSlot {
   Pyramid p;
   Upload up;
   uint32_t* intra = nullptr;
+  uint8_t* imask_used = nullptr;  // the intra mode groups `intra` was made with, one byte per superblock
   // prepare() fills a slot on the side stream while launches run on the main stream: each stream
   // waits for the other's last work on the slot before touching it.
   hipEvent_t side_done = nullptr, main_done = nullptr;
@@ -162,6 +163,8 @@ struct celv2hip_engine {
   uint2* best = nullptr;
   uint8_t* kmap = nullptr;       // per-superblock reference limits on the device (when kmap_on)
   bool kmap_on = false;
+  uint8_t* imask = nullptr;      // per-superblock intra mode groups on the device (when imask_on)
+  bool imask_on = false;
   int64_t* sb_cost = nullptr;    // per-superblock decision cost of the last launch
   // The learned pruner (celv2hip_set_pruner) and the per-superblock features it reads.
   const celv2hip_pruner* pruner = nullptr;
@@ -210,6 +213,7 @@ struct celv2hip_engine {
     for (SrcSlot& s : srcs) {
       if (s.p.mem) (void)hipFree(s.p.mem);
       if (s.intra) (void)hipFree(s.intra);
+      if (s.imask_used) (void)hipFree(s.imask_used);
       if (s.side_done) (void)hipEventDestroy(s.side_done);
       if (s.main_done) (void)hipEventDestroy(s.main_done);
       if (s.pyr_done) (void)hipEventDestroy(s.pyr_done);
@@ -224,6 +228,7 @@ struct celv2hip_engine {
         if (f) (void)hipFree(f);
     if (best) (void)hipFree(best);
     if (kmap) (void)hipFree(kmap);
+    if (imask) (void)hipFree(imask);
     for (uint32_t* c : cost2)
       if (c) (void)hipFree(c);
     cnn_free(infer);
@@ -314,7 +319,8 @@ struct celv2hip_engine {
 
   IntraItem intra_item(SrcSlot& s) {
     if (!s.intra) check(hipMalloc(&s.intra, nsb() * kBlocks * kModes * sizeof(uint32_t)), "hipMalloc intra");
-    return IntraItem{s.p.lv[0], sb_cols, 0, sb_rows, s.intra};
+    if (!s.imask_used) check(hipMalloc(&s.imask_used, nsb()), "hipMalloc intra masks");
+    return IntraItem{s.p.lv[0], sb_cols, 0, sb_rows, s.intra, imask_on ? imask : nullptr, s.imask_used};
   }
 
   // Orders stream st's coming work on slot s after the other stream's last work on it (the side
@@ -512,6 +518,23 @@ int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count) {
     e->sync_all();
     check(hipMemcpy(e->kmap, k, static_cast<size_t>(count), hipMemcpyHostToDevice), "hipMemcpy kmap");
     e->kmap_on = true;
+  });
+}
+
+int celv2hip_set_intra_masks(celv2hip_engine* e, const uint8_t* masks, int count) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    if (!masks || count == 0) {
+      e->imask_on = false;
+      return;
+    }
+    if (count != static_cast<int>(e->nsb())) throw Failure(CELV2_ERR_SIZE, "one mask per superblock");
+    for (int i = 0; i < count; i++)
+      if (masks[i] > kAllGroups) throw Failure(CELV2_ERR_ARG, "intra masks are 4-bit group sets");
+    e->sync_all();
+    if (!e->imask) check(hipMalloc(&e->imask, e->nsb()), "hipMalloc intra masks");
+    check(hipMemcpy(e->imask, masks, static_cast<size_t>(count), hipMemcpyHostToDevice), "hipMemcpy intra masks");
+    e->imask_on = true;
   });
 }
 
@@ -1033,6 +1056,7 @@ void launch_batch(const LaunchIn* in, int n) {
     fa.phase_bytes = in[i].nrefs ? used[i][0]->p.phase_bytes : 0;
     fa.best = e->best, fa.intra = cur[i]->intra, fa.out = e->rec_dev, fa.counter = e->counters + kCounterWords;
     fa.kmap = use_kmap[i] ? e->kmap : nullptr, fa.sb_cost = e->sb_cost;
+    fa.imask = cur[i]->imask_used;
     for (int r = 0; r < in[i].nrefs; r++) fa.ref[r] = used[i][r]->p.lv[0];
     return fa;
   };

@@ -142,6 +142,15 @@ __device__ __forceinline__ void block_geom(int b, int& x, int& y, int& w, int& h
 
 __device__ __forceinline__ int mode_bits(int m) { return m == 0 ? 1 : (m < 4 ? 3 : 9); }  // intra_common.h:32
 
+// Intra mode groups of the pruned form: 0 = DC, V, H (always scored); 1 = smooth, smooth V, smooth H,
+// Paeth; 2 = the directional modes at 45, 67 and 113 degrees; 3 = at 135, 157 and 203 degrees. A
+// superblock's mask has bit g set when group g is scored; a block's intra choice is then the least
+// cost over the scored modes.
+constexpr int kIntraGroups = 4;
+constexpr uint8_t kAllGroups = 0xF;
+__host__ __device__ __forceinline__ int mode_group(int m) { return m < 3 ? 0 : m < 7 ? 1 : m < 10 ? 2 : 3; }
+__host__ __device__ __forceinline__ bool mode_scored(uint32_t mask, int m) { return ((mask | 1u) >> mode_group(m)) & 1u; }
+
 // Cross-lane moves within 16-lane rows (DPP, no LDS traffic): quad_perm [1,0,3,2] = lane ^ 1,
 // quad_perm [2,3,0,1] = lane ^ 2, row_ror:8 = lane ^ 8 (within a row), row_half_mirror maps lane i of
 // each 8 to 7 - i (the other quad of the 8, for values uniform within quads), row_ror:4 = lane + 4

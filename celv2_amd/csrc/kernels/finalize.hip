@@ -159,7 +159,9 @@ __global__ void __launch_bounds__(kThreads) k_finalize(Batch<FinalizeArgs> batch
     int m = 0;
     if (b >= 5) {  // blocks above 32x32 are inter only (intra_common.h:39)
       const uint32_t* is = a.intra + (sb * kBlocks + b) * kModes;
+      const uint32_t gm = a.imask ? a.imask[sb] : kAllGroups;
       for (int k = 0; k < kModes; k++) {
+        if (!mode_scored(gm, k)) continue;
         const int64_t ck = static_cast<int64_t>(is[k]) * 256 + lam * (mode_bits(k) + (n > 0 ? kIntraOnInter : 0));
         if (ck < c) c = ck, m = k;
       }

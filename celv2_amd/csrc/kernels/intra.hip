@@ -76,6 +76,10 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
   uint32_t* __restrict__ const out = it.out;
   const int ox = 64 * sbx, oy = 64 * sby;
   const int p0 = g_group_pos[q], np = g_group_pos[q + 1] - p0;
+  // The superblock's scored mode groups (uniform over the workgroup); group 0 is always scored.
+  const uint32_t gm = (it.mask ? it.mask[sby * sb_cols + sbx] : kAllGroups) | 1u;
+  const bool g1 = gm & 2u, g2 = gm & 4u, g3 = gm & 8u;
+  if (it.used_mask && q == 0 && tid == 0) it.used_mask[sby * sb_cols + sbx] = static_cast<uint8_t>(gm);
 
   // Stage: region row r (frame row oy + r - 1, clamped), dword d = columns ox - 4 + 4d .. + 3 (a
   // dword entirely right of the frame reads the replicated border at w + 60).
@@ -165,9 +169,11 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
       emit(0, sat(pd, pd), sat(halves01(a4), halves23(a4)));
       const uint32_t ph = (static_cast<uint32_t>(lft) | 0x6400u) * 0x10001u;
       const float s2 = sat(ph, ph);
+      // Modes of a group that is not scored are not computed (0 is left in their place; finalize
+      // skips them), and a pair of them is not summed.
       // 3 Smooth: (h-i) a[j] + (i+1) bl + (w-j) l[i] + (j+1) tr, rounded, / (w + h + 2).
-      float s3;
-      {
+      float s3 = 0.f;
+      if (g1) {
         const int base = (pi + 1) * bl + (w - pj0) * lft + (pj0 + 1) * tr + sh.den / 2, dl = tr - lft, al = h - pi;
         uint32_t v[4];
 #pragma unroll
@@ -176,8 +182,8 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
       }
       emit(2, s2, s3);
       // 4 Smooth V: ((h-1-i) a[j] + (i+1) left[h-1] + h/2) / h.
-      float s4;
-      {
+      float s4 = 0.f;
+      if (g1) {
         const int c = (pi + 1) * bl + h / 2 + (0x6400 << sh.lh), al = h - 1 - pi;
         uint32_t v[4];
 #pragma unroll
@@ -185,18 +191,18 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
         s4 = sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
       }
       // 5 Smooth H: ((w-1-j) l[i] + (j+1) above[w-1] + w/2) / w.
-      float s5;
-      {
+      float s5 = 0.f;
+      if (g1) {
         const int c = (w - 1 - pj0) * lft + (pj0 + 1) * tr + w / 2 + (0x6400 << sh.lw), dl = tr - lft;
         uint32_t v[4];
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) v[jj] = static_cast<uint32_t>((c + jj * dl) >> sh.lw);
         s5 = sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
       }
-      emit(4, s4, s5);
+      if (g1) emit(4, s4, s5);
       // 6 Paeth: the neighbour nearest to above + left - top-left, ties to left, then above.
-      float s6;
-      {
+      float s6 = 0.f;
+      if (g1) {
         const int pa = abs(lft - tl);
         uint32_t v[4];
 #pragma unroll
@@ -225,10 +231,12 @@ __global__ void __launch_bounds__(256) k_intra(Batch<IntraItem> batch) {
         }
         return sat(pack_lo16(v[0], v[1]), pack_lo16(v[2], v[3]));
       };
-      emit(6, s6, dir(0));
-      emit(8, dir(1), dir(2));
-      emit(10, dir(3), dir(4));
-      emit(12, dir(5), 0.f);
+      if (g1 || g2) emit(6, s6, g2 ? dir(0) : 0.f);
+      if (g2) emit(8, dir(1), dir(2));
+      if (g3) {
+        emit(10, dir(3), dir(4));
+        emit(12, dir(5), 0.f);
+      }
     };
     switch (sh.seg) {
       case 16: body(std::integral_constant<int, 16>{}); break;

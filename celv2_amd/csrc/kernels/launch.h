@@ -60,6 +60,10 @@ struct IntraItem {
   DPlane src;
   int sb_cols, row0, nrows;
   uint32_t* out;
+  // Intra mode groups per superblock (may be null: every group); the masks used are written to
+  // used_mask (one byte per superblock), which finalize then obeys.
+  const uint8_t* mask;
+  uint8_t* used_mask;
 };
 void add_intra(Batch<IntraItem>& b, const IntraItem& t);
 void launch_intra(const Batch<IntraItem>& b, hipStream_t s);
@@ -93,6 +97,7 @@ struct FinalizeArgs {
   void* out;               // celv2_sb records, one per superblock of the frame
   unsigned long long* counter;  // compound blocks scored, a counter of 64 slots (may be null)
   const uint8_t* kmap;     // per-superblock reference limits (may be null): n = min(nrefs, kmap[sb])
+  const uint8_t* imask;    // the intra mode groups the source's intra scores were made with (k_intra's used_mask)
   int64_t* sb_cost;        // the superblock's decision cost (the 64x64 square's DP cost) (may be null)
 };
 void add_finalize(Batch<FinalizeArgs>& b, const FinalizeArgs& a);

@@ -59,6 +59,7 @@ def lib():
         L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_set_ref_limits.argtypes = [P, P, I]
+        L.celv2hip_set_intra_masks.argtypes = [P, P, I]
         L.celv2hip_pruner_create.argtypes = [I, I, ctypes.c_uint, ctypes.POINTER(P)]
         L.celv2hip_pruner_destroy.argtypes = [P]
         L.celv2hip_pruner_get_params.argtypes = [P, P, I]
@@ -373,6 +374,15 @@ class RawEngine:
             return
         k = np.ascontiguousarray(k, dtype=np.uint8)
         self._check(lib().celv2hip_set_ref_limits(self.h, k.ctypes.data, int(k.size)), "set_ref_limits")
+
+    def set_intra_masks(self, m=None):
+        """Per-superblock intra mode groups (uint8 bit sets, 0..15, row-major) for the sources scored
+        from now on; None: every group."""
+        if m is None:
+            self._check(lib().celv2hip_set_intra_masks(self.h, None, 0), "set_intra_masks")
+            return
+        m = np.ascontiguousarray(m, dtype=np.uint8)
+        self._check(lib().celv2hip_set_intra_masks(self.h, m.ctypes.data, int(m.size)), "set_intra_masks")
 
     def fetch_sb_costs(self):
         out = np.zeros(self.sb_cols * self.sb_rows, np.int64)

@@ -184,6 +184,14 @@ int celv2hip_graph_stats(const celv2hip_engine* e, long long* launches, long lon
  * superblocks of the frame, or k NULL / count 0 to decide every superblock with all references. No
  * counterpart in the reference's DecisionEngine: the hook of the learned pruner. */
 int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count);
+/* Per-superblock intra mode groups for the sources scored from now on (the next prepare() or
+ * launch): bit g of masks[s] set = group g scored (0: DC, V, H, always scored; 1: smooth, smooth V,
+ * smooth H, Paeth; 2: the directional modes at 45, 67, 113 degrees; 3: at 135, 157, 203 degrees).
+ * Every block of superblock s takes the least intra cost over its scored modes, as the reference's
+ * choice (src/gpu/kernels_ref.cpp:46-50) restricted to them; the other modes' scores are not
+ * computed. A source keeps the masks it was scored with. masks NULL / count 0: every group. No
+ * counterpart in the reference: the intra dimension of the pruned form. */
+int celv2hip_set_intra_masks(celv2hip_engine* e, const uint8_t* masks, int count);
 
 /* The last launch's per-superblock decision cost (the 64x64 square's partition DP cost, SATD*256 +
  * lambda*bits units: J of the superblock) into out[0 .. superblocks); returns the count or a negative

@@ -82,6 +82,16 @@ int orc_decide_frame(int w, int h, const uint8_t* src, const uint8_t* const* ref
   }
 }
 
+// The same restricted to per-superblock intra mode groups (bit g of masks[s]: group g scored; group
+// 0, modes 0..2, always): the pruned form's intra dimension.
+int orc_decide_frame_masked(int w, int h, const uint8_t* src, const uint8_t* const* refs, int nrefs, int64_t lam,
+                            int sb_size, const uint8_t* masks, celv2_sb* out) {
+  g_intra_masks = masks, g_intra_mask_cols = (w + sb_size - 1) / sb_size;
+  const int rc = orc_decide_frame(w, h, src, refs, nrefs, lam, sb_size, out);
+  g_intra_masks = nullptr;
+  return rc;
+}
+
 // The threaded CpuEngine. fail_row >= 0: the row hook throws std::runtime_error("row hook") there.
 void* orc_engine_create(int w, int h, int sb_size, int threads, int fail_row) {
   try {

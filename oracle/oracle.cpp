@@ -514,6 +514,21 @@ struct Scores {
   int nrefs = 0;
 };
 
+}  // namespace
+
+// Intra mode groups per superblock for the decision in progress on this thread (the pruned form's
+// restriction; null: every mode): group 0 = modes 0..2 (always), 1 = 3..6, 2 = 7..9, 3 = 10..12.
+thread_local const uint8_t* g_intra_masks = nullptr;
+thread_local int g_intra_mask_cols = 0;
+
+namespace {
+
+bool mode_allowed(int x, int y, int m) {
+  if (!g_intra_masks) return true;
+  const int g = m < 3 ? 0 : m < 7 ? 1 : m < 10 ? 2 : 3;
+  return ((g_intra_masks[(y / 64) * g_intra_mask_cols + x / 64] | 1) >> g) & 1;
+}
+
 // kernels_ref.cpp:41-79 — intra modes in order; each reference's candidates in order (repeats
 // skipped); the best reference, the runner-up and their compound (first strict minimum throughout).
 Scores score_block(const Plane8& s, const RefSet& rs, int x, int y, int w, int h, int64_t lam, bool with_intra) {
@@ -521,6 +536,7 @@ Scores score_block(const Plane8& s, const RefSet& rs, int x, int y, int w, int h
   const int n = rs.size();
   if (with_intra)
     for (int m = 0; m < kModes; m++) {
+      if (!mode_allowed(x, y, m)) continue;
       const int64_t c = intra_satd(s, x, y, w, h, m) * 256 + lam * (mode_bits(m) + (n > 0 ? kIntraOnInter - (ORC_PLANT == 1) : 0));
       if (c < sc.intra.cost) sc.intra = Pick{c, m, 0, 0, 0};
     }

@@ -163,7 +163,11 @@ struct RefSet {
 };
 
 Sb decide_sb(const Luma& src, const RefSet& rs, int sb_col, int sb_row, int sb_size, int64_t lam);  // kernels_ref.cpp:253-270
-Map decide_frame(const Luma& src, const std::vector<const Luma*>& refs, int sb_size, int64_t lam);  // kernels_ref.cpp:200-209, 282-301
+Map decide_frame(const Luma& src, const std::vector<const Luma*>& refs, int sb_size, int64_t lam);
+// The pruned form's intra restriction for decisions made on this thread (test of the engine's intra
+// mode groups; null: every mode): one 4-bit group set per superblock of `cols` columns.
+extern thread_local const uint8_t* g_intra_masks;
+extern thread_local int g_intra_mask_cols;  // kernels_ref.cpp:200-209, 282-301
 
 // One square's bottom-up decision (kernels_ref.cpp:124-198), exposed for the brute-force tests: the
 // cost it returns and the subtree it appends.

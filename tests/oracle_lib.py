@@ -70,6 +70,7 @@ def oracle():
         lib = _prims(ctypes.CDLL(ORACLE_SO), "orc_")
         lib.orc_last_error.restype = ctypes.c_char_p
         lib.orc_decide_frame.argtypes = [I, I, P, P, I, I64, I, P]
+        lib.orc_decide_frame_masked.argtypes = [I, I, P, P, I, I64, I, P, P]
         lib.orc_engine_create.argtypes = [I, I, I, I, I]
         lib.orc_engine_create.restype = P
         lib.orc_engine_destroy.argtypes = [P]
@@ -104,15 +105,22 @@ def ptr_array(arrays):
     return arr
 
 
-def decide_frame(y8, refs, lam, width=None, height=None, sb_size=64):
-    """The oracle's whole-frame decision (single thread): numpy array of SB_DTYPE records."""
+def decide_frame(y8, refs, lam, width=None, height=None, sb_size=64, intra_masks=None):
+    """The oracle's whole-frame decision (single thread): numpy array of SB_DTYPE records;
+    intra_masks: per-superblock intra mode groups (the pruned form's restriction)."""
     ph, pw = y8.shape
     width = width or pw
     height = height or ph
     cols, rows = (width + sb_size - 1) // sb_size, (height + sb_size - 1) // sb_size
     out = np.zeros(cols * rows, dtype=SB_DTYPE)
     refs = [np.ascontiguousarray(r) for r in refs]
-    rc = oracle().orc_decide_frame(width, height, y8.ctypes.data, ptr_array(refs), len(refs), lam, sb_size, out.ctypes.data)
+    if intra_masks is not None:
+        m = np.ascontiguousarray(intra_masks, dtype=np.uint8)
+        assert m.size == cols * rows
+        rc = oracle().orc_decide_frame_masked(width, height, y8.ctypes.data, ptr_array(refs), len(refs), lam, sb_size,
+                                              m.ctypes.data, out.ctypes.data)
+    else:
+        rc = oracle().orc_decide_frame(width, height, y8.ctypes.data, ptr_array(refs), len(refs), lam, sb_size, out.ctypes.data)
     if rc != 0:
         raise RuntimeError(oracle().orc_last_error().decode())
     return out

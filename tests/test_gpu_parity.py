@@ -294,3 +294,35 @@ def test_graph_replays_decide_as_eager_launches(gpu):
     finally:
         for e in alone_g + batch_g + eager:
             e.close()
+
+
+def test_intra_mode_groups_decide_as_the_restricted_oracle(gpu):
+    """celv2hip_set_intra_masks: with per-superblock intra mode groups (random, none beyond DC/V/H,
+    all), every decision equals the oracle's with the same restriction, on an inter frame and a key
+    frame; a source keeps the masks it was scored with (set before prepare(), changed before the
+    launch that takes it)."""
+    w, h = 352, 288
+    _, _, y8 = clip(1, 29, 4, w, h)
+    lam = gpu.lambda_q8(115)
+    e = gpu.RawEngine(w, h)
+    nsb = e.sb_cols * e.sb_rows
+    rng = np.random.default_rng(8)
+    try:
+        for masks in (rng.integers(0, 16, nsb).astype(np.uint8), np.zeros(nsb, np.uint8), np.full(nsb, 15, np.uint8)):
+            e.set_intra_masks(masks)
+            for refs in ([y8[2], y8[1]], []):
+                e.launch(gpu.RawEngine.luma(y8[3]), [gpu.RawEngine.luma(r) for r in refs], lam)
+                want = ol.decide_frame(y8[3], refs, lam, w, h, intra_masks=masks)
+                check(e.collect(), want, e.sb_cols)
+        masks = rng.integers(0, 16, nsb).astype(np.uint8)
+        e.set_intra_masks(masks)
+        e.prepare(gpu.RawEngine.luma(y8[3]), 3)
+        e.set_intra_masks(None)
+        e.launch(gpu.RawEngine.luma(y8[3]), [gpu.RawEngine.luma(y8[2])], lam, keys=[2], src_key=3)
+        check(e.collect(), ol.decide_frame(y8[3], [y8[2]], lam, w, h, intra_masks=masks), e.sb_cols)
+        e.launch(gpu.RawEngine.luma(y8[3]), [gpu.RawEngine.luma(y8[2])], lam, keys=[2])
+        check(e.collect(), ol.decide_frame(y8[3], [y8[2]], lam, w, h), e.sb_cols)
+        with pytest.raises(gpu.EngineError):
+            e.set_intra_masks(np.full(nsb, 16, np.uint8))
+    finally:
+        e.close()

@@ -212,3 +212,39 @@ def test_trained_network_beats_nearest_references(gpu):
     regret_n = (1 - frac) * (jk[lo] / jk[7] - 1) + frac * (jk[min(lo + 1, 7)] / jk[7] - 1)
     assert regret_l <= 0.005, (regret_l, m)
     assert regret_l < regret_n, (regret_l, regret_n, m)
+
+
+def test_intra_mode_groups_cut_the_intra_work(gpu):
+    """The intra dimension's gain: with only DC/V/H scored in every superblock the 1080p source's intra
+    scores take at least 15 % less time than with every group (about 30 % measured; the directional
+    groups' scores are the costly ones), and each group left out saves time."""
+    w, h, nr = 1920, 1080, 2
+    y16, _ = synth.clip_frames(synth.params(1, 3, w, h), 8)
+    dev = [gpu.DeviceBuffer(f) for f in y16]
+    lum = [gpu.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]
+    e = gpu.RawEngine(w, h)
+    e.set_resident(True)
+    lam = gpu.lambda_q8(130)
+    nsb = e.sb_cols * e.sb_rows
+    try:
+        def intra_ms(mask):
+            e.set_intra_masks(np.full(nsb, mask, np.uint8))
+            e.set_timing(True)
+            e.stats_reset()
+            for s in range(12):
+                i = 2 + s % 6
+                e.prepare(lum[i], i)
+                e.launch(lum[i], [lum[i - 1], lum[i - 2]], lam, keys=[i - 1, i - 2], src_key=i)
+            ms, cnt = e.stats()
+            e.set_timing(False)
+            return ms["intra_prepare"] / max(1, cnt["launches"])
+
+        intra_ms(15)  # warm-up
+        full, none_ = intra_ms(15), intra_ms(0)
+        no_dir = intra_ms(3)
+        assert none_ < 0.85 * full, (full, none_)
+        assert no_dir < full, (full, no_dir)
+    finally:
+        e.close()
+        for b in dev:
+            b.free()
