@@ -244,13 +244,13 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
     lam = engine.lambda_q8(qindex)
     w, cfg = pruner.load()
     op = pruner.operating_point(cfg, qindex)
-    threshold, band = op if op else (None, None)
+    threshold, band, gth = op if op else (None, None, 0.0)
     P = engine.Pruner(max_positions=(1920 // 64 + 1) * (1080 // 64 + 1), seed=1)
     P.set_params(w)
     E = engine.RawEngine(1920, 1080)
     E.set_resident(True)
     if op:
-        E.set_pruner(P, threshold, band)
+        E.set_pruner(P, threshold, band, gthreshold=gth)
 
     def step(s):
         i = decidable[s % len(decidable)]

@@ -107,6 +107,8 @@ struct SrcSlot {
   // pyramid on the launch stream (its finalize, on the side stream, follows the intra scores there).
   hipEvent_t pyr_done = nullptr;
   bool pyr_pending = false;
+  // prepare() left the intra scores to the launch (the pruner chooses their mode groups there).
+  bool intra_deferred = false;
 };
 
 }  // namespace
@@ -165,10 +167,17 @@ struct celv2hip_engine {
   bool kmap_on = false;
   uint8_t* imask = nullptr;      // per-superblock intra mode groups on the device (when imask_on)
   bool imask_on = false;
+  uint8_t* nmask = nullptr;      // the pruner's intra mode groups of the last launch
   int64_t* sb_cost = nullptr;    // per-superblock decision cost of the last launch
   // The learned pruner (celv2hip_set_pruner) and the per-superblock features it reads.
   const celv2hip_pruner* pruner = nullptr;
   float prune_threshold = 0.5f, prune_band = 0.f;
+  // > 0: the pruner also chooses each superblock's intra mode groups (celv2hip_set_pruner_groups);
+  // the source's intra scores then run after the network, on the side stream (intra_ev behind them).
+  float prune_gthreshold = 0.f;
+  hipEvent_t infer_ev = nullptr, intra_ev = nullptr;
+  bool intra_ev_pending = false;
+  bool prune_groups() const { return pruner && prune_gthreshold > 0.f; }
   bool want_features = false, features_valid = false;
   bool limits_last = false;  // the last launch obeyed per-superblock reference limits (kmap)
   uint32_t* cost2[kMaxRefs] = {};  // level-2 best search cost per block and reference
@@ -229,6 +238,7 @@ struct celv2hip_engine {
     if (best) (void)hipFree(best);
     if (kmap) (void)hipFree(kmap);
     if (imask) (void)hipFree(imask);
+    if (nmask) (void)hipFree(nmask);
     for (uint32_t* c : cost2)
       if (c) (void)hipFree(c);
     cnn_free(infer);
@@ -252,6 +262,8 @@ struct celv2hip_engine {
     if (sub_done) (void)hipEventDestroy(sub_done);
     if (order_ev) (void)hipEventDestroy(order_ev);
     if (batch_done) (void)hipEventDestroy(batch_done);
+    if (infer_ev) (void)hipEventDestroy(infer_ev);
+    if (intra_ev) (void)hipEventDestroy(intra_ev);
     if (deep_ev) (void)hipEventDestroy(deep_ev);
   }
 
@@ -419,6 +431,8 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     check(hipEventCreateWithFlags(&e->sub_done, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->order_ev, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->batch_done, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&e->infer_ev, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&e->intra_ev, hipEventDisableTiming), "hipEventCreate");
     check(hipEventCreateWithFlags(&e->deep_ev, hipEventDisableTiming), "hipEventCreate");
     for (SrcSlot& s : e->srcs) {
       check(hipEventCreateWithFlags(&s.side_done, hipEventDisableTiming), "hipEventCreate");
@@ -712,6 +726,16 @@ int celv2hip_set_pruner(celv2hip_engine* e, const celv2hip_pruner* p, float thre
   return CELV2_OK;
 }
 
+int celv2hip_set_pruner_groups(celv2hip_engine* e, float gthreshold) {
+  if (!e) return CELV2_ERR_ARG;
+  if (!(gthreshold >= 0.f && gthreshold <= 1.f)) {
+    g_error = "group threshold is 0..1 (0: every group)";
+    return CELV2_ERR_ARG;
+  }
+  e->prune_gthreshold = gthreshold;
+  return CELV2_OK;
+}
+
 int celv2hip_set_features(celv2hip_engine* e, int on) {
   if (!e) return CELV2_ERR_ARG;
   e->want_features = on != 0;
@@ -821,8 +845,11 @@ int celv2hip_prepare(celv2hip_engine* e, const celv2_luma* next, long long key) 
     e->claim(s, e->side);
     run_builds({e->build_job(*next, s.p, s.up, false, e->side)}, e->side);
     check(hipEventRecord(s.pyr_done, e->side), "hipEventRecord"), s.pyr_pending = true;
-    if (e->deep_pending) check(hipStreamWaitEvent(e->side, e->deep_ev, 0), "hipStreamWaitEvent");
-    run_intra({{e, &s}}, e->side);
+    s.intra_deferred = e->prune_groups();
+    if (!s.intra_deferred) {
+      if (e->deep_pending) check(hipStreamWaitEvent(e->side, e->deep_ev, 0), "hipStreamWaitEvent");
+      run_intra({{e, &s}}, e->side);
+    }
     e->release(s, e->side);
     s.key = key;
   });
@@ -875,6 +902,8 @@ void launch_batch(const LaunchIn* in, int n) {
   }
   std::vector<celv2hip_engine::BuildJob> jobs;
   std::vector<std::pair<celv2hip_engine*, SrcSlot*>> intra_now;
+  // Engines whose source's intra scores wait for the network's mode groups (after level 2).
+  std::vector<char> intra_after(n);
   std::vector<SrcSlot*> cur(n);
   std::vector<std::vector<RefSlot*>> used(n);
   std::vector<char> use_kmap(n);
@@ -909,13 +938,20 @@ void launch_batch(const LaunchIn* in, int n) {
     } else {
       e->claim(claimed, S);
     }
+    const bool groups_here = e->prune_groups() && x.nrefs >= 1;
     if (use < 0) {
       use = 1 - e->cur_set;
       SrcSlot& s = e->srcs[use];
       s.key = -1;
       jobs.push_back(e->build_job(*x.src, s.p, s.up, false, S));
-      intra_now.push_back({e, &s});
+      if (groups_here) intra_after[i] = true;
+      else intra_now.push_back({e, &s});
+    } else if (claimed.intra_deferred) {
+      // prepare() left the intra scores to this launch: after the network, or now with every group.
+      if (groups_here) intra_after[i] = true;
+      else intra_now.push_back({e, &claimed});
     }
+    e->srcs[use].intra_deferred = false;
     e->srcs[use].key = -1;  // taken: a later prepare() may overwrite it
     e->cur_set = use;
     cur[i] = &e->srcs[use];
@@ -959,6 +995,11 @@ void launch_batch(const LaunchIn* in, int n) {
       if (e->copy_pending) check(hipStreamWaitEvent(S, e->copy_done, 0), "hipStreamWaitEvent"), e->copy_pending = false;
     }
     for (auto& it : intra_now) (void)it.first->intra_item(*it.second);
+    for (int i = 0; i < n; i++)
+      if (intra_after[i]) {
+        if (!in[i].e->nmask) check(hipMalloc(&in[i].e->nmask, in[i].e->nsb()), "hipMalloc intra masks");
+        (void)in[i].e->intra_item(*cur[i]);
+      }
     check(hipStreamBeginCapture(S, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     capture.s = S;
   }
@@ -1026,9 +1067,27 @@ void launch_batch(const LaunchIn* in, int n) {
         launch_features(fa, S);
         e->features_valid = true;
         if (e->pruner) {
+          const bool groups = intra_after[i] != 0;
+          if (groups && !e->nmask) check(hipMalloc(&e->nmask, e->nsb()), "hipMalloc intra masks");
           cnn_infer(e->infer.x, 1, e->sb_rows, e->sb_cols, e->pruner->params, e->pruner->pb, nullptr, in[i].nrefs,
-                    e->prune_threshold, e->prune_band, e->kmap, e->counters + 3 * kCounterWords, S);
+                    e->prune_threshold, e->prune_band, e->kmap, e->counters + 3 * kCounterWords, S,
+                    groups ? e->nmask : nullptr, e->prune_gthreshold);
           use_kmap[i] = true;
+          if (groups) {
+            // The source's intra scores with the network's groups, beside levels 1 and 0 (on the launch
+            // stream in the graph form); finalize waits for them.
+            const hipStream_t I = use_graph ? S : e->side;
+            if (I != S) {
+              check(hipEventRecord(e->infer_ev, S), "hipEventRecord");
+              check(hipStreamWaitEvent(I, e->infer_ev, 0), "hipStreamWaitEvent");
+            }
+            IntraItem it = e->intra_item(*cur[i]);
+            it.mask = e->nmask;
+            Batch<IntraItem> b;
+            add_intra(b, it);
+            launch_intra(b, I);
+            if (I != S) check(hipEventRecord(e->intra_ev, I), "hipEventRecord"), e->intra_ev_pending = true;
+          }
         }
         check(hipGetLastError(), "pruner kernels");
       }
@@ -1102,6 +1161,8 @@ void launch_batch(const LaunchIn* in, int n) {
     launch_inter(bi, S);
     for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[3], S), "hipEventRecord");
     for (int i = 0; i < n; i++) if (late_claim[i]) in[i].e->claim(*cur[i], S);
+    for (int i = 0; i < n; i++)
+      if (in[i].e->intra_ev_pending) check(hipStreamWaitEvent(S, in[i].e->intra_ev, 0), "hipStreamWaitEvent"), in[i].e->intra_ev_pending = false;
     Batch<FinalizeArgs> bf;
     for (int i = 0; i < n; i++) {
       FinalizeArgs fa = finalize_args(i);
@@ -1158,6 +1219,7 @@ void launch_batch(const LaunchIn* in, int n) {
       }
       // Wavefront form: the group's finalize on the side stream once its inter scores are done.
       const hipStream_t F = wavefront ? e->side : S;
+      if (!wavefront && e->intra_ev_pending) check(hipStreamWaitEvent(S, e->intra_ev, 0), "hipStreamWaitEvent"), e->intra_ev_pending = false;
       if (wavefront) {
         check(hipEventRecord(e->inter_ev[g], S), "hipEventRecord");
         check(hipStreamWaitEvent(F, e->inter_ev[g], 0), "hipStreamWaitEvent");

@@ -11,7 +11,9 @@ namespace celv2hip {
 
 constexpr int kFeatures = 14;       // inputs per superblock (pruner.hip)
 constexpr int kHidden = 32;         // channels of the two 3x3 layers
-constexpr int kOut = 6;             // output j: needs more than j + 1 references
+constexpr int kRefOut = 6;          // output j < 6: needs more than j + 1 references
+constexpr int kGroupOut = 3;        // output 6 + g: needs intra mode group g + 1 (groups 1..3; group 0 always scored)
+constexpr int kOut = kRefOut + kGroupOut;
 constexpr int kK1 = 128;            // im2col row of layer 1: 9 * kFeatures = 126, padded
 constexpr int kK2 = 9 * kHidden;    // im2col row of layer 2
 // Parameters: W1[9 kFeatures][kHidden], b1, W2[kK2][kHidden], b2, W3[kHidden][kOut], b3 (row k of a
@@ -35,7 +37,7 @@ struct CnnWork {
   float* dz1 = nullptr;
   uint16_t* dz1b = nullptr;
   float* loss_part = nullptr;
-  uint8_t* kstar = nullptr;  // labels: the references a superblock needs (1..7)
+  uint8_t* kstar = nullptr;  // labels: the references a superblock needs (1..7, bits 0..2) and the intra groups it needs (bit 3 + g - 1 for group g)
   size_t cap = 0;
   size_t positions_capacity() const;
 };
@@ -50,9 +52,12 @@ void cnn_adamw(float* params, const float* grads, float* m1, float* m2, uint16_t
 void cnn_limits(const float* logits, int m, int nrefs, float threshold, float band, uint8_t* kmap,
                 unsigned long long* fallbacks, hipStream_t s);
 // Inference in one launch over x (b frames of h x wd superblocks): logits (if non-null, as cnn_forward
-// leaves them in w.z3, bit for bit) and reference limits into kmap (if non-null, as cnn_limits).
+// leaves them in w.z3, bit for bit), reference limits into kmap (if non-null, as cnn_limits) and, if
+// imask is non-null, each superblock's intra mode groups: group g (1..3) when sigma(logit 5 + g) >=
+// gthreshold, every group when a reference output was within band of its threshold (a fallback).
 void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
-               float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s);
+               float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s,
+               uint8_t* imask = nullptr, float gthreshold = 0.f);
 
 // The static scratch of the no-allocation forward pass: features [kStaticPositions][kFeatures]
 // followed by logits [kStaticPositions][kOut] (1.3 MB), one copy per device. cnn_scratch_init takes

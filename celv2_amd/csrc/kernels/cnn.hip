@@ -83,7 +83,7 @@ __device__ __forceinline__ int limit_of(const float* z, int nrefs, float thresho
   int k = 1;
   bool open = true;
   unsure = false;
-  for (int j = 0; j < kOut; j++) {
+  for (int j = 0; j < kRefOut; j++) {
     const float s = 1.f / (1.f + expf(-z[j]));
     if (fabsf(s - threshold) < band) unsure = true;
     if (open && s >= threshold) k = j + 2;
@@ -103,14 +103,25 @@ struct Infer {
   unsigned long long* fallbacks;
   int nrefs;
   float threshold, band;
+  uint8_t* imask;  // [b][h][w] intra mode groups (may be null)
+  float gthreshold;
 };
+
+// A superblock's intra mode groups from the group outputs (group 0 always; every group on a fallback).
+__device__ __forceinline__ uint8_t groups_of(const float* z, float gthreshold, bool unsure) {
+  if (unsure) return kAllGroups;
+  uint8_t m = 1;
+  for (int g = 0; g < kGroupOut; g++)
+    if (1.f / (1.f + expf(-z[kRefOut + g])) >= gthreshold) m |= static_cast<uint8_t>(2u << g);
+  return m;
+}
 
 constexpr int kInferThreads = 512;  // 8 waves: layer 2's 8 tiles one per wave
 __global__ void __launch_bounds__(kInferThreads) k_infer(Infer a) {
   __shared__ uint16_t xs[kXH * kXW * kFeatures];
   __shared__ uint16_t z1s[kZ1 * kHidden];
   __shared__ uint16_t z2s[kTileH * kTileW * kHidden];
-  __shared__ float z3s[kTileH * kTileW * 8];
+  __shared__ float z3s[kTileH * kTileW * 16];
   // The BF16 weights in LDS (the products read them at every step).
   __shared__ uint16_t w1s[9 * kFeatures * kHidden], w2s[kK2 * kHidden], w3s[kHidden * kOut];
   const int tid = static_cast<int>(threadIdx.x), wave = tid >> 6, lane = tid & 63, i = lane & 15, q = lane >> 4;
@@ -186,20 +197,21 @@ __global__ void __launch_bounds__(kInferThreads) k_infer(Infer a) {
     }
     if (i < kOut)
 #pragma unroll
-      for (int r = 0; r < 4; r++) z3s[(tm * kTileW + 4 * q + r) * 8 + i] = acc[r] + a.params[kB3 + i];
+      for (int r = 0; r < 4; r++) z3s[(tm * kTileW + 4 * q + r) * 16 + i] = acc[r] + a.params[kB3 + i];
   }
   __syncthreads();
   if (tid < kTileH * kTileW) {
     const int yy = ty0 + tid / kTileW, xx = tx0 + tid % kTileW;
     if (yy < a.h && xx < a.w) {
       const size_t p = frame + static_cast<size_t>(yy) * a.w + xx;
-      const float* z = z3s + tid * 8;
+      const float* z = z3s + tid * 16;
       if (a.logits)
         for (int j = 0; j < kOut; j++) a.logits[p * kOut + j] = z[j];
       if (a.kmap) {
         bool unsure;
         a.kmap[p] = static_cast<uint8_t>(limit_of(z, a.nrefs, a.threshold, a.band, unsure));
         if (unsure && a.fallbacks) count(a.fallbacks, static_cast<int>(p), 1ull);
+        if (a.imask) a.imask[p] = groups_of(z, a.gthreshold, unsure);
       }
     }
   }
@@ -266,7 +278,8 @@ __global__ void k_bias_grad(const float* g, long long m, int n, float* db) {
   if (threadIdx.x == 0) db[col] = part[0];
 }
 
-// Binary cross-entropy with logits, labels y_j = (kstar > j + 1); mean over m x kOut. Writes the
+// Binary cross-entropy with logits, labels y_j = (kstar > j + 1) for the reference outputs (kstar =
+// label bits 0..2) and y = label bit 3 + g for group output 6 + g; mean over m x kOut. Writes the
 // gradient (sigma(z) - y) / (m kOut), its bf16 copy, and per-block partial losses.
 __global__ void k_bce(const float* z, const uint8_t* kstar, long long m, float* dz, uint16_t* dzb, float* loss_part) {
   __shared__ float part[256];
@@ -275,7 +288,8 @@ __global__ void k_bce(const float* z, const uint8_t* kstar, long long m, float* 
   if (i < m * kOut) {
     const long long p = i / kOut;
     const int j = static_cast<int>(i - p * kOut);
-    const float x = z[i], y = kstar[p] > j + 1 ? 1.f : 0.f;
+    const uint32_t lab = kstar[p];
+    const float x = z[i], y = (j < kRefOut ? (lab & 7u) > static_cast<uint32_t>(j + 1) : ((lab >> (3 + j - kRefOut)) & 1u)) ? 1.f : 0.f;
     l = fmaxf(x, 0.f) - x * y + log1pf(expf(-fabsf(x)));
     const float s = 1.f / (1.f + expf(-x));
     const float g = (s - y) / static_cast<float>(m * kOut);
@@ -407,9 +421,10 @@ void cnn_adamw(float* params, const float* grads, float* m1, float* m2, uint16_t
 }
 
 void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
-               float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s) {
+               float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s, uint8_t* imask,
+               float gthreshold) {
   if (b <= 0 || h <= 0 || wd <= 0) return;
-  const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band};
+  const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band, imask, gthreshold};
   k_infer<<<dim3((wd + kTileW - 1) / kTileW, (h + kTileH - 1) / kTileH, b), kInferThreads, 0, s>>>(a);
 }
 

@@ -21,6 +21,7 @@ P = ctypes.c_void_p
 I = ctypes.c_int
 LL = ctypes.c_longlong
 FEATURES = 14  # the learned pruner's inputs per superblock (include/celv2hip.h)
+N_OUT = 9  # its outputs: 6 reference outputs, 3 intra mode-group outputs (CELV2HIP_PRUNER_OUTPUTS)
 
 
 class Luma(ctypes.Structure):
@@ -60,6 +61,7 @@ def lib():
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_set_ref_limits.argtypes = [P, P, I]
         L.celv2hip_set_intra_masks.argtypes = [P, P, I]
+        L.celv2hip_set_pruner_groups.argtypes = [P, ctypes.c_float]
         L.celv2hip_pruner_create.argtypes = [I, I, ctypes.c_uint, ctypes.POINTER(P)]
         L.celv2hip_pruner_destroy.argtypes = [P]
         L.celv2hip_pruner_get_params.argtypes = [P, P, I]
@@ -391,8 +393,10 @@ class RawEngine:
             self._check(int(got), "fetch_sb_costs")
         return out
 
-    def set_pruner(self, pruner, threshold=0.5, band=0.0):
+    def set_pruner(self, pruner, threshold=0.5, band=0.0, gthreshold=0.0):
+        """Attaches a pruner (None detaches); gthreshold > 0: it also chooses the intra mode groups."""
         self._check(lib().celv2hip_set_pruner(self.h, pruner.h if pruner is not None else None, threshold, band), "set_pruner")
+        self._check(lib().celv2hip_set_pruner_groups(self.h, gthreshold if pruner is not None else 0.0), "set_pruner_groups")
 
     def set_features(self, on):
         self._check(lib().celv2hip_set_features(self.h, 1 if on else 0), "set_features")
@@ -464,7 +468,7 @@ class Pruner:
     def forward(self, feats):
         feats = np.ascontiguousarray(feats, dtype=np.float32)
         b, h, w, _ = feats.shape
-        out = np.zeros((b, h, w, 6), np.float32)
+        out = np.zeros((b, h, w, N_OUT), np.float32)
         if lib().celv2hip_pruner_forward(self.h, feats.ctypes.data, b, h, w, out.ctypes.data) != 0:
             raise EngineError(last_error())
         return out
@@ -473,7 +477,7 @@ class Pruner:
         """forward() through the entry point that allocates nothing (at most 16384 positions)."""
         feats = np.ascontiguousarray(feats, dtype=np.float32)
         b, h, w, _ = feats.shape
-        out = np.zeros((b, h, w, 6), np.float32)
+        out = np.zeros((b, h, w, N_OUT), np.float32)
         if lib().celv2hip_pruner_forward_static(self.h, feats.ctypes.data, b, h, w, out.ctypes.data) != 0:
             raise EngineError(last_error())
         return out

@@ -57,12 +57,15 @@ def clip_spec(seed):
 
 
 def clip_logs(E, seed, y16):
-    """(features [n, sb_rows, sb_cols, F], costs [n, sbs, 7]) of a clip's decidable frames. Reference
-    keys are unique across clips (seed * 1000 + frame), so the engine's cached planes never mix clips."""
+    """(features [n, sb_rows, sb_cols, F], costs [n, sbs, 7], group costs [n, sbs, 3]) of a clip's
+    decidable frames: J_k with the nearest k references, and J_7 with each of the intra mode groups
+    1..3 left out in turn. Reference keys are unique across clips (seed * 1000 + frame), so the
+    engine's cached planes never mix clips."""
     lam = engine.lambda_q8(clip_spec(seed)[1])
     dev = [engine.DeviceBuffer(f) for f in y16]
     lum = [engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]
-    feats, costs = [], []
+    feats, costs, gcosts = [], [], []
+    nsb = E.sb_cols * E.sb_rows
     for f in range(NR, len(y16)):
         refs = [f - k for k in range(1, NR + 1)]
         keys = [seed * 1000 + r for r in refs]
@@ -74,10 +77,19 @@ def clip_logs(E, seed, y16):
             J.append(E.fetch_sb_costs())
         feats.append(E.fetch_features())
         costs.append(np.stack(J, axis=1))
+        E.set_features(False)
+        G = []
+        for g in range(1, 4):
+            E.set_intra_masks(np.full(nsb, 15 & ~(1 << g), np.uint8))
+            E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
+            E.sync()
+            G.append(E.fetch_sb_costs())
+        E.set_intra_masks(None)
+        gcosts.append(np.stack(G, axis=1))
     E.set_features(False)
     for b in dev:
         b.free()
-    return np.stack(feats), np.stack(costs)
+    return np.stack(feats), np.stack(costs), np.stack(gcosts)
 
 
 def gather(E, seeds, keep=None):
@@ -125,6 +137,36 @@ def optimal_labels(costs, target):
     return allocation(costs, hi), hi
 
 
+def group_labels(costs, gcosts):
+    """Label bits 3 + g - 1: intra mode group g (1..3) is needed where leaving it out raises the
+    superblock's decision cost at 7 references."""
+    j7 = costs[..., NR - 1]
+    need = gcosts > j7[..., None]
+    return (need.astype(np.uint8) << np.array([3, 4, 5], np.uint8)).sum(-1).astype(np.uint8)
+
+
+def group_regret(logits, costs, gcosts, gthreshold):
+    """The cost regret of leaving out the groups whose output falls below gthreshold (each group's
+    cost taken alone at 7 references: the groups' effects summed), and the groups kept per superblock."""
+    s = 1.0 / (1.0 + np.exp(-logits[:, 6:].astype(np.float64)))
+    drop = s < gthreshold
+    c = costs.reshape(-1, NR)[:, NR - 1].astype(np.float64)
+    g = gcosts.reshape(-1, 3).astype(np.float64)
+    return float(((g - c[:, None]) * drop).sum() / c.sum()), float((~drop).sum(1).mean())
+
+
+def group_point(logits, costs, gcosts, budget):
+    """The highest group threshold (the most groups left out) whose regret on every clip is within
+    budget; logits, costs and gcosts are lists over clips."""
+    best = None
+    for gth in np.arange(0.05, 0.96, 0.05):
+        rs = [group_regret(z, c, g, gth) for z, c, g in zip(logits, costs, gcosts)]
+        if max(r for r, _ in rs) <= budget:
+            best = {"gthreshold": round(float(gth), 3), "regret": max(r for r, _ in rs),
+                    "groups_kept": float(np.mean([k for _, k in rs]))}
+    return best
+
+
 def limits(logits, threshold, band):
     """The engine's limit rule (celv2hip_set_pruner) on logits [positions, 6], all 7 references present."""
     s = 1.0 / (1.0 + np.exp(-logits.astype(np.float64)))
@@ -157,7 +199,8 @@ def operating_point(logits, costs, target):
 def engine_check(E, P, op, seeds, clips):
     """An operating point through the engine itself on clips (seed -> y16 frames): (the largest clip
     regret against the plain engine's J_7, mean limit, fallback rate)."""
-    E.set_pruner(P, op["threshold"], op["band"])
+    gth = op.get("gthreshold", 0.0)
+    E.set_pruner(P, op["threshold"], op["band"], gthreshold=gth)
     E.stats_reset()
     worst = 0.0
     ks = []
@@ -174,7 +217,7 @@ def engine_check(E, P, op, seeds, clips):
             E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
             E.sync()
             j7 = E.fetch_sb_costs().astype(np.float64)
-            E.set_pruner(P, op["threshold"], op["band"])
+            E.set_pruner(P, op["threshold"], op["band"], gthreshold=gth)
             E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
             E.sync()
             num += (E.fetch_sb_costs() - j7).sum()
@@ -195,6 +238,7 @@ def main():
     ap.add_argument("--val-clips", type=int, default=16)
     ap.add_argument("--label-target", type=float, default=0.004, help="the total regret of the training labels")
     ap.add_argument("--target", type=float, default=0.004, help="largest total cost regret an operating point may have")
+    ap.add_argument("--group-budget", type=float, default=0.0005, help="the part of the target the intra groups may take")
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=3e-3)
@@ -214,11 +258,14 @@ def main():
     log(f"validation logs: {len(val_seeds)} clips in {time.perf_counter() - t0:.0f}s")
 
     # Labels per clip.
-    labels = {s: optimal_labels(tl[s][1], args.label_target)[0] for s in train_seeds}
+    labels = {s: optimal_labels(tl[s][1], args.label_target)[0] | group_labels(tl[s][1], tl[s][2]) for s in train_seeds}
     for q in QINDICES:
         ss = [s for s in train_seeds if clip_spec(s)[1] == q]
-        k = np.concatenate([labels[s] for s in ss]) if ss else np.zeros(0, np.uint8)
-        log(f"qindex {q}: {len(ss)} clips, label mean refs {k.mean():.3f}, k* histogram {np.bincount(k.ravel(), minlength=8)[1:].tolist()}")
+        lab = np.concatenate([labels[s] for s in ss]) if ss else np.zeros(0, np.uint8)
+        k = lab & 7
+        gneed = [float(((lab >> (3 + g)) & 1).mean()) if lab.size else 0.0 for g in range(3)]
+        log(f"qindex {q}: {len(ss)} clips, label mean refs {k.mean():.3f}, k* histogram {np.bincount(k.ravel(), minlength=8)[1:].tolist()}, "
+            f"groups needed {[round(v, 4) for v in gneed]}")
     xt = np.concatenate([tl[s][0] for s in train_seeds])
     kt = np.concatenate([labels[s] for s in train_seeds])
     rows, cols = xt.shape[1], xt.shape[2]
@@ -238,10 +285,13 @@ def main():
         ss = [s for s in val_seeds if clip_spec(s)[1] == q]
         if not ss:
             continue
-        logits = [np.concatenate([P.forward(vl[s][0][i:i + args.batch]) for i in range(0, vl[s][0].shape[0], args.batch)]).reshape(-1, 6)
+        logits = [np.concatenate([P.forward(vl[s][0][i:i + args.batch]) for i in range(0, vl[s][0].shape[0], args.batch)]).reshape(-1, engine.N_OUT)
                   for s in ss]
         costs = [vl[s][1] for s in ss]
-        op = operating_point(logits, costs, args.target)
+        gcosts = [vl[s][2] for s in ss]
+        gp = group_point(logits, costs, gcosts, args.group_budget)
+        op = operating_point([z[:, :6] for z in logits], costs, args.target - (gp["regret"] if gp else 0.0))
+        op["gthreshold"] = gp["gthreshold"] if gp else 0.0
         r, mk, fb = engine_check(E, P, op, ss, val_clips)
         best = [optimal_labels(c, args.target)[0] for c in costs]
         # The rule without the network: the nearest k references everywhere, k the fewest within
@@ -249,13 +299,14 @@ def main():
         nearest = {k: max(regret(c, np.full(c.shape[:2], k))[0] for c in costs) for k in range(1, NR + 1)}
         k_plain = min(k for k in range(1, NR + 1) if nearest[k] <= args.target)
         # An operating point that keeps every reference everywhere is no pruning: the plain engine.
-        ops[str(q)] = {"threshold": op["threshold"], "band": op["band"]} if mk < NR - 0.05 else None
+        ops[str(q)] = {"threshold": op["threshold"], "band": op["band"], "gthreshold": op["gthreshold"]} if mk < NR - 0.05 else None
         report[str(q)] = {"val_clips": len(ss), "val_frames": int(sum(c.shape[0] for c in costs)),
                           "worst_clip_regret": r, "mean_refs": mk, "fallback_rate": fb,
                           "simulated_worst_clip_regret": op["regret"], "simulated_mean_refs": op["mean_refs"],
                           "best_allocation_mean_refs": float(np.mean(np.concatenate([b.ravel() for b in best]))),
                           "nearest_k_worst_clip_regret": {str(k): v for k, v in nearest.items()},
-                          "nearest_k_within_target": k_plain}
+                          "nearest_k_within_target": k_plain,
+                          "groups": gp or {"gthreshold": 0.0, "regret": 0.0, "groups_kept": 3.0}}
         log(f"qindex {q}: {json.dumps(ops[str(q)])} worst clip regret {r:.5f}, mean limit {mk:.3f} "
             f"(best allocation {report[str(q)]['best_allocation_mean_refs']:.3f}, nearest-k within target: {k_plain})")
         if args.logs and q == 130:

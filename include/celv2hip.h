@@ -209,6 +209,9 @@ long long celv2hip_fetch_sb_costs(celv2hip_engine* e, int64_t* out, long long ca
  * attached to several engines; it must not be trained while their launches run. */
 typedef struct celv2hip_pruner celv2hip_pruner;
 #define CELV2HIP_PRUNER_FEATURES 14
+/* Outputs per superblock: 6 reference outputs (j: needs more than j + 1 references), then 3 intra
+ * mode-group outputs (6 + g - 1: needs group g, g = 1..3). */
+#define CELV2HIP_PRUNER_OUTPUTS 9
 
 /* Number of FP32 parameters of the network. */
 int celv2hip_pruner_param_count(void);
@@ -218,7 +221,7 @@ int celv2hip_pruner_destroy(celv2hip_pruner* p);
 int celv2hip_pruner_get_params(celv2hip_pruner* p, float* out, int cap);
 /* Loads parameters (n = param count) and resets the AdamW state. */
 int celv2hip_pruner_set_params(celv2hip_pruner* p, const float* in, int n);
-/* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 14 + i]: the 6 logits per superblock. */
+/* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 14 + i]: the 9 logits per superblock. */
 int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
 /* The same forward pass without any device allocation: features and logits are staged in a static
  * device array of the library (one per device, taken at celv2hip_pruner_create), so a caller that
@@ -227,8 +230,9 @@ int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h
  * that array and race; the array keeps the last call's contents. No counterpart in the reference: an
  * entry point for callers that cannot allocate device memory. */
 int celv2hip_pruner_forward_static(const celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
-/* Mean binary cross-entropy against labels kstar (the references each superblock needs, 1..7) and its
- * gradient with respect to every parameter, without updating them. */
+/* Mean binary cross-entropy against labels kstar (per superblock: the references it needs, 1..7, in
+ * bits 0..2, and bit 2 + g set when it needs intra mode group g = 1..3) and its gradient with respect
+ * to every parameter, without updating them. */
 int celv2hip_pruner_gradients(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w,
                               float* grads, float* loss);
 /* One AdamW step (beta 0.9 / 0.999, eps 1e-8) on that loss; *loss is the loss before the step. */
@@ -238,6 +242,12 @@ int celv2hip_pruner_train_step(celv2hip_pruner* p, const float* feats, const uin
  * is >= threshold references (at most nrefs); if any output lies within `band` of the threshold the
  * superblock falls back to all references (counted in celv2hip_stats counts[5]). */
 int celv2hip_set_pruner(celv2hip_engine* e, const celv2hip_pruner* p, float threshold, float band);
+/* gthreshold > 0: the attached pruner also chooses each superblock's intra mode groups (as
+ * celv2hip_set_intra_masks): group g when the probability of output 5 + g is >= gthreshold, every
+ * group where the superblock fell back to all references; the source's intra scores then run after
+ * the network (prepare() leaves them to the launch). 0: every group (the default). No counterpart in
+ * the reference. */
+int celv2hip_set_pruner_groups(celv2hip_engine* e, float gthreshold);
 /* on != 0: launches compute the pruner's features even without a pruner (training data). */
 int celv2hip_set_features(celv2hip_engine* e, int on);
 /* The last launch's features (superblocks x 14 floats); returns the count or a negative status. */

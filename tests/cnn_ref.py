@@ -4,7 +4,8 @@ rounding=True) BF16 rounding of every GEMM operand exactly where the GPU rounds 
 outputs and gradients differ from it only by FP32 accumulation order."""
 import numpy as np
 
-F, HID, OUT = 14, 32, 6
+F, HID, REF_OUT, GROUP_OUT = 14, 32, 6, 3
+OUT = REF_OUT + GROUP_OUT
 W1 = 0
 B1 = W1 + 9 * F * HID
 W2 = B1 + HID
@@ -61,7 +62,11 @@ def forward(p, x, rounding=True):
 
 
 def labels(kstar):
-    return (np.asarray(kstar).reshape(-1, 1) > np.arange(OUT).reshape(1, -1) + 1).astype(np.float64)
+    """Targets per position: reference outputs j: (label & 7) > j + 1; group outputs g: bit 3 + g."""
+    lab = np.asarray(kstar).reshape(-1, 1).astype(np.int64)
+    refs = (lab & 7) > np.arange(REF_OUT).reshape(1, -1) + 1
+    groups = (lab >> (3 + np.arange(GROUP_OUT).reshape(1, -1))) & 1
+    return np.concatenate([refs, groups.astype(bool)], axis=1).astype(np.float64)
 
 
 def loss_and_grads(p, x, kstar, rounding=True):
@@ -96,13 +101,24 @@ def adamw(p, g, m1, m2, step, lr, wd, b1=0.9, b2=0.999, eps=1e-8):
 
 def limits(logits, nrefs, threshold, band):
     """The engine's per-superblock reference limit from the network's outputs."""
-    s = 1.0 / (1.0 + np.exp(-np.asarray(logits, np.float64)))
+    s = 1.0 / (1.0 + np.exp(-np.asarray(logits, np.float64)))[:, :REF_OUT]
     k = np.ones(s.shape[0], np.int64)
     open_ = np.ones(s.shape[0], bool)
-    for j in range(OUT):
+    for j in range(REF_OUT):
         hit = open_ & (s[:, j] >= threshold)
         k[hit] = j + 2
         open_ &= s[:, j] >= threshold
     unsure = (np.abs(s - threshold) < band).any(1)
     k[unsure] = 7
     return np.minimum(k, nrefs), unsure
+
+
+def groups(logits, gthreshold, unsure):
+    """The engine's intra mode groups from the group outputs: bit 0 always, bit g when
+    sigma(output 5 + g) >= gthreshold, every group where the superblock fell back."""
+    s = 1.0 / (1.0 + np.exp(-np.asarray(logits, np.float64)))[:, REF_OUT:]
+    m = np.ones(s.shape[0], np.int64)
+    for g in range(GROUP_OUT):
+        m |= (s[:, g] >= gthreshold).astype(np.int64) << (g + 1)
+    m[unsure] = 15
+    return m

@@ -21,14 +21,14 @@ def rel(a, b):
 def batch():
     rng = np.random.default_rng(11)
     x = rng.uniform(0, 1, (3, 5, 7, cr.F)).astype(np.float32)
-    kstar = rng.integers(1, 8, 3 * 5 * 7).astype(np.uint8)
+    kstar = (rng.integers(1, 8, 3 * 5 * 7) | rng.integers(0, 8, 3 * 5 * 7) << 3).astype(np.uint8)  # refs + group bits
     return x, kstar
 
 
 def test_forward_matches_cpu_reference(gpu, batch):
     x, _ = batch
     p = gpu.Pruner(max_positions=1024, seed=5)
-    got = p.forward(x).reshape(-1, 6)
+    got = p.forward(x).reshape(-1, cr.OUT)
     want, _ = cr.forward(p.params(), x)
     assert rel(got, want) < 2e-3, rel(got, want)
     p.close()
@@ -40,7 +40,7 @@ def test_forward_on_a_1080p_grid(gpu):
     rng = np.random.default_rng(12)
     x = rng.uniform(0, 1, (2, 17, 30, cr.F)).astype(np.float32)
     p = gpu.Pruner(max_positions=2 * 17 * 30, seed=13)
-    got = p.forward(x).reshape(-1, 6)
+    got = p.forward(x).reshape(-1, cr.OUT)
     want, _ = cr.forward(p.params(), x)
     assert rel(got, want) < 2e-3, rel(got, want)
     p.close()
@@ -90,7 +90,7 @@ def test_engine_uses_the_network_limits(gpu):
     pruned, pruned_cost = e.collect(), e.fetch_sb_costs()
     kmap = e.fetch_ref_limits()
     feats = e.fetch_features()
-    want_k, unsure = cr.limits(p.forward(feats[None]).reshape(-1, 6), 7, 0.5, 0.02)
+    want_k, unsure = cr.limits(p.forward(feats[None]).reshape(-1, cr.OUT), 7, 0.5, 0.02)
     assert np.array_equal(kmap, want_k.astype(np.uint8)), (kmap.tolist(), want_k.tolist())
     e.set_pruner(None)
     e.set_ref_limits(kmap)
@@ -99,6 +99,44 @@ def test_engine_uses_the_network_limits(gpu):
     assert np.array_equal(e.fetch_sb_costs(), pruned_cost)
     e.close()
     p.close()
+
+
+@pytest.mark.parametrize("prepared", [False, True])
+def test_engine_uses_the_network_groups(gpu, prepared):
+    """With the group outputs on (celv2hip_set_pruner_groups), a launch decides exactly as one with
+    the network's reference limits and intra mode groups given explicitly (celv2hip_set_ref_limits,
+    celv2hip_set_intra_masks), the groups being the reference rule of cnn_ref.groups on the same
+    logits; with the source prepared (its intra scores left to the launch) and without."""
+    w, h = 352, 288
+    y8 = synth.clip_frames(synth.params(1, 33, w, h), 9)[1]
+    lam = gpu.lambda_q8(140)
+    f, refs = 8, list(range(7, 0, -1))
+    src, rl = gpu.RawEngine.luma(y8[f]), [gpu.RawEngine.luma(y8[r]) for r in refs]
+    p = gpu.Pruner(max_positions=1024, seed=4)
+    e = gpu.RawEngine(w, h)
+    try:
+        for gth in (0.5, 0.45):
+            e.set_pruner(p, 0.5, 0.02, gthreshold=gth)
+            if prepared:
+                e.prepare(src, f)
+            e.launch(src, rl, lam, keys=refs, src_key=f if prepared else -1)
+            pruned, pruned_cost = e.collect(), e.fetch_sb_costs()
+            kmap = e.fetch_ref_limits()
+            logits = p.forward(e.fetch_features()[None]).reshape(-1, cr.OUT)
+            want_k, unsure = cr.limits(logits, 7, 0.5, 0.02)
+            masks = cr.groups(logits, gth, unsure).astype(np.uint8)
+            assert np.array_equal(kmap, want_k.astype(np.uint8))
+            e.set_pruner(None)
+            e.set_ref_limits(kmap)
+            e.set_intra_masks(masks)
+            e.launch(src, rl, lam, keys=refs)
+            assert compare_maps(e.collect(), pruned, e.sb_cols) is None, (gth, prepared)
+            assert np.array_equal(e.fetch_sb_costs(), pruned_cost)
+            e.set_ref_limits(None)
+            e.set_intra_masks(None)
+    finally:
+        e.close()
+        p.close()
 
 
 def cpu_features(src2, costs, sb_cols, sb_rows, lam):
