@@ -159,7 +159,7 @@ def group_point(logits, costs, gcosts, budget):
     """The highest group threshold (the most groups left out) whose regret on every clip is within
     budget; logits, costs and gcosts are lists over clips."""
     best = None
-    for gth in np.arange(0.05, 0.96, 0.05):
+    for gth in np.concatenate([[0.005, 0.01, 0.02, 0.03, 0.04], np.arange(0.05, 0.96, 0.05)]):
         rs = [group_regret(z, c, g, gth) for z, c, g in zip(logits, costs, gcosts)]
         if max(r for r, _ in rs) <= budget:
             best = {"gthreshold": round(float(gth), 3), "regret": max(r for r, _ in rs),
@@ -238,7 +238,7 @@ def main():
     ap.add_argument("--val-clips", type=int, default=16)
     ap.add_argument("--label-target", type=float, default=0.004, help="the total regret of the training labels")
     ap.add_argument("--target", type=float, default=0.004, help="largest total cost regret an operating point may have")
-    ap.add_argument("--group-budget", type=float, default=0.0005, help="the part of the target the intra groups may take")
+    ap.add_argument("--group-budget", type=float, default=0.001, help="the part of the target the intra groups may take")
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=3e-3)
