@@ -769,6 +769,21 @@ long long celv2hip_fetch_ref_limits(celv2hip_engine* e, uint8_t* out, long long 
   return rc ? rc : n;
 }
 
+long long celv2hip_fetch_intra_masks(celv2hip_engine* e, uint8_t* out, long long cap) {
+  if (int rc = validate(e)) return rc;
+  long long n = 0;
+  const int rc = guarded([&] {
+    n = static_cast<long long>(e->nsb());
+    if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "mask buffer too small");
+    if (!e->launched) throw Failure(CELV2_ERR_STATE, "no launch");
+    const SrcSlot& s = e->srcs[e->cur_set];
+    if (!s.imask_used) throw Failure(CELV2_ERR_STATE, "no intra scores");
+    e->sync_all();
+    check(hipMemcpy(out, s.imask_used, static_cast<size_t>(n), hipMemcpyDeviceToHost), "hipMemcpy intra masks");
+  });
+  return rc ? rc : n;
+}
+
 int celv2hip_set_counting(celv2hip_engine* e, int on) {
   if (!e) return CELV2_ERR_ARG;
   e->counting = on != 0;

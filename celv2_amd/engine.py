@@ -61,6 +61,8 @@ def lib():
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_set_ref_limits.argtypes = [P, P, I]
         L.celv2hip_set_intra_masks.argtypes = [P, P, I]
+        L.celv2hip_fetch_intra_masks.argtypes = [P, P, ctypes.c_longlong]
+        L.celv2hip_fetch_intra_masks.restype = ctypes.c_longlong
         L.celv2hip_set_pruner_groups.argtypes = [P, ctypes.c_float]
         L.celv2hip_pruner_create.argtypes = [I, I, ctypes.c_uint, ctypes.POINTER(P)]
         L.celv2hip_pruner_destroy.argtypes = [P]
@@ -385,6 +387,14 @@ class RawEngine:
             return
         m = np.ascontiguousarray(m, dtype=np.uint8)
         self._check(lib().celv2hip_set_intra_masks(self.h, m.ctypes.data, int(m.size)), "set_intra_masks")
+
+    def fetch_intra_masks(self):
+        """The intra mode groups the last launch's source was scored with (uint8 per superblock)."""
+        out = np.zeros(self.sb_cols * self.sb_rows, np.uint8)
+        got = lib().celv2hip_fetch_intra_masks(self.h, out.ctypes.data, out.size)
+        if got < 0:
+            self._check(int(-got), "fetch_intra_masks")
+        return out
 
     def fetch_sb_costs(self):
         out = np.zeros(self.sb_cols * self.sb_rows, np.int64)

@@ -192,6 +192,9 @@ int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count);
  * computed. A source keeps the masks it was scored with. masks NULL / count 0: every group. No
  * counterpart in the reference: the intra dimension of the pruned form. */
 int celv2hip_set_intra_masks(celv2hip_engine* e, const uint8_t* masks, int count);
+/* The intra mode groups the last launch's source was scored with (one byte per superblock; the
+ * network's when it chose them); returns the count or a negative status. */
+long long celv2hip_fetch_intra_masks(celv2hip_engine* e, uint8_t* out, long long cap);
 
 /* The last launch's per-superblock decision cost (the 64x64 square's partition DP cost, SATD*256 +
  * lambda*bits units: J of the superblock) into out[0 .. superblocks); returns the count or a negative
