@@ -244,13 +244,13 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
     lam = engine.lambda_q8(qindex)
     w, cfg = pruner.load()
     op = pruner.operating_point(cfg, qindex)
-    threshold, band, gth = op if op else (None, None, 0.0)
+    threshold, band, gth, sth = op if op else (None, None, 0.0, 0.0)
     P = engine.Pruner(max_positions=(1920 // 64 + 1) * (1080 // 64 + 1), seed=1)
     P.set_params(w)
     E = engine.RawEngine(1920, 1080)
     E.set_resident(True)
     if op:
-        E.set_pruner(P, threshold, band, gthreshold=gth)
+        E.set_pruner(P, threshold, band, gthreshold=gth, sthreshold=sth)
 
     def step(s):
         i = decidable[s % len(decidable)]
@@ -300,7 +300,9 @@ def bench_learned(lum, decidable, NR, qindex, steps, warmup, plain_fps):
             "mean_reference_limit": round(float(ks.mean()), 3), "fallback_rate": round(c2["pruner_fallbacks"] / ks.size, 5),
             "threshold": threshold, "band": band, "stage_ms_per_frame": per, "roofline": roof,
             "inter_candidates_per_frame": round(inter_l), "compound_blocks_per_frame": round(c2["compound_blocks"] / nd),
-            "network": f"conv3x3 11-32, conv3x3 32-32, conv1x1 32-6, BF16 on MFMA, trained on {cfg['train_superblocks']} superblocks of synthetic clips (celv2_amd/pruner/train.py)"}
+            "network": f"conv3x3 {engine.FEATURES}-32, conv3x3 32-32, conv1x1 32-{engine.N_OUT} (6 reference, 3 intra-group, 1 search outputs), "
+                       f"BF16 on MFMA, trained on {cfg['train_superblocks']} superblocks of synthetic clips (celv2_amd/pruner/train.py)",
+            "search_threshold": sth}
 
 
 def stage_times(E, step, first, steps):

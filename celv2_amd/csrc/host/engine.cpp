@@ -165,6 +165,8 @@ struct celv2hip_engine {
   uint2* best = nullptr;
   uint8_t* kmap = nullptr;       // per-superblock reference limits on the device (when kmap_on)
   bool kmap_on = false;
+  uint8_t* smap = nullptr;       // per-superblock motion-search modes on the device (when smap_on)
+  bool smap_on = false;
   uint8_t* imask = nullptr;      // per-superblock intra mode groups on the device (when imask_on)
   bool imask_on = false;
   uint8_t* nmask = nullptr;      // the pruner's intra mode groups of the last launch
@@ -175,6 +177,9 @@ struct celv2hip_engine {
   // > 0: the pruner also chooses each superblock's intra mode groups (celv2hip_set_pruner_groups);
   // the source's intra scores then run after the network, on the side stream (intra_ev behind them).
   float prune_gthreshold = 0.f;
+  // > 0: the pruner also chooses each superblock's motion-search modes (celv2hip_set_pruner_search).
+  float prune_sthreshold = 0.f;
+  bool modes_last = false;  // the last launch's levels 1 and 0 obeyed per-superblock search modes (smap)
   hipEvent_t infer_ev = nullptr, intra_ev = nullptr;
   bool intra_ev_pending = false;
   bool prune_groups() const { return pruner && prune_gthreshold > 0.f; }
@@ -237,6 +242,7 @@ struct celv2hip_engine {
         if (f) (void)hipFree(f);
     if (best) (void)hipFree(best);
     if (kmap) (void)hipFree(kmap);
+    if (smap) (void)hipFree(smap);
     if (imask) (void)hipFree(imask);
     if (nmask) (void)hipFree(nmask);
     for (uint32_t* c : cost2)
@@ -446,6 +452,7 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     }
     check(hipMalloc(&e->best, e->nsb() * kMaxRefs * kBlocks * sizeof(uint2)), "hipMalloc best");
     check(hipMalloc(&e->kmap, e->nsb()), "hipMalloc kmap");
+    check(hipMalloc(&e->smap, e->nsb()), "hipMalloc search modes");
     check(hipMalloc(&e->sb_cost, e->nsb() * sizeof(int64_t)), "hipMalloc sb_cost");
     check(hipMalloc(&e->rec_dev, e->nsb() * sizeof(celv2_sb)), "hipMalloc records");
     check(hipHostMalloc(&e->rec_host, e->nsb() * sizeof(celv2_sb), hipHostMallocDefault), "hipHostMalloc records");
@@ -532,6 +539,22 @@ int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count) {
     e->sync_all();
     check(hipMemcpy(e->kmap, k, static_cast<size_t>(count), hipMemcpyHostToDevice), "hipMemcpy kmap");
     e->kmap_on = true;
+  });
+}
+
+int celv2hip_set_search_modes(celv2hip_engine* e, const uint8_t* modes, int count) {
+  if (int rc = validate(e)) return rc;
+  return guarded([&] {
+    if (!modes || count == 0) {
+      e->smap_on = false;
+      return;
+    }
+    if (count != static_cast<int>(e->nsb())) throw Failure(CELV2_ERR_SIZE, "one search mode per superblock");
+    for (int i = 0; i < count; i++)
+      if (modes[i] > 3) throw Failure(CELV2_ERR_ARG, "search modes are 2-bit level sets");
+    e->sync_all();
+    check(hipMemcpy(e->smap, modes, static_cast<size_t>(count), hipMemcpyHostToDevice), "hipMemcpy search modes");
+    e->smap_on = true;
   });
 }
 
@@ -736,6 +759,30 @@ int celv2hip_set_pruner_groups(celv2hip_engine* e, float gthreshold) {
   return CELV2_OK;
 }
 
+int celv2hip_set_pruner_search(celv2hip_engine* e, float sthreshold) {
+  if (!e) return CELV2_ERR_ARG;
+  if (!(sthreshold >= 0.f && sthreshold <= 1.f)) {
+    g_error = "search threshold is 0..1 (0: every search full)";
+    return CELV2_ERR_ARG;
+  }
+  e->prune_sthreshold = sthreshold;
+  return CELV2_OK;
+}
+
+long long celv2hip_fetch_search_modes(celv2hip_engine* e, uint8_t* out, long long cap) {
+  if (int rc = validate(e)) return rc;
+  long long n = 0;
+  const int rc = guarded([&] {
+    n = static_cast<long long>(e->nsb());
+    if (!out || cap < n) throw Failure(CELV2_ERR_SIZE, "mode buffer too small");
+    if (!e->launched) throw Failure(CELV2_ERR_STATE, "no launch");
+    e->sync_all();
+    if (e->modes_last) check(hipMemcpy(out, e->smap, static_cast<size_t>(n), hipMemcpyDeviceToHost), "hipMemcpy search modes");
+    else std::fill(out, out + n, static_cast<uint8_t>(0));  // every search full
+  });
+  return rc ? rc : n;
+}
+
 int celv2hip_set_features(celv2hip_engine* e, int on) {
   if (!e) return CELV2_ERR_ARG;
   e->want_features = on != 0;
@@ -921,8 +968,8 @@ void launch_batch(const LaunchIn* in, int n) {
   std::vector<char> intra_after(n);
   std::vector<SrcSlot*> cur(n);
   std::vector<std::vector<RefSlot*>> used(n);
-  std::vector<char> use_kmap(n);
-  for (int i = 0; i < n; i++) use_kmap[i] = in[i].e->kmap_on;
+  std::vector<char> use_kmap(n), use_smap(n);
+  for (int i = 0; i < n; i++) use_kmap[i] = in[i].e->kmap_on, use_smap[i] = in[i].e->smap_on;
   // Prepared sources whose intra scores the launch stream waits for only before finalize.
   std::vector<char> late_claim(n);
   // One engine with one row stream whose records go to the host (the wavefront form): level 0 of the
@@ -1040,6 +1087,7 @@ void launch_batch(const LaunchIn* in, int n) {
       a.level0 = l == 0, a.phase_bytes = used[i][0]->p.phase_bytes;
       a.searches = e->counters + 4 * kCounterWords;
       a.kmap = use_kmap[i] ? e->kmap : nullptr, a.sb_cols = e->sb_cols, a.sb_rows = e->sb_rows, a.level = l;
+      a.smap = use_smap[i] ? e->smap : nullptr;
       const bool feats = l == 2 && (e->pruner || e->want_features);
       for (int r = 0; r < nr; r++) {
         if (feats && !e->cost2[r]) check(hipMalloc(&e->cost2[r], static_cast<size_t>(e->fcols[2]) * e->frows[2] * sizeof(uint32_t)), "hipMalloc cost2");
@@ -1084,10 +1132,12 @@ void launch_batch(const LaunchIn* in, int n) {
         if (e->pruner) {
           const bool groups = intra_after[i] != 0;
           if (groups && !e->nmask) check(hipMalloc(&e->nmask, e->nsb()), "hipMalloc intra masks");
+          const bool modes = e->prune_sthreshold > 0.f;
           cnn_infer(e->infer.x, 1, e->sb_rows, e->sb_cols, e->pruner->params, e->pruner->pb, nullptr, in[i].nrefs,
                     e->prune_threshold, e->prune_band, e->kmap, e->counters + 3 * kCounterWords, S,
-                    groups ? e->nmask : nullptr, e->prune_gthreshold);
+                    groups ? e->nmask : nullptr, e->prune_gthreshold, modes ? e->smap : nullptr, e->prune_sthreshold);
           use_kmap[i] = true;
+          if (modes) use_smap[i] = true;
           if (groups) {
             // The source's intra scores with the network's groups, beside levels 1 and 0 (on the launch
             // stream in the graph form); finalize waits for them.
@@ -1292,6 +1342,7 @@ void launch_batch(const LaunchIn* in, int n) {
     e->release(*cur[i], S);
     e->nrefs_last = in[i].nrefs;
     e->limits_last = use_kmap[i] != 0;
+    e->modes_last = use_smap[i] != 0;
     e->last_refs = used[i];
     e->launched = true;
   }

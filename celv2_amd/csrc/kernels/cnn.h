@@ -13,7 +13,8 @@ constexpr int kFeatures = 14;       // inputs per superblock (pruner.hip)
 constexpr int kHidden = 32;         // channels of the two 3x3 layers
 constexpr int kRefOut = 6;          // output j < 6: needs more than j + 1 references
 constexpr int kGroupOut = 3;        // output 6 + g: needs intra mode group g + 1 (groups 1..3; group 0 always scored)
-constexpr int kOut = kRefOut + kGroupOut;
+constexpr int kSearchOut = 1;       // output 9: needs the full motion search at levels 1 and 0 (else narrow)
+constexpr int kOut = kRefOut + kGroupOut + kSearchOut;
 constexpr int kK1 = 128;            // im2col row of layer 1: 9 * kFeatures = 126, padded
 constexpr int kK2 = 9 * kHidden;    // im2col row of layer 2
 // Parameters: W1[9 kFeatures][kHidden], b1, W2[kK2][kHidden], b2, W3[kHidden][kOut], b3 (row k of a
@@ -37,7 +38,9 @@ struct CnnWork {
   float* dz1 = nullptr;
   uint16_t* dz1b = nullptr;
   float* loss_part = nullptr;
-  uint8_t* kstar = nullptr;  // labels: the references a superblock needs (1..7, bits 0..2) and the intra groups it needs (bit 3 + g - 1 for group g)
+  // labels: the references a superblock needs (1..7, bits 0..2), the intra groups it needs (bit 3 + g - 1
+  // for group g) and whether it needs the full motion search (bit 6)
+  uint8_t* kstar = nullptr;
   size_t cap = 0;
   size_t positions_capacity() const;
 };
@@ -54,10 +57,12 @@ void cnn_limits(const float* logits, int m, int nrefs, float threshold, float ba
 // Inference in one launch over x (b frames of h x wd superblocks): logits (if non-null, as cnn_forward
 // leaves them in w.z3, bit for bit), reference limits into kmap (if non-null, as cnn_limits) and, if
 // imask is non-null, each superblock's intra mode groups: group g (1..3) when sigma(logit 5 + g) >=
-// gthreshold, every group when a reference output was within band of its threshold (a fallback).
+// gthreshold, every group when a reference output was within band of its threshold (a fallback);
+// if smap is non-null, each superblock's motion-search modes: the full search (0) when sigma(logit 9)
+// >= sthreshold or on a fallback, else narrow at levels 1 and 0 (3).
 void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
                float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s,
-               uint8_t* imask = nullptr, float gthreshold = 0.f);
+               uint8_t* imask = nullptr, float gthreshold = 0.f, uint8_t* smap = nullptr, float sthreshold = 0.f);
 
 // The static scratch of the no-allocation forward pass: features [kStaticPositions][kFeatures]
 // followed by logits [kStaticPositions][kOut] (1.3 MB), one copy per device. cnn_scratch_init takes

@@ -105,6 +105,8 @@ struct Infer {
   float threshold, band;
   uint8_t* imask;  // [b][h][w] intra mode groups (may be null)
   float gthreshold;
+  uint8_t* smap;   // [b][h][w] motion-search modes (may be null)
+  float sthreshold;
 };
 
 // A superblock's intra mode groups from the group outputs (group 0 always; every group on a fallback).
@@ -116,6 +118,7 @@ __device__ __forceinline__ uint8_t groups_of(const float* z, float gthreshold, b
   return m;
 }
 
+static_assert(kOut <= 16, "the output layer is one 16-column tile");
 constexpr int kInferThreads = 512;  // 8 waves: layer 2's 8 tiles one per wave
 __global__ void __launch_bounds__(kInferThreads) k_infer(Infer a) {
   __shared__ uint16_t xs[kXH * kXW * kFeatures];
@@ -212,6 +215,7 @@ __global__ void __launch_bounds__(kInferThreads) k_infer(Infer a) {
         a.kmap[p] = static_cast<uint8_t>(limit_of(z, a.nrefs, a.threshold, a.band, unsure));
         if (unsure && a.fallbacks) count(a.fallbacks, static_cast<int>(p), 1ull);
         if (a.imask) a.imask[p] = groups_of(z, a.gthreshold, unsure);
+        if (a.smap) a.smap[p] = unsure || 1.f / (1.f + expf(-z[kRefOut + kGroupOut])) >= a.sthreshold ? 0 : 3;
       }
     }
   }
@@ -279,7 +283,8 @@ __global__ void k_bias_grad(const float* g, long long m, int n, float* db) {
 }
 
 // Binary cross-entropy with logits, labels y_j = (kstar > j + 1) for the reference outputs (kstar =
-// label bits 0..2) and y = label bit 3 + g for group output 6 + g; mean over m x kOut. Writes the
+// label bits 0..2), y = label bit 3 + g for group output 6 + g and y = label bit 6 for the search
+// output 9 (the same rule: bit 3 + j - 6); mean over m x kOut. Writes the
 // gradient (sigma(z) - y) / (m kOut), its bf16 copy, and per-block partial losses.
 __global__ void k_bce(const float* z, const uint8_t* kstar, long long m, float* dz, uint16_t* dzb, float* loss_part) {
   __shared__ float part[256];
@@ -422,9 +427,9 @@ void cnn_adamw(float* params, const float* grads, float* m1, float* m2, uint16_t
 
 void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
                float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s, uint8_t* imask,
-               float gthreshold) {
+               float gthreshold, uint8_t* smap, float sthreshold) {
   if (b <= 0 || h <= 0 || wd <= 0) return;
-  const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band, imask, gthreshold};
+  const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band, imask, gthreshold, smap, sthreshold};
   k_infer<<<dim3((wd + kTileW - 1) / kTileW, (h + kTileH - 1) / kTileH, b), kInferThreads, 0, s>>>(a);
 }
 

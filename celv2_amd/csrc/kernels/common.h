@@ -21,6 +21,7 @@ constexpr int kBlocks = 425;        // src/gpu/sb_blocks.h:10
 constexpr int kSquares = 85;        // src/gpu/sb_blocks.h:9
 constexpr int kRange = 8;           // src/gpu/inter_common.h:35, 40
 constexpr int kPelPenalty = 4;      // src/gpu/inter_common.h:55
+constexpr int kNarrowRange = 2;     // the pruned form's narrow full search (+-2 instead of +-kRange)
 constexpr int kQpelPenalty = 16;    // src/gpu/inter_common.h:140
 constexpr int kSplitBits = 2;       // src/gpu/intra_common.h:33
 constexpr int kIntraOnInter = 32;   // src/gpu/inter_common.h:50

@@ -44,6 +44,10 @@ struct MeLevelArgs {
   // At levels 0 and 1 a block is searched for reference r only when a superblock that keeps r reads
   // it: its own, or a neighbour whose ring it lies in (1 block at level 0, 2 at level 1).
   const uint8_t* kmap;
+  // Per-superblock search modes (may be null): bit L set narrows level L's full searches (L = 0, 1)
+  // to +-kNarrowRange around each centre. A block is searched narrow at level L only when every
+  // superblock that keeps the reference and reads the block (the same ring as above) has bit L set.
+  const uint8_t* smap;
   int sb_cols, sb_rows, level;
   uint32_t* cost_out[kMaxRefs];  // the searched best cost (SAD + distance penalty) per block (may be null)
   // The block rows this launch searches: [row0, row0 + nrun) (nrun 0: every row). rows stays the

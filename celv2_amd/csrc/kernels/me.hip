@@ -114,6 +114,53 @@ __device__ __forceinline__ uint32_t score_pair(const uint32_t (&t)[16], const ui
   return best;
 }
 
+// The narrow form of the two full searches (the pruned form's search mode): me_search with range
+// kNarrowRange instead of kRange around each centre, read straight from the plane (the two windows are
+// small enough to stay in the cache). Lane (window w, column dx, row group g) scores kNarrowRows
+// vertically adjacent positions from the kNarrowRows + 7 rows it loads; the last group starts early
+// enough to stay inside the window (positions scored twice give the same key). Keys as in the full
+// form: (SAD << 10) + (penalty << 10 | w << 9 | (dy + kRange - R) * 17 + dx + kRange - R), so the wave
+// minimum is the first least cost in dy-major order, window 1 before window 2 at equal cost, and
+// decodes as the full form's keys do.
+constexpr int kNarrowSide = 2 * kNarrowRange + 1;                         // positions per window side
+constexpr int kNarrowRows = kNarrowRange <= 2 ? 1 : kNarrowRange == 3 ? 2 : 3;  // positions per lane
+constexpr int kNarrowGroups = (kNarrowSide + kNarrowRows - 1) / kNarrowRows;
+constexpr int kNarrowLanes = kNarrowSide * kNarrowGroups;                // lanes per window
+constexpr int kNarrowWin = 2 * kNarrowRange + 8;                          // a narrow window's side
+static_assert(2 * kNarrowLanes <= 64 && kNarrowRange < kRange, "narrow search lanes");
+
+__device__ __forceinline__ uint32_t search_narrow(const BPlane& rb, int X1, int Y1, int X2, int Y2, bool two,
+                                                  const uint32_t c[16], int lane) {
+  const int w = lane >= kNarrowLanes ? 1 : 0, q = lane - kNarrowLanes * w;
+  const int g = q / kNarrowSide, dx = q - kNarrowSide * g;
+  const int dy0 = min(g * kNarrowRows, kNarrowSide - kNarrowRows);
+  const bool active = lane < 2 * kNarrowLanes && (w == 0 || two);
+  // Inactive lanes read window 1's first rows (in range) and are given no key.
+  const int X = w && two ? X2 : X1, Y = w && two ? Y2 : Y1;
+  const int off = corner_offset_lane(rb, Y + (active ? dy0 : 0), X + (active ? dx : 0));
+  const int base = off & ~3, sh = off & 3;
+  uint32_t rw[kNarrowRows + 7][2];
+#pragma unroll
+  for (int i = 0; i < kNarrowRows + 7; i++) {
+    uint32_t d0, d1, d2;
+    bload12(rb, base, i * rb.stride, d0, d1, d2);
+    rw[i][0] = __builtin_amdgcn_alignbyte(d1, d0, sh);
+    rw[i][1] = __builtin_amdgcn_alignbyte(d2, d1, sh);
+  }
+  uint32_t best = kNoPosition;
+#pragma unroll
+  for (int k = 0; k < kNarrowRows; k++) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) acc = sad4(c[2 * i + 1], rw[k + i][1], sad4(c[2 * i], rw[k + i][0], acc));
+    const int dy = dy0 + k;
+    const uint32_t pen = static_cast<uint32_t>(kPelPenalty * (abs(dx - kNarrowRange) + abs(dy - kNarrowRange)));
+    const uint32_t idx = static_cast<uint32_t>((dy + kRange - kNarrowRange) * 17 + dx + kRange - kNarrowRange);
+    best = min(best, ((acc + pen) << 10) | static_cast<uint32_t>(w) << 9 | idx);
+  }
+  return active ? best : kNoPosition;
+}
+
 // me_refine's 9 positions (src/gpu/inter_common.h:135-154), 2 bits each as offset + 1; 12..15 pad.
 constexpr uint32_t kRefDx = (1u << 0) | (0u << 2) | (1u << 4) | (2u << 6) | (0u << 8) | (2u << 10) | (0u << 12) | (1u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
 constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u << 8) | (1u << 10) | (2u << 12) | (2u << 14) | (2u << 16) | (1u << 18) | (1u << 20) | (1u << 22);
@@ -124,7 +171,7 @@ constexpr uint32_t kRefDy = (1u << 0) | (0u << 2) | (0u << 4) | (0u << 6) | (1u 
 // here by lane shuffles, not from memory again: the search is bound by the vector-memory data path),
 // the coarser MVs under and beside it are read by scalar loads where they are used.
 struct Opening {
-  uint32_t km = 0, mine = 0;
+  uint32_t km = 0, sm = 0, mine = 0;
 };
 
 struct BlockPos {
@@ -138,10 +185,11 @@ __device__ __forceinline__ int div_small(int n, int d, float inv) {
   return q;
 }
 __device__ __forceinline__ bool limited(const MeLevelArgs& a, int r) { return a.kmap && a.level <= 1 && r > 0; }
+__device__ __forceinline__ bool moded(const MeLevelArgs& a) { return a.smap && a.level <= 1; }
 
 __device__ __forceinline__ Opening open_block(const MeLevelArgs& a, const BlockPos& q, int lane) {
   Opening o;
-  if (limited(a, q.r)) {
+  if (limited(a, q.r) || moded(a)) {
     // A superblock's decisions read the level-0 field on it and a ring of 1 block (its inter
     // candidates), and those blocks' searches read the level-1 field on them and 1 more block each
     // way (their coarser centres): a ring of 2 at level 1. Lane 3 (dy + 1) + dx + 1 asks the
@@ -152,7 +200,11 @@ __device__ __forceinline__ Opening open_block(const MeLevelArgs& a, const BlockP
     const int sx = (q.bx >> shift) + dx, sy = (q.by >> shift) + dy, lx = q.bx & (n - 1), ly = q.by & (n - 1);
     const bool in_x = dx == 0 || (dx < 0 ? lx < reach : lx >= n - reach);
     const bool in_y = dy == 0 || (dy < 0 ? ly < reach : ly >= n - reach);
-    if (lane < 9 && in_x && in_y && sx >= 0 && sy >= 0 && sx < a.sb_cols && sy < a.sb_rows) o.km = a.kmap[sy * a.sb_cols + sx];
+    if (lane < 9 && in_x && in_y && sx >= 0 && sy >= 0 && sx < a.sb_cols && sy < a.sb_rows) {
+      const int s = sy * a.sb_cols + sx;
+      o.km = a.kmap ? a.kmap[s] : static_cast<uint32_t>(kMaxRefs);
+      o.sm = a.smap ? a.smap[s] : 0u;
+    }
   }
   const BPlane cb = as_buffer(a.cur);
   if (lane < 16) o.mine = bload4(cb, corner_offset(cb, 8 * q.by + (lane >> 1), 8 * q.bx + 4 * (lane & 1)), 0);
@@ -166,6 +218,8 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   const Opening o = open_block(a, q, lane);
   // Searched only near a superblock that keeps reference r.
   if (limited(a, r) && __ballot(o.km > static_cast<uint32_t>(r)) == 0) return;  // whole wave
+  // Narrow when no superblock that keeps r and reads the block asks for the full search.
+  const bool narrow = moded(a) && __ballot(o.km > static_cast<uint32_t>(r) && ((o.sm >> a.level) & 1u) == 0) == 0;
   const DPlane ref = a.ref[r];
   const BPlane rb = as_buffer(ref);
 
@@ -227,6 +281,13 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     }
   }
 
+  uint32_t key;
+  if (narrow) {
+    const int R = kNarrowRange;
+    const int X1 = clamp_origin_uniform(8 * bx + p1x - R, ref.w, kNarrowWin), Y1 = clamp_origin_uniform(8 * by + p1y - R, ref.h, kNarrowWin);
+    const int X2 = clamp_origin_uniform(8 * bx + p2x - R, ref.w, kNarrowWin), Y2 = clamp_origin_uniform(8 * by + p2y - R, ref.h, kNarrowWin);
+    key = wave_min_u32(search_narrow(rb, X1, Y1, X2, Y2, two, c, lane));
+  } else {
   // Full searches (me_search, src/gpu/inter_common.h:106-119) from windows staged in LDS.
   // Stage item k = lane + 64 i is word d of row r of a window (r = k / 6 = (171 k) >> 10 for k < 192,
   // d = k - 6 r); item 2 exists for lanes < 16 only.
@@ -250,7 +311,8 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   // window again (its keys lose every tie to the first window's).
   uint32_t tk[16];
   search_lanes(lane, tk);
-  const uint32_t key = wave_min_u32(score_pair(tk, win1, two ? win2 : win1, X1 & 3, two ? X2 & 3 : X1 & 3, c, lane));
+  key = wave_min_u32(score_pair(tk, win1, two ? win2 : win1, X1 & 3, two ? X2 & 3 : X1 & 3, c, lane));
+  }
   const uint32_t best_cost = key >> 10;
   const int ip = static_cast<int>(key & 511);
   const bool second = ((key >> 9) & 1) != 0;

@@ -11,8 +11,8 @@ from . import dlog, engine, gop, y4m
 def decide_y4m(path, log_path, refs=7, qindex=130, key=None, long_gop=True, pruner=None, operating_point=None,
                device=0, max_frames=None):
     """Decides every frame of `path`; key: per-frame keyframe flags (None: frame 0 only). With `pruner`
-    (an engine.Pruner) and `operating_point(qindex) -> (threshold, band, group threshold) or None`, each frame runs the
-    pruned form at its qindex's point. Returns the schedule's coded entries."""
+    (an engine.Pruner) and `operating_point(qindex) -> (threshold, band, group threshold, search threshold)
+    or None`, each frame runs the pruned form at its qindex's point. Returns the schedule's coded entries."""
     r = y4m.Y4mReader(path)
     frames = []
     for f in r:

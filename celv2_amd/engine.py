@@ -21,7 +21,7 @@ P = ctypes.c_void_p
 I = ctypes.c_int
 LL = ctypes.c_longlong
 FEATURES = 14  # the learned pruner's inputs per superblock (include/celv2hip.h)
-N_OUT = 9  # its outputs: 6 reference outputs, 3 intra mode-group outputs (CELV2HIP_PRUNER_OUTPUTS)
+N_OUT = 10  # its outputs: 6 reference outputs, 3 intra mode-group outputs, the search output (CELV2HIP_PRUNER_OUTPUTS)
 
 
 class Luma(ctypes.Structure):
@@ -61,6 +61,10 @@ def lib():
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_set_ref_limits.argtypes = [P, P, I]
         L.celv2hip_set_intra_masks.argtypes = [P, P, I]
+        L.celv2hip_set_search_modes.argtypes = [P, P, I]
+        L.celv2hip_fetch_search_modes.argtypes = [P, P, ctypes.c_longlong]
+        L.celv2hip_fetch_search_modes.restype = ctypes.c_longlong
+        L.celv2hip_set_pruner_search.argtypes = [P, ctypes.c_float]
         L.celv2hip_fetch_intra_masks.argtypes = [P, P, ctypes.c_longlong]
         L.celv2hip_fetch_intra_masks.restype = ctypes.c_longlong
         L.celv2hip_set_pruner_groups.argtypes = [P, ctypes.c_float]
@@ -379,6 +383,15 @@ class RawEngine:
         k = np.ascontiguousarray(k, dtype=np.uint8)
         self._check(lib().celv2hip_set_ref_limits(self.h, k.ctypes.data, int(k.size)), "set_ref_limits")
 
+    def set_search_modes(self, m=None):
+        """Per-superblock motion-search modes (uint8, bit L narrows level L's full searches, L = 0, 1;
+        row-major) for later launches; None: every search full."""
+        if m is None:
+            self._check(lib().celv2hip_set_search_modes(self.h, None, 0), "set_search_modes")
+            return
+        m = np.ascontiguousarray(m, dtype=np.uint8)
+        self._check(lib().celv2hip_set_search_modes(self.h, m.ctypes.data, int(m.size)), "set_search_modes")
+
     def set_intra_masks(self, m=None):
         """Per-superblock intra mode groups (uint8 bit sets, 0..15, row-major) for the sources scored
         from now on; None: every group."""
@@ -403,10 +416,20 @@ class RawEngine:
             self._check(int(got), "fetch_sb_costs")
         return out
 
-    def set_pruner(self, pruner, threshold=0.5, band=0.0, gthreshold=0.0):
-        """Attaches a pruner (None detaches); gthreshold > 0: it also chooses the intra mode groups."""
+    def set_pruner(self, pruner, threshold=0.5, band=0.0, gthreshold=0.0, sthreshold=0.0):
+        """Attaches a pruner (None detaches); gthreshold > 0: it also chooses the intra mode groups;
+        sthreshold > 0: and the motion-search modes."""
         self._check(lib().celv2hip_set_pruner(self.h, pruner.h if pruner is not None else None, threshold, band), "set_pruner")
         self._check(lib().celv2hip_set_pruner_groups(self.h, gthreshold if pruner is not None else 0.0), "set_pruner_groups")
+        self._check(lib().celv2hip_set_pruner_search(self.h, sthreshold if pruner is not None else 0.0), "set_pruner_search")
+
+    def fetch_search_modes(self):
+        """The motion-search modes the last launch's levels 1 and 0 obeyed (uint8 per superblock)."""
+        out = np.zeros(self.sb_cols * self.sb_rows, np.uint8)
+        got = lib().celv2hip_fetch_search_modes(self.h, out.ctypes.data, out.size)
+        if got < 0:
+            self._check(int(got), "fetch_search_modes")
+        return out
 
     def set_features(self, on):
         self._check(lib().celv2hip_set_features(self.h, 1 if on else 0), "set_features")

@@ -16,9 +16,12 @@ def load(directory=HERE):
 
 
 def operating_point(cfg, qindex):
-    """(threshold, band, group threshold) chosen for the trained qindex nearest to `qindex`, or None
-    where no threshold keeps every held-out clip within the budget short of keeping all references
-    (run plain there). A group threshold of 0 scores every intra mode group."""
+    """(threshold, band, group threshold, search threshold) chosen for the trained qindex nearest to
+    `qindex`, or None where no threshold keeps every held-out clip within the budget short of keeping
+    all references (run plain there). A group threshold of 0 scores every intra mode group, a search
+    threshold of 0 keeps every motion search full."""
     ops = cfg["operating_points"]
     q = min(ops, key=lambda k: abs(int(k) - qindex))
-    return (ops[q]["threshold"], ops[q]["band"], ops[q].get("gthreshold", 0.0)) if ops[q] else None
+    if not ops[q]:
+        return None
+    return (ops[q]["threshold"], ops[q]["band"], ops[q].get("gthreshold", 0.0), ops[q].get("sthreshold", 0.0))

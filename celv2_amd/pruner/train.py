@@ -5,19 +5,24 @@ with the grain sigma, object speed and pan drawn per clip; some are flat-shaded 
 (workload 4 at 1080p). Each clip is decided at one qindex from {100, 130, 160, 190}. For every
 decidable frame (frame i >= 7 against the 7 source frames before it, nearest first) the engine
 decides the frame with its nearest k = 1..7 references and records each superblock's decision cost
-J_k (celv2hip_fetch_sb_costs) and, at k = 7, the pruner's features (celv2hip_fetch_features). Each
-superblock's decision depends only on its own limit, so the total cost of any per-superblock
-allocation k(s) follows from these logs exactly.
+J_k (celv2hip_fetch_sb_costs) and, at k = 7, the pruner's features (celv2hip_fetch_features), the
+cost with each optional intra mode group left out, and the cost with every motion search narrow at
+levels 1 and 0. Each superblock's decision depends only on its own limit, so the total cost of any
+per-superblock allocation k(s) follows from these logs exactly (the group and search costs are each
+taken alone at 7 references: their effects are summed).
 
 The budget is per sequence (a BD-rate is measured on one sequence): a clip's cost regret is
 sum(J_k(s) - J_7) / sum(J_7) over its superblocks. Labels: per clip, the allocation that keeps the
 fewest references within the target regret: with a multiplier mu, k*(s) = argmin_k k + mu (J_k(s) -
-J_7(s)), mu set by bisection so the clip's label regret equals the target. The network (BF16 on the
-matrix cores, celv2hip_pruner_*) learns P(k* > j + 1) with AdamW on the GPU.
+J_7(s)), mu set by bisection so the clip's label regret equals the target; an intra group, or the
+full search, is labelled needed where leaving it out raises the superblock's cost. The network (BF16
+on the matrix cores, celv2hip_pruner_*) learns P(k* > j + 1), P(group needed) and P(full search
+needed) with AdamW on the GPU.
 
-Operating points: on held-out clips, per qindex, the (threshold, fallback band) with the fewest
-references per superblock whose regret on every held-out clip of that qindex stays within the
-target, measured again through the engine itself (the network run inside the launch). Writes
+Operating points: on held-out clips, per qindex, first the highest group threshold and the highest
+search threshold whose regret on every held-out clip stays within their shares of the target, then
+the (threshold, fallback band) with the fewest references per superblock whose regret stays within
+the rest, measured again through the engine itself (the network run inside the launch). Writes
 weights.f32 (the FP32 parameters) and pruner.json.
 
     python -m celv2_amd.pruner.train [--clips 120] [--val-clips 16] [--steps 6000] [--out DIR]
@@ -57,14 +62,15 @@ def clip_spec(seed):
 
 
 def clip_logs(E, seed, y16):
-    """(features [n, sb_rows, sb_cols, F], costs [n, sbs, 7], group costs [n, sbs, 3]) of a clip's
-    decidable frames: J_k with the nearest k references, and J_7 with each of the intra mode groups
-    1..3 left out in turn. Reference keys are unique across clips (seed * 1000 + frame), so the
-    engine's cached planes never mix clips."""
+    """(features [n, sb_rows, sb_cols, F], costs [n, sbs, 7], group costs [n, sbs, 3], narrow costs
+    [n, sbs]) of a clip's decidable frames: J_k with the nearest k references, J_7 with each of the
+    intra mode groups 1..3 left out in turn, and J_7 with every superblock's motion search narrow at
+    levels 1 and 0. Reference keys are unique across clips (seed * 1000 + frame), so the engine's
+    cached planes never mix clips."""
     lam = engine.lambda_q8(clip_spec(seed)[1])
     dev = [engine.DeviceBuffer(f) for f in y16]
     lum = [engine.RawEngine.luma(dev16=b.ptr, bit_depth=10) for b in dev]
-    feats, costs, gcosts = [], [], []
+    feats, costs, gcosts, ncosts = [], [], [], []
     nsb = E.sb_cols * E.sb_rows
     for f in range(NR, len(y16)):
         refs = [f - k for k in range(1, NR + 1)]
@@ -86,10 +92,15 @@ def clip_logs(E, seed, y16):
             G.append(E.fetch_sb_costs())
         E.set_intra_masks(None)
         gcosts.append(np.stack(G, axis=1))
+        E.set_search_modes(np.full(nsb, 3, np.uint8))
+        E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
+        E.sync()
+        ncosts.append(E.fetch_sb_costs())
+        E.set_search_modes(None)
     E.set_features(False)
     for b in dev:
         b.free()
-    return np.stack(feats), np.stack(costs), np.stack(gcosts)
+    return np.stack(feats), np.stack(costs), np.stack(gcosts), np.stack(ncosts)
 
 
 def gather(E, seeds, keep=None):
@@ -145,10 +156,38 @@ def group_labels(costs, gcosts):
     return (need.astype(np.uint8) << np.array([3, 4, 5], np.uint8)).sum(-1).astype(np.uint8)
 
 
+def search_labels(costs, ncosts):
+    """Label bit 6: the superblock needs the full motion search where the narrow search at levels 1
+    and 0 raises its decision cost at 7 references."""
+    return ((ncosts > costs[..., NR - 1]).astype(np.uint8) << 6).astype(np.uint8)
+
+
+def search_regret(logits, costs, ncosts, sthreshold):
+    """The cost regret of the narrow search where the search output falls below sthreshold (each
+    superblock's cost with every search narrow, at 7 references), and the share of superblocks narrow."""
+    s = 1.0 / (1.0 + np.exp(-logits[:, 9].astype(np.float64)))
+    narrow = s < sthreshold
+    c = costs.reshape(-1, NR)[:, NR - 1].astype(np.float64)
+    n = ncosts.reshape(-1).astype(np.float64)
+    return float(((n - c) * narrow).sum() / c.sum()), float(narrow.mean())
+
+
+def search_point(logits, costs, ncosts, budget):
+    """The highest search threshold (the most superblocks narrow) whose regret on every clip is within
+    budget; logits, costs and ncosts are lists over clips."""
+    best = None
+    for sth in np.concatenate([[0.005, 0.01, 0.02, 0.03, 0.04], np.arange(0.05, 0.96, 0.05)]):
+        rs = [search_regret(z, c, n, sth) for z, c, n in zip(logits, costs, ncosts)]
+        if max(r for r, _ in rs) <= budget:
+            best = {"sthreshold": round(float(sth), 3), "regret": max(r for r, _ in rs),
+                    "narrow_share": float(np.mean([k for _, k in rs]))}
+    return best
+
+
 def group_regret(logits, costs, gcosts, gthreshold):
     """The cost regret of leaving out the groups whose output falls below gthreshold (each group's
     cost taken alone at 7 references: the groups' effects summed), and the groups kept per superblock."""
-    s = 1.0 / (1.0 + np.exp(-logits[:, 6:].astype(np.float64)))
+    s = 1.0 / (1.0 + np.exp(-logits[:, 6:9].astype(np.float64)))
     drop = s < gthreshold
     c = costs.reshape(-1, NR)[:, NR - 1].astype(np.float64)
     g = gcosts.reshape(-1, 3).astype(np.float64)
@@ -199,8 +238,8 @@ def operating_point(logits, costs, target):
 def engine_check(E, P, op, seeds, clips):
     """An operating point through the engine itself on clips (seed -> y16 frames): (the largest clip
     regret against the plain engine's J_7, mean limit, fallback rate)."""
-    gth = op.get("gthreshold", 0.0)
-    E.set_pruner(P, op["threshold"], op["band"], gthreshold=gth)
+    gth, sth = op.get("gthreshold", 0.0), op.get("sthreshold", 0.0)
+    E.set_pruner(P, op["threshold"], op["band"], gthreshold=gth, sthreshold=sth)
     E.stats_reset()
     worst = 0.0
     ks = []
@@ -217,7 +256,7 @@ def engine_check(E, P, op, seeds, clips):
             E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
             E.sync()
             j7 = E.fetch_sb_costs().astype(np.float64)
-            E.set_pruner(P, op["threshold"], op["band"], gthreshold=gth)
+            E.set_pruner(P, op["threshold"], op["band"], gthreshold=gth, sthreshold=sth)
             E.launch(lum[f], [lum[r] for r in refs], lam, keys=keys)
             E.sync()
             num += (E.fetch_sb_costs() - j7).sum()
@@ -239,6 +278,7 @@ def main():
     ap.add_argument("--label-target", type=float, default=0.004, help="the total regret of the training labels")
     ap.add_argument("--target", type=float, default=0.004, help="largest total cost regret an operating point may have")
     ap.add_argument("--group-budget", type=float, default=0.001, help="the part of the target the intra groups may take")
+    ap.add_argument("--search-budget", type=float, default=0.0005, help="the part of the target the narrow searches may take")
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--lr", type=float, default=3e-3)
@@ -258,14 +298,16 @@ def main():
     log(f"validation logs: {len(val_seeds)} clips in {time.perf_counter() - t0:.0f}s")
 
     # Labels per clip.
-    labels = {s: optimal_labels(tl[s][1], args.label_target)[0] | group_labels(tl[s][1], tl[s][2]) for s in train_seeds}
+    labels = {s: optimal_labels(tl[s][1], args.label_target)[0] | group_labels(tl[s][1], tl[s][2]) | search_labels(tl[s][1], tl[s][3])
+              for s in train_seeds}
     for q in QINDICES:
         ss = [s for s in train_seeds if clip_spec(s)[1] == q]
         lab = np.concatenate([labels[s] for s in ss]) if ss else np.zeros(0, np.uint8)
         k = lab & 7
         gneed = [float(((lab >> (3 + g)) & 1).mean()) if lab.size else 0.0 for g in range(3)]
+        sneed = float(((lab >> 6) & 1).mean()) if lab.size else 0.0
         log(f"qindex {q}: {len(ss)} clips, label mean refs {k.mean():.3f}, k* histogram {np.bincount(k.ravel(), minlength=8)[1:].tolist()}, "
-            f"groups needed {[round(v, 4) for v in gneed]}")
+            f"groups needed {[round(v, 4) for v in gneed]}, full search needed {sneed:.4f}")
     xt = np.concatenate([tl[s][0] for s in train_seeds])
     kt = np.concatenate([labels[s] for s in train_seeds])
     rows, cols = xt.shape[1], xt.shape[2]
@@ -289,9 +331,13 @@ def main():
                   for s in ss]
         costs = [vl[s][1] for s in ss]
         gcosts = [vl[s][2] for s in ss]
+        ncosts = [vl[s][3] for s in ss]
         gp = group_point(logits, costs, gcosts, args.group_budget)
-        op = operating_point([z[:, :6] for z in logits], costs, args.target - (gp["regret"] if gp else 0.0))
+        sp = search_point(logits, costs, ncosts, args.search_budget)
+        op = operating_point([z[:, :6] for z in logits], costs,
+                             args.target - (gp["regret"] if gp else 0.0) - max(0.0, sp["regret"] if sp else 0.0))
         op["gthreshold"] = gp["gthreshold"] if gp else 0.0
+        op["sthreshold"] = sp["sthreshold"] if sp else 0.0
         r, mk, fb = engine_check(E, P, op, ss, val_clips)
         best = [optimal_labels(c, args.target)[0] for c in costs]
         # The rule without the network: the nearest k references everywhere, k the fewest within
@@ -299,14 +345,16 @@ def main():
         nearest = {k: max(regret(c, np.full(c.shape[:2], k))[0] for c in costs) for k in range(1, NR + 1)}
         k_plain = min(k for k in range(1, NR + 1) if nearest[k] <= args.target)
         # An operating point that keeps every reference everywhere is no pruning: the plain engine.
-        ops[str(q)] = {"threshold": op["threshold"], "band": op["band"], "gthreshold": op["gthreshold"]} if mk < NR - 0.05 else None
+        ops[str(q)] = ({"threshold": op["threshold"], "band": op["band"], "gthreshold": op["gthreshold"], "sthreshold": op["sthreshold"]}
+                       if mk < NR - 0.05 else None)
         report[str(q)] = {"val_clips": len(ss), "val_frames": int(sum(c.shape[0] for c in costs)),
                           "worst_clip_regret": r, "mean_refs": mk, "fallback_rate": fb,
                           "simulated_worst_clip_regret": op["regret"], "simulated_mean_refs": op["mean_refs"],
                           "best_allocation_mean_refs": float(np.mean(np.concatenate([b.ravel() for b in best]))),
                           "nearest_k_worst_clip_regret": {str(k): v for k, v in nearest.items()},
                           "nearest_k_within_target": k_plain,
-                          "groups": gp or {"gthreshold": 0.0, "regret": 0.0, "groups_kept": 3.0}}
+                          "groups": gp or {"gthreshold": 0.0, "regret": 0.0, "groups_kept": 3.0},
+                          "search": sp or {"sthreshold": 0.0, "regret": 0.0, "narrow_share": 0.0}}
         log(f"qindex {q}: {json.dumps(ops[str(q)])} worst clip regret {r:.5f}, mean limit {mk:.3f} "
             f"(best allocation {report[str(q)]['best_allocation_mean_refs']:.3f}, nearest-k within target: {k_plain})")
         if args.logs and q == 130:

@@ -184,6 +184,18 @@ int celv2hip_graph_stats(const celv2hip_engine* e, long long* launches, long lon
  * superblocks of the frame, or k NULL / count 0 to decide every superblock with all references. No
  * counterpart in the reference's DecisionEngine: the hook of the learned pruner. */
 int celv2hip_set_ref_limits(celv2hip_engine* e, const uint8_t* k, int count);
+/* Per-superblock motion-search modes for later launches (the pruned form's search dimension): bit L
+ * of modes[s] (L = 0, 1) narrows the full searches of motion-search level L from +-8 to +-2 samples
+ * around each of the two centres (me_search, src/gpu/inter_common.h:106-119, with a smaller range;
+ * the centres, their ranking and the refinement are unchanged). A block at level L is searched narrow
+ * only when every superblock that keeps the reference and reads the block (its own, or a neighbour
+ * whose ring it lies in: 1 block at level 0, 2 at level 1) has bit L set; so each superblock is decided
+ * exactly as the oracle decides it under the same modes and limits. modes NULL / count 0: every
+ * search full. No counterpart in the reference: the motion-search dimension of the pruned form. */
+int celv2hip_set_search_modes(celv2hip_engine* e, const uint8_t* modes, int count);
+/* The search modes the last launch's levels 1 and 0 obeyed (set, or chosen by the pruner; 0 where
+ * every search was full) into out[0 .. superblocks); returns the count or a negative status. */
+long long celv2hip_fetch_search_modes(celv2hip_engine* e, uint8_t* out, long long cap);
 /* Per-superblock intra mode groups for the sources scored from now on (the next prepare() or
  * launch): bit g of masks[s] set = group g scored (0: DC, V, H, always scored; 1: smooth, smooth V,
  * smooth H, Paeth; 2: the directional modes at 45, 67, 113 degrees; 3: at 135, 157, 203 degrees).
@@ -213,8 +225,9 @@ long long celv2hip_fetch_sb_costs(celv2hip_engine* e, int64_t* out, long long ca
 typedef struct celv2hip_pruner celv2hip_pruner;
 #define CELV2HIP_PRUNER_FEATURES 14
 /* Outputs per superblock: 6 reference outputs (j: needs more than j + 1 references), then 3 intra
- * mode-group outputs (6 + g - 1: needs group g, g = 1..3). */
-#define CELV2HIP_PRUNER_OUTPUTS 9
+ * mode-group outputs (6 + g - 1: needs group g, g = 1..3), then the search output (9: needs the full
+ * motion search at levels 1 and 0). */
+#define CELV2HIP_PRUNER_OUTPUTS 10
 
 /* Number of FP32 parameters of the network. */
 int celv2hip_pruner_param_count(void);
@@ -224,7 +237,7 @@ int celv2hip_pruner_destroy(celv2hip_pruner* p);
 int celv2hip_pruner_get_params(celv2hip_pruner* p, float* out, int cap);
 /* Loads parameters (n = param count) and resets the AdamW state. */
 int celv2hip_pruner_set_params(celv2hip_pruner* p, const float* in, int n);
-/* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 14 + i]: the 9 logits per superblock. */
+/* b frames of h x w superblocks, feats[((f * h + y) * w + x) * 14 + i]: the 10 logits per superblock. */
 int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
 /* The same forward pass without any device allocation: features and logits are staged in a static
  * device array of the library (one per device, taken at celv2hip_pruner_create), so a caller that
@@ -234,7 +247,8 @@ int celv2hip_pruner_forward(celv2hip_pruner* p, const float* feats, int b, int h
  * entry point for callers that cannot allocate device memory. */
 int celv2hip_pruner_forward_static(const celv2hip_pruner* p, const float* feats, int b, int h, int w, float* logits);
 /* Mean binary cross-entropy against labels kstar (per superblock: the references it needs, 1..7, in
- * bits 0..2, and bit 2 + g set when it needs intra mode group g = 1..3) and its gradient with respect
+ * bits 0..2, bit 2 + g set when it needs intra mode group g = 1..3, and bit 6 set when it needs the
+ * full motion search) and its gradient with respect
  * to every parameter, without updating them. */
 int celv2hip_pruner_gradients(celv2hip_pruner* p, const float* feats, const uint8_t* kstar, int b, int h, int w,
                               float* grads, float* loss);
@@ -251,6 +265,11 @@ int celv2hip_set_pruner(celv2hip_engine* e, const celv2hip_pruner* p, float thre
  * the network (prepare() leaves them to the launch). 0: every group (the default). No counterpart in
  * the reference. */
 int celv2hip_set_pruner_groups(celv2hip_engine* e, float gthreshold);
+/* sthreshold > 0: the attached pruner also chooses each superblock's motion-search modes (as
+ * celv2hip_set_search_modes): the full search when the probability of output 9 is >= sthreshold or
+ * the superblock fell back to all references, else narrow at levels 1 and 0. 0: every search full
+ * (the default). No counterpart in the reference: the network's fallback to the full search. */
+int celv2hip_set_pruner_search(celv2hip_engine* e, float sthreshold);
 /* on != 0: launches compute the pruner's features even without a pruner (training data). */
 int celv2hip_set_features(celv2hip_engine* e, int on);
 /* The last launch's features (superblocks x 14 floats); returns the count or a negative status. */

@@ -92,6 +92,32 @@ int orc_decide_frame_masked(int w, int h, const uint8_t* src, const uint8_t* con
   return rc;
 }
 
+// The pruned form's restrictions together (each may be null): per-superblock reference limits
+// (1..7), motion-search modes (bit L: narrow full searches at level L = 0, 1) and intra mode groups;
+// costs (may be null): each superblock's decision cost.
+int orc_decide_frame_pruned(int w, int h, const uint8_t* src, const uint8_t* const* refs, int nrefs, int64_t lam,
+                            int sb_size, const uint8_t* limits, const uint8_t* modes, const uint8_t* masks,
+                            celv2_sb* out, int64_t* costs) {
+  g_sb_cols = (w + sb_size - 1) / sb_size, g_sb_rows = (h + sb_size - 1) / sb_size;
+  g_ref_limits = limits, g_search_modes = modes;
+  g_intra_masks = masks, g_intra_mask_cols = g_sb_cols;
+  int rc = 0;
+  try {
+    const Luma s = make_luma(w, h, src);
+    const std::vector<Luma> r = lumas(w, h, refs, nrefs);
+    std::vector<const Luma*> rp;
+    for (const Luma& l : r) rp.push_back(&l);
+    const Map m = decide_frame(s, rp, sb_size, lam);
+    export_map(m, out);
+    if (costs)
+      for (size_t i = 0; i < m.sbs.size(); i++) costs[i] = m.sbs[i].cost;
+  } catch (const std::exception& e) {
+    rc = fail(e);
+  }
+  g_ref_limits = g_search_modes = g_intra_masks = nullptr;
+  return rc;
+}
+
 // The threaded CpuEngine. fail_row >= 0: the row hook throws std::runtime_error("row hook") there.
 void* orc_engine_create(int w, int h, int sb_size, int threads, int fail_row) {
   try {
@@ -197,6 +223,30 @@ int orc_search_cell(const uint8_t* cur, const uint8_t* ref, const uint8_t* ref_p
   const int cost = search_cell(pc, pr, bx, by, pmx, pmy, pcols, prows, &mv[0], &mv[1], level0 != 0);
   if (level0 && ref_phases) refine_qpel(pc, pr, bx, by, &mv[0], &mv[1]);
   return cost;
+}
+
+// The same search with the full searches' range given (the pruned form's narrow search: range
+// kNarrowRange), and me_search alone: +-range around (px, py), the winning cost returned.
+int orc_search_cell_range(const uint8_t* cur, const uint8_t* ref, const uint8_t* ref_phases, int w, int h, int bx, int by,
+                          const int* pmx, const int* pmy, int pcols, int prows, int level0, int range, int* mv) {
+  const Plane8 pc{cur, w, w, h, 0};
+  const Plane8 pr{ref_phases ? ref_phases : ref, w, w, h, ref_phases ? static_cast<size_t>(w) * h : 0};
+  const int cost = search_cell(pc, pr, bx, by, pmx, pmy, pcols, prows, &mv[0], &mv[1], level0 != 0, range);
+  if (level0 && ref_phases) refine_qpel(pc, pr, bx, by, &mv[0], &mv[1]);
+  return cost;
+}
+int orc_full_search(const uint8_t* cur, const uint8_t* ref, int w, int h, int bx, int by, int px, int py, int range, int* mv) {
+  return full_search(Plane8{cur, w, w, h, 0}, Plane8{ref, w, w, h, 0}, bx, by, px, py, range, &mv[0], &mv[1]);
+}
+// Which blocks of motion-search level `level` (cols x rows) are searched narrow for reference r under
+// per-superblock limits (may be null) and search modes on a sb_cols x sb_rows grid: out[i] = 1 / 0.
+int orc_narrow_map(int level, int cols, int rows, int sb_cols, int sb_rows, const uint8_t* limits, const uint8_t* modes,
+                   int r, uint8_t* out) {
+  g_sb_cols = sb_cols, g_sb_rows = sb_rows, g_ref_limits = limits, g_search_modes = modes;
+  for (int by = 0; by < rows; by++)
+    for (int bx = 0; bx < cols; bx++) out[by * cols + bx] = narrow_search(level, bx, by, r) ? 1 : 0;
+  g_ref_limits = g_search_modes = nullptr;
+  return 0;
 }
 
 // SATDs on raw planes (w x h, stride w; ref with phases if ref_phases).

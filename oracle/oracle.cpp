@@ -151,7 +151,7 @@ void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx,
 // above and below neighbours, then zero), each distinct one ranked by SAD; the best and the
 // runner-up are both searched and the runner-up's result kept only if strictly cheaper.
 int search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
-                int prows, int* mvx, int* mvy, bool /*level0*/) {
+                int prows, int* mvx, int* mvy, bool /*level0*/, int range) {
   int c1x = 0, c1y = 0, c2x = 0, c2y = 0;
   bool have2 = false;
   if (pmx) {
@@ -174,10 +174,10 @@ int search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int*
       }
     }
   }
-  const int cost1 = full_search(cur, ref, bx, by, c1x, c1y, kRange, mvx, mvy);
+  const int cost1 = full_search(cur, ref, bx, by, c1x, c1y, range, mvx, mvy);
   if (have2) {
     int mx, my;
-    const int cost2 = full_search(cur, ref, bx, by, c2x, c2y, kRange, &mx, &my);
+    const int cost2 = full_search(cur, ref, bx, by, c2x, c2y, range, &mx, &my);
     if (cost2 < cost1) {
       *mvx = mx, *mvy = my;
       return cost2;
@@ -469,25 +469,54 @@ Pyramid build_pyramid(const Luma& f, bool phases) {
 }
 
 // kernels_ref.cpp:240-251.
+// The pruned form's restrictions for the decision in progress on this thread (null: none). Reference
+// limits: superblock s is decided with its nearest g_ref_limits[s] references. Search modes: bit L of
+// g_search_modes[s] (L = 0, 1) asks for the narrow full search (+-kNarrowRange) at level L. A level-L
+// block is read by the superblock it lies in and by each neighbour whose ring holds it (a superblock's
+// inter candidates read the level-0 field one block beyond it, and those blocks' centres the level-1
+// field one block beyond them: a ring of 1 block at level 0, 2 at level 1); it is searched narrow for
+// reference r when every such superblock that keeps r asks for it.
+thread_local const uint8_t* g_ref_limits = nullptr;
+thread_local const uint8_t* g_search_modes = nullptr;
+thread_local int g_sb_cols = 0, g_sb_rows = 0;
+
+bool narrow_search(int level, int bx, int by, int r) {
+  if (!g_search_modes || level > 1) return false;
+  const int shift = 3 - level, n = 1 << shift, reach = level == 0 ? 1 : 2;
+  const int lx = bx & (n - 1), ly = by & (n - 1);
+  for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      const int sx = (bx >> shift) + dx, sy = (by >> shift) + dy;
+      if (sx < 0 || sy < 0 || sx >= g_sb_cols || sy >= g_sb_rows) continue;
+      const bool in_x = dx == 0 || (dx < 0 ? lx < reach : lx >= n - reach);
+      const bool in_y = dy == 0 || (dy < 0 ? ly < reach : ly >= n - reach);
+      if (!in_x || !in_y) continue;
+      const int s = sy * g_sb_cols + sx;
+      const int k = g_ref_limits ? g_ref_limits[s] : kMaxRefs;
+      if (k > r && !((g_search_modes[s] >> level) & 1)) return false;
+    }
+  return true;
+}
+
 void search_rows(const Pyramid& cur, const Pyramid& ref, int level, const Field* coarser, int row0, int row1,
-                 Field& out) {
+                 Field& out, int ref_index) {
   for (int by = row0; by < row1; by++)
     for (int bx = 0; bx < out.cols; bx++) {
       const size_t i = static_cast<size_t>(by) * out.cols + bx;
       search_cell(cur.plane[level], ref.plane[level], bx, by, coarser ? coarser->mvx.data() : nullptr,
                   coarser ? coarser->mvy.data() : nullptr, coarser ? coarser->cols : 0, coarser ? coarser->rows : 0,
-                  &out.mvx[i], &out.mvy[i], level == 0);
+                  &out.mvx[i], &out.mvy[i], level == 0, narrow_search(level, bx, by, ref_index) ? kNarrowRange : kRange);
       if (level == 0) refine_qpel(cur.plane[0], ref.plane[0], bx, by, &out.mvx[i], &out.mvy[i]);
     }
 }
 
-void search_all(const Pyramid& cur, const Pyramid& ref, std::vector<Field>& lv) {
+void search_all(const Pyramid& cur, const Pyramid& ref, std::vector<Field>& lv, int ref_index) {
   lv.assign(kLevels, Field{});
   for (int l = kLevels - 1; l >= 0; l--) {
     Field& o = lv[l];
     o.cols = (cur.plane[l].w + 7) / 8, o.rows = (cur.plane[l].h + 7) / 8;
     o.mvx.assign(static_cast<size_t>(o.cols) * o.rows, 0), o.mvy = o.mvx;
-    search_rows(cur, ref, l, l == kLevels - 1 ? nullptr : &lv[l + 1], 0, o.rows, o);
+    search_rows(cur, ref, l, l == kLevels - 1 ? nullptr : &lv[l + 1], 0, o.rows, o, ref_index);
   }
 }
 
@@ -667,7 +696,12 @@ Sb decide_sb(const Luma& src, const RefSet& rs, int sb_col, int sb_row, int sb_s
     }
   }
   const int ox = sb_col * sb_size, oy = sb_row * sb_size;
-  decide_square(s, rs, ox, oy, sb_size, lam, sb, sb.cands.empty() ? nullptr : &sb.cands, ox, oy);
+  RefSet near = rs;  // the superblock's nearest references under a limit
+  if (g_ref_limits) {
+    const size_t k = std::min<size_t>(near.plane.size(), g_ref_limits[sb_row * g_sb_cols + sb_col]);
+    near.plane.resize(k), near.field.resize(k);
+  }
+  sb.cost = decide_square(s, near, ox, oy, sb_size, lam, sb, sb.cands.empty() ? nullptr : &sb.cands, ox, oy);
   return sb;
 }
 
@@ -678,7 +712,7 @@ Map decide_frame(const Luma& src, const std::vector<const Luma*>& refs, int sb_s
   RefSet rs;
   for (const Luma* f : refs) pyr.push_back(build_pyramid(*f, true));
   for (size_t r = 0; r < refs.size(); r++) {
-    search_all(cur, pyr[r], lv[r]);
+    search_all(cur, pyr[r], lv[r], static_cast<int>(r));
     rs.plane.push_back(&pyr[r].plane[0]);
     rs.field.push_back(&lv[r][0]);
   }

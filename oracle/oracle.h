@@ -36,6 +36,7 @@ constexpr int kSquares = 85;       // sb_blocks.h:9
 constexpr int kBlocks = 425;       // sb_blocks.h:10
 constexpr int kRange = 8;          // inter_common.h:35, 40 (both levels)
 constexpr int kPelPenalty = 4;     // inter_common.h:55
+constexpr int kNarrowRange = 2;    // the pruned form's narrow full search (no counterpart in the reference)
 constexpr int kQpelPenalty = 16;   // inter_common.h:140
 constexpr int kIntraMaxSize = 32;  // intra_common.h:39
 constexpr int kInterMaxSize = 64;  // intra_common.h:40
@@ -78,7 +79,7 @@ int satd_8x8_q(const Plane8& cur, int x, int y, const Plane8& ref, int qmx, int 
 void refine_qpel(const Plane8& cur, const Plane8& ref, int bx, int by, int* mvx, int* mvy);  // inter_common.h:135-154
 // Returns the integer search's winning cost (SAD + distance penalty from its centre).
 int search_cell(const Plane8& cur, const Plane8& ref, int bx, int by, const int* pmx, const int* pmy, int pcols,
-                int prows, int* mvx, int* mvy, bool level0);                          // inter_common.h:156-193
+                int prows, int* mvx, int* mvy, bool level0, int range = kRange);                          // inter_common.h:156-193
 int edge_above(const Plane8& s, int x, int y, int k);                                 // intra_common.h:47-51
 int edge_left(const Plane8& s, int x, int y, int k);                                  // intra_common.h:52-56
 int directional(const Plane8& s, int x, int y, int w, int h, int angle, int i, int j); // intra_common.h:61-85
@@ -125,8 +126,8 @@ struct Field {  // RefMvLevel (kernels_ref.h:33-36)
   int cols = 0, rows = 0;
 };
 void search_rows(const Pyramid& cur, const Pyramid& ref, int level, const Field* coarser, int row0, int row1,
-                 Field& out);  // me_level_rows_ref (kernels_ref.cpp:240-251)
-void search_all(const Pyramid& cur, const Pyramid& ref, std::vector<Field>& levels);  // levels 4..0
+                 Field& out, int ref_index = 0);  // me_level_rows_ref (kernels_ref.cpp:240-251)
+void search_all(const Pyramid& cur, const Pyramid& ref, std::vector<Field>& levels, int ref_index = 0);  // levels 4..0
 
 // ---- the decision map --------------------------------------------------------------------------
 enum Part : uint8_t { kNone = 0, kHorz = 1, kVert = 2, kSplit = 3 };
@@ -149,6 +150,7 @@ struct Sb {  // SbDecision (decision_map.h:52-72), the fields the engines fill
   std::vector<uint8_t> tree;
   std::vector<Block> blocks;
   std::vector<Cands> cands;
+  int64_t cost = 0;  // the superblock's decision cost (the root square's DP cost)
 };
 struct Map {
   int sb_size = 0, sb_cols = 0, sb_rows = 0;
@@ -166,6 +168,13 @@ Sb decide_sb(const Luma& src, const RefSet& rs, int sb_col, int sb_row, int sb_s
 Map decide_frame(const Luma& src, const std::vector<const Luma*>& refs, int sb_size, int64_t lam);
 // The pruned form's intra restriction for decisions made on this thread (test of the engine's intra
 // mode groups; null: every mode): one 4-bit group set per superblock of `cols` columns.
+// The pruned form's reference limits and motion-search modes (oracle.cpp), with the superblock grid
+// they index; null: none.
+extern thread_local const uint8_t* g_ref_limits;
+extern thread_local const uint8_t* g_search_modes;
+extern thread_local int g_sb_cols, g_sb_rows;
+// Whether block (bx, by) of motion-search level `level` is searched narrow for reference r under them.
+bool narrow_search(int level, int bx, int by, int r);
 extern thread_local const uint8_t* g_intra_masks;
 extern thread_local int g_intra_mask_cols;  // kernels_ref.cpp:200-209, 282-301
 

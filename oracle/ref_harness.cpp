@@ -47,6 +47,11 @@ int ref_search_cell(const uint8_t* cur, const uint8_t* ref, const uint8_t* ref_p
   return 0;
 }
 
+// me_search alone (the reference's range parameter): +-range around (px, py); returns its cost.
+int ref_me_search(const uint8_t* cur, const uint8_t* ref, int w, int h, int bx, int by, int px, int py, int range, int* mv) {
+  return celv2::me_search(Plane{cur, w, w, h}, Plane{ref, w, w, h}, bx, by, px, py, range, &mv[0], &mv[1]);
+}
+
 long long ref_intra_satd(const uint8_t* s, int w, int h, int x, int y, int bw, int bh, int mode) {
   return celv2::block_mode_satd(Plane{s, w, w, h}, x, y, bw, bh, mode);
 }

@@ -4,8 +4,8 @@ rounding=True) BF16 rounding of every GEMM operand exactly where the GPU rounds 
 outputs and gradients differ from it only by FP32 accumulation order."""
 import numpy as np
 
-F, HID, REF_OUT, GROUP_OUT = 14, 32, 6, 3
-OUT = REF_OUT + GROUP_OUT
+F, HID, REF_OUT, GROUP_OUT, SEARCH_OUT = 14, 32, 6, 3, 1
+OUT = REF_OUT + GROUP_OUT + SEARCH_OUT
 W1 = 0
 B1 = W1 + 9 * F * HID
 W2 = B1 + HID
@@ -62,11 +62,13 @@ def forward(p, x, rounding=True):
 
 
 def labels(kstar):
-    """Targets per position: reference outputs j: (label & 7) > j + 1; group outputs g: bit 3 + g."""
+    """Targets per position: reference outputs j: (label & 7) > j + 1; group outputs g: bit 3 + g;
+    the search output: bit 6."""
     lab = np.asarray(kstar).reshape(-1, 1).astype(np.int64)
     refs = (lab & 7) > np.arange(REF_OUT).reshape(1, -1) + 1
     groups = (lab >> (3 + np.arange(GROUP_OUT).reshape(1, -1))) & 1
-    return np.concatenate([refs, groups.astype(bool)], axis=1).astype(np.float64)
+    search = (lab >> 6) & 1
+    return np.concatenate([refs, groups.astype(bool), search.astype(bool)], axis=1).astype(np.float64)
 
 
 def loss_and_grads(p, x, kstar, rounding=True):
@@ -111,6 +113,13 @@ def limits(logits, nrefs, threshold, band):
     unsure = (np.abs(s - threshold) < band).any(1)
     k[unsure] = 7
     return np.minimum(k, nrefs), unsure
+
+
+def search_modes(logits, sthreshold, unsure):
+    """The engine's motion-search modes from the search output: 0 (full) where sigma(output 9) >=
+    sthreshold or the superblock fell back, else 3 (narrow at levels 1 and 0)."""
+    s = 1.0 / (1.0 + np.exp(-np.asarray(logits, np.float64)))[:, REF_OUT + GROUP_OUT]
+    return np.where(unsure | (s >= sthreshold), 0, 3)
 
 
 def groups(logits, gthreshold, unsure):

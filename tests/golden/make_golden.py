@@ -1,4 +1,6 @@
-"""Writes tests/golden/primitives.npz: input/output vectors of CelV2's own decision primitives.
+"""Writes tests/golden/primitives.npz: input/output vectors of CelV2's own decision primitives, and
+tests/golden/narrow_search.npz: the reference's me_search at the small ranges of the pruned form's
+narrow search.
 
 Run in a tree where the reference is mounted and oracle/_ref is built (`make -C oracle ref`): the
 outputs come from the reference's header-only primitives (src/gpu/intra_common.h, inter_common.h,
@@ -133,7 +135,35 @@ def main():
         g[f"field_mvx{l}"], g[f"field_mvy{l}"] = mx, my
     np.savez_compressed(os.path.join(HERE, "primitives.npz"), **g)
     print("wrote", os.path.join(HERE, "primitives.npz"), {k: v.shape for k, v in g.items()})
+    narrow(R)
+
+
+def narrow(R):
+    """me_search (src/gpu/inter_common.h:106-119) at ranges 0..4 around centres reaching past every
+    edge of a 64x64 pair: rows (bx, by, px, py, range, mvx, mvy, cost)."""
+    rng = np.random.default_rng(20261204)
+    cur, ref = plane_noise(rng, 64, 64), plane_noise(rng, 64, 64)
+    ref[8:40, 16:48] = np.roll(cur, (3, -2), axis=(0, 1))[8:40, 16:48]  # a region whose motion a search finds
+    I = __import__("ctypes").c_int
+    R.ref_me_search.argtypes = [__import__("ctypes").c_void_p] * 2 + [I] * 7 + [__import__("ctypes").c_void_p]
+    rows = []
+    for i in range(240):
+        bx, by = int(rng.integers(0, 8)), int(rng.integers(0, 8))
+        px, py = int(rng.integers(-14, 15)), int(rng.integers(-14, 15))
+        rng_ = i % 5
+        mv = np.zeros(2, np.int32)
+        cost = R.ref_me_search(ptr(cur), ptr(ref), 64, 64, bx, by, px, py, rng_, ptr(mv))
+        rows.append([bx, by, px, py, rng_, int(mv[0]), int(mv[1]), cost])
+    out = os.path.join(HERE, "narrow_search.npz")
+    np.savez_compressed(out, cur=cur, ref=ref, rows=np.array(rows, np.int32))
+    print("wrote", out)
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "narrow":
+        R = ol.ref()
+        if R is None:
+            raise SystemExit("oracle/_ref/libcelv2ref.so is not built (make -C oracle ref)")
+        narrow(R)
+    else:
+        main()

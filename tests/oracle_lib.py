@@ -71,6 +71,7 @@ def oracle():
         lib.orc_last_error.restype = ctypes.c_char_p
         lib.orc_decide_frame.argtypes = [I, I, P, P, I, I64, I, P]
         lib.orc_decide_frame_masked.argtypes = [I, I, P, P, I, I64, I, P, P]
+        lib.orc_decide_frame_pruned.argtypes = [I, I, P, P, I, I64, I, P, P, P, P, P]
         lib.orc_engine_create.argtypes = [I, I, I, I, I]
         lib.orc_engine_create.restype = P
         lib.orc_engine_destroy.argtypes = [P]
@@ -124,6 +125,34 @@ def decide_frame(y8, refs, lam, width=None, height=None, sb_size=64, intra_masks
     if rc != 0:
         raise RuntimeError(oracle().orc_last_error().decode())
     return out
+
+
+def decide_frame_pruned(y8, refs, lam, width=None, height=None, limits=None, modes=None, masks=None, sb_size=64):
+    """The oracle's whole-frame decision under the pruned form's restrictions (each per superblock,
+    row-major, or None): reference limits, motion-search modes, intra mode groups. Returns (records,
+    per-superblock decision costs)."""
+    ph, pw = y8.shape
+    width = width or pw
+    height = height or ph
+    cols, rows = (width + sb_size - 1) // sb_size, (height + sb_size - 1) // sb_size
+    out = np.zeros(cols * rows, dtype=SB_DTYPE)
+    costs = np.zeros(cols * rows, np.int64)
+    refs = [np.ascontiguousarray(r) for r in refs]
+    keep = []
+
+    def arr(a):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.uint8)
+        assert a.size == cols * rows
+        keep.append(a)
+        return a.ctypes.data
+
+    rc = oracle().orc_decide_frame_pruned(width, height, y8.ctypes.data, ptr_array(refs), len(refs), lam, sb_size,
+                                          arr(limits), arr(modes), arr(masks), out.ctypes.data, costs.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(oracle().orc_last_error().decode())
+    return out, costs
 
 
 class CpuEngine:

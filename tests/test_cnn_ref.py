@@ -9,7 +9,7 @@ def test_reference_gradients_match_finite_differences():
     rng = np.random.default_rng(3)
     p = rng.normal(0, 0.3, cr.NPARAMS)
     x = rng.normal(0, 1, (2, 3, 4, cr.F))
-    kstar = rng.integers(1, 8, 2 * 3 * 4)
+    kstar = rng.integers(1, 8, 2 * 3 * 4) | rng.integers(0, 16, 2 * 3 * 4) << 3  # refs, groups, search
     loss, g = cr.loss_and_grads(p, x, kstar, rounding=False)
     assert np.isfinite(loss)
     idx = np.concatenate([rng.integers(lo, hi, 6) for lo, hi in

@@ -113,11 +113,14 @@ def test_two_4k_cel_streams_batched_plain_and_learned(gpu):
     limits = [e.fetch_ref_limits() for e in engines]
     costs = [e.fetch_sb_costs() for e in engines]
     masks = [e.fetch_intra_masks() for e in engines]
+    modes = [e.fetch_search_modes() for e in engines]
     for i, e in enumerate(engines):
-        # The same decisions, one stream alone, with those limits and intra groups set explicitly.
+        # The same decisions, one stream alone, with those limits, intra groups and search modes set
+        # explicitly.
         e.set_pruner(None)
         e.set_ref_limits(limits[i])
         e.set_intra_masks(masks[i])
+        e.set_search_modes(modes[i])
         e.launch(srcs[i], refs[i], lam, keys=ref_ids, src_key=7)
         assert compare_maps(e.collect(), learned[i], e.sb_cols) is None, i
         assert np.array_equal(e.fetch_sb_costs(), costs[i]), i

@@ -21,7 +21,7 @@ def rel(a, b):
 def batch():
     rng = np.random.default_rng(11)
     x = rng.uniform(0, 1, (3, 5, 7, cr.F)).astype(np.float32)
-    kstar = (rng.integers(1, 8, 3 * 5 * 7) | rng.integers(0, 8, 3 * 5 * 7) << 3).astype(np.uint8)  # refs + group bits
+    kstar = (rng.integers(1, 8, 3 * 5 * 7) | rng.integers(0, 16, 3 * 5 * 7) << 3).astype(np.uint8)  # refs + group + search bits
     return x, kstar
 
 
@@ -103,10 +103,12 @@ def test_engine_uses_the_network_limits(gpu):
 
 @pytest.mark.parametrize("prepared", [False, True])
 def test_engine_uses_the_network_groups(gpu, prepared):
-    """With the group outputs on (celv2hip_set_pruner_groups), a launch decides exactly as one with
-    the network's reference limits and intra mode groups given explicitly (celv2hip_set_ref_limits,
-    celv2hip_set_intra_masks), the groups being the reference rule of cnn_ref.groups on the same
-    logits; with the source prepared (its intra scores left to the launch) and without."""
+    """With the group and search outputs on (celv2hip_set_pruner_groups, celv2hip_set_pruner_search),
+    a launch decides exactly as one with the network's reference limits, intra mode groups and
+    motion-search modes given explicitly (celv2hip_set_ref_limits, celv2hip_set_intra_masks,
+    celv2hip_set_search_modes), the groups and modes being the reference rules of cnn_ref.groups and
+    cnn_ref.search_modes on the same logits; with the source prepared (its intra scores left to the
+    launch) and without."""
     w, h = 352, 288
     y8 = synth.clip_frames(synth.params(1, 33, w, h), 9)[1]
     lam = gpu.lambda_q8(140)
@@ -115,25 +117,31 @@ def test_engine_uses_the_network_groups(gpu, prepared):
     p = gpu.Pruner(max_positions=1024, seed=4)
     e = gpu.RawEngine(w, h)
     try:
-        for gth in (0.5, 0.45):
-            e.set_pruner(p, 0.5, 0.02, gthreshold=gth)
+        for gth, sth in ((0.5, 0.5), (0.45, 0.0)):
+            e.set_pruner(p, 0.5, 0.02, gthreshold=gth, sthreshold=sth)
             if prepared:
                 e.prepare(src, f)
             e.launch(src, rl, lam, keys=refs, src_key=f if prepared else -1)
             pruned, pruned_cost = e.collect(), e.fetch_sb_costs()
-            kmap = e.fetch_ref_limits()
+            kmap, modes = e.fetch_ref_limits(), e.fetch_search_modes()
             logits = p.forward(e.fetch_features()[None]).reshape(-1, cr.OUT)
             want_k, unsure = cr.limits(logits, 7, 0.5, 0.02)
             masks = cr.groups(logits, gth, unsure).astype(np.uint8)
+            want_modes = cr.search_modes(logits, sth, unsure) if sth > 0 else np.zeros(kmap.size)
             assert np.array_equal(kmap, want_k.astype(np.uint8))
+            assert np.array_equal(modes, want_modes.astype(np.uint8))
+            if sth > 0:
+                assert 0 < np.count_nonzero(modes) < modes.size  # both kinds of superblock present
             e.set_pruner(None)
             e.set_ref_limits(kmap)
             e.set_intra_masks(masks)
+            e.set_search_modes(modes)
             e.launch(src, rl, lam, keys=refs)
-            assert compare_maps(e.collect(), pruned, e.sb_cols) is None, (gth, prepared)
+            assert compare_maps(e.collect(), pruned, e.sb_cols) is None, (gth, sth, prepared)
             assert np.array_equal(e.fetch_sb_costs(), pruned_cost)
             e.set_ref_limits(None)
             e.set_intra_masks(None)
+            e.set_search_modes(None)
     finally:
         e.close()
         p.close()

@@ -56,3 +56,30 @@ def test_limit_rule_matches_the_engine_rule():
         k, fb = train.limits(z, threshold, band)
         want, unsure = cnn_ref.limits(z, 7, threshold, band)
         assert np.array_equal(k, want) and np.isclose(fb, unsure.mean())
+
+
+def test_search_labels_and_point():
+    """Label bit 6 marks exactly the superblocks whose narrow search costs more; the search point is
+    the highest threshold within the budget on every clip, and with logits that rank the harmless
+    superblocks lowest it makes every harmless superblock narrow and no harmful one."""
+    rng = np.random.default_rng(9)
+    clips = []
+    for _ in range(3):
+        j7 = rng.integers(1000, 5000, (4, 20)).astype(np.int64)
+        costs = np.repeat(j7[..., None], 7, axis=-1)
+        harm = rng.random((4, 20)) < 0.3
+        ncosts = j7 + np.where(harm, rng.integers(50, 200, (4, 20)), -rng.integers(0, 3, (4, 20)))
+        lab = train.search_labels(costs, ncosts)
+        assert np.array_equal(lab >> 6, harm.astype(np.uint8)) and not np.any(lab & 63)
+        logits = np.zeros((80, 10))
+        logits[:, 9] = np.where(harm.ravel(), 3.0, -3.0)  # sigma 0.95 where harmful, 0.05 elsewhere
+        clips.append((logits, costs, ncosts))
+    sp = train.search_point(*map(list, zip(*clips)), budget=0.0)
+    assert sp is not None and 0.05 < sp["sthreshold"] <= 0.95 and sp["regret"] <= 0.0
+    for logits, costs, ncosts in clips:
+        r, share = train.search_regret(logits, costs, ncosts, sp["sthreshold"])
+        harm = (ncosts > costs[..., 6]).ravel()
+        assert np.isclose(share, 1 - harm.mean())
+    # A budget that no threshold meets: none chosen, the searches stay full.
+    bad = [(np.full((80, 10), -9.0), c, c[..., 6] + 100) for _, c, _ in clips]
+    assert train.search_point(*map(list, zip(*bad)), budget=0.0) is None
