@@ -325,14 +325,16 @@ struct celv2hip_engine {
   // A plane build for the batched plane launches: import (rounding >8-bit luma), the 15 phase planes
   // of a reference, the 4 coarser levels.
   struct BuildJob {
-    ImportItem imp;
+    PyramidItem pyr;
     Pyramid* p;
     bool phases;
   };
   BuildJob build_job(const celv2_luma& f, Pyramid& p, Upload& up, bool phases, hipStream_t st) {
     ensure_pyramid(p, phases);
     const void* src = stage(f, up, st);
-    return BuildJob{ImportItem{src, pw, f.y16 ? f.bit_depth : 8, f.y16 ? 1 : 0, p.lv[0]}, &p, phases};
+    PyramidItem it{src, pw, f.y16 ? f.bit_depth : 8, f.y16 ? 1 : 0, {}};
+    for (int l = 0; l < kLevels; l++) it.lv[l] = p.lv[l];
+    return BuildJob{it, &p, phases};
   }
 
   IntraItem intra_item(SrcSlot& s) {
@@ -839,21 +841,21 @@ int celv2hip_set_counting(celv2hip_engine* e, int on) {
 
 namespace {
 
-// Plane builds of any engines in batched launches: every import, then every reference's phase
-// planes, then each coarser level of every pyramid. With `e` (the launching engine), the phase planes
-// go to its aux stream after the imports and the caller waits for e->sub_done before they are read;
-// returns whether any were queued there.
+// Plane builds of any engines in batched launches: every import with its pyramid (one launch), then
+// every reference's phase planes. With `e` (the launching engine), the phase planes go to its aux
+// stream after the pyramids and the caller waits for e->sub_done before they are read; returns whether
+// any were queued there.
 bool run_builds(const std::vector<celv2hip_engine::BuildJob>& jobs, hipStream_t st, celv2hip_engine* e = nullptr) {
   bool on_aux = false;
   for (size_t j0 = 0; j0 < jobs.size(); j0 += kMaxItems) {
     const size_t j1 = std::min(jobs.size(), j0 + kMaxItems);
-    Batch<ImportItem> bi;
+    Batch<PyramidItem> bp;
     Batch<SubpelItem> bs;
     for (size_t j = j0; j < j1; j++) {
-      add_import(bi, jobs[j].imp);
+      add_pyramid(bp, jobs[j].pyr);
       if (jobs[j].phases) add_subpel(bs, SubpelItem{jobs[j].p->lv[0], jobs[j].p->phase_bytes});
     }
-    launch_import(bi, st);
+    launch_pyramid(bp, st);
     if (e && bs.total() > 0) {
       check(hipEventRecord(e->sub_start, st), "hipEventRecord");
       check(hipStreamWaitEvent(e->aux, e->sub_start, 0), "hipStreamWaitEvent");
@@ -862,11 +864,6 @@ bool run_builds(const std::vector<celv2hip_engine::BuildJob>& jobs, hipStream_t 
       on_aux = true;
     } else {
       launch_subpel(bs, st);
-    }
-    for (int l = 1; l < kLevels; l++) {
-      Batch<DownItem> bd;
-      for (size_t j = j0; j < j1; j++) add_downsample(bd, DownItem{jobs[j].p->lv[l - 1], jobs[j].p->lv[l]});
-      launch_downsample(bd, st);
     }
   }
   check(hipGetLastError(), "plane kernels");
@@ -1423,7 +1420,7 @@ int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out) {
   });
 }
 
-int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint8_t* out, int* w, int* h) {
+static int fetch_plane(celv2hip_engine* e, int ref, int level, int phase, bool bordered, uint8_t* out, int* w, int* h) {
   if (int rc = validate(e)) return rc;
   return guarded([&] {
     if (!e->launched) throw Failure(CELV2_ERR_STATE, "no launch");
@@ -1434,10 +1431,18 @@ int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint
     else throw Failure(CELV2_ERR_ARG, "no such reference");
     if (phase && !p->phase_bytes) throw Failure(CELV2_ERR_ARG, "no phases");
     const DPlane& d = p->lv[level];
-    *w = d.w, *h = d.h;
+    const int b = bordered ? kBorder : 0;
+    *w = d.w + 2 * b, *h = d.h + 2 * b;
     e->sync_all();
-    check(hipMemcpy2D(out, d.w, d.p + static_cast<ptrdiff_t>(phase) * p->phase_bytes, d.stride, d.w, d.h, hipMemcpyDeviceToHost), "hipMemcpy2D");
+    const uint8_t* first = d.p + static_cast<ptrdiff_t>(phase) * p->phase_bytes - static_cast<ptrdiff_t>(b) * d.stride - b;
+    check(hipMemcpy2D(out, *w, first, d.stride, *w, *h, hipMemcpyDeviceToHost), "hipMemcpy2D");
   });
+}
+int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint8_t* out, int* w, int* h) {
+  return fetch_plane(e, ref, level, phase, false, out, w, h);
+}
+int celv2hip_fetch_plane_bordered(celv2hip_engine* e, int ref, int level, int phase, uint8_t* out, int* w, int* h) {
+  return fetch_plane(e, ref, level, phase, true, out, w, h);
 }
 
 int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, int32_t* mvy, int* cols, int* rows) {

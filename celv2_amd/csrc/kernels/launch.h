@@ -10,23 +10,18 @@
 namespace celv2hip {
 
 // planes.hip: every launch takes a batch of planes (add_* appends one with its workgroup count).
-struct ImportItem {  // a frame's luma (8-bit, or 9..16-bit in uint16) into a bordered level-0 plane
+struct PyramidItem {  // a frame's luma (8-bit, or 9..16-bit in uint16) into bordered levels 0..4
   const void* src;
   int sstride, bit_depth, wide;
-  DPlane out;
-};
-struct DownItem {  // level l+1 from level l
-  DPlane in, out;
+  DPlane lv[kLevels];  // level l + 1 is level l's size halved (rounded down)
 };
 struct SubpelItem {  // the 15 quarter-pel phase planes of a level-0 plane, phase_bytes apart
   DPlane in;
   long long phase_bytes;
 };
-void add_import(Batch<ImportItem>& b, const ImportItem& t);
-void add_downsample(Batch<DownItem>& b, const DownItem& t);
+void add_pyramid(Batch<PyramidItem>& b, const PyramidItem& t);
 void add_subpel(Batch<SubpelItem>& b, const SubpelItem& t);
-void launch_import(const Batch<ImportItem>& b, hipStream_t s);
-void launch_downsample(const Batch<DownItem>& b, hipStream_t s);
+void launch_pyramid(const Batch<PyramidItem>& b, hipStream_t s);
 void launch_subpel(const Batch<SubpelItem>& b, hipStream_t s);
 
 // me.hip: one pyramid level's motion fields for n references at once. cur: the source's level;

@@ -58,6 +58,7 @@ def lib():
         L.celv2hip_set_timing.argtypes = [P, I]
         L.celv2hip_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), I, ctypes.POINTER(LL), I]
         L.celv2hip_fetch_plane.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.celv2hip_fetch_plane_bordered.argtypes = [P, I, I, I, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_fetch_field.argtypes = [P, I, I, P, P, ctypes.POINTER(I), ctypes.POINTER(I)]
         L.celv2hip_set_ref_limits.argtypes = [P, P, I]
         L.celv2hip_set_intra_masks.argtypes = [P, P, I]
@@ -349,12 +350,14 @@ class RawEngine:
     def sync(self):
         self._check(lib().celv2hip_sync(self.h), "sync")
 
-    def fetch_plane(self, ref, level, phase=0):
-        """Interior of a plane of the last launch (ref -1: the source pyramid)."""
+    def fetch_plane(self, ref, level, phase=0, bordered=False):
+        """Interior of a plane of the last launch (ref -1: the source pyramid); bordered: with its
+        64-sample border."""
         pw, ph = (self.width + 7) & ~7, (self.height + 7) & ~7
-        out = np.zeros(pw * ph, np.uint8)
+        out = np.zeros((pw + 128) * (ph + 128), np.uint8)
         w, h = I(0), I(0)
-        self._check(lib().celv2hip_fetch_plane(self.h, ref, level, phase, out.ctypes.data, ctypes.byref(w), ctypes.byref(h)), "fetch_plane")
+        f = lib().celv2hip_fetch_plane_bordered if bordered else lib().celv2hip_fetch_plane
+        self._check(f(self.h, ref, level, phase, out.ctypes.data, ctypes.byref(w), ctypes.byref(h)), "fetch_plane")
         return out[: w.value * h.value].reshape(h.value, w.value).copy()
 
     def fetch_field(self, ref, level):

@@ -292,6 +292,9 @@ int celv2hip_fetch_row(celv2hip_engine* e, int row, celv2_sb* out);
 /* Interior samples (w x h of that level) of the source pyramid (ref < 0) or of reference `ref`'s
  * (its index in the last launch); phase 1..15: that quarter-pel plane of level 0. */
 int celv2hip_fetch_plane(celv2hip_engine* e, int ref, int level, int phase, uint8_t* out, int* w, int* h);
+/* The same plane with its border: (w + 128) x (h + 128) samples, the 64-sample border that holds each
+ * clamped position's value. */
+int celv2hip_fetch_plane_bordered(celv2hip_engine* e, int ref, int level, int phase, uint8_t* out, int* w, int* h);
 
 /* Motion field of reference `ref` at `level` (cols x rows, level 0 in quarter pels). */
 int celv2hip_fetch_field(celv2hip_engine* e, int ref, int level, int32_t* mvx, int32_t* mvy, int* cols, int* rows);

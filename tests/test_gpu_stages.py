@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 @pytest.fixture(scope="module")
 def launched(gpu):
     out = {}
-    for (w, h) in [(352, 288), (202, 122)]:
+    for (w, h) in [(352, 288), (202, 122), (64, 33), (200, 72)]:
         p = synth.params(0, 5, w, h)
         y8 = synth.clip_frames(p, 3)[1]
         e = gpu.RawEngine(w, h)
@@ -25,14 +25,26 @@ def launched(gpu):
     return out
 
 
-@pytest.mark.parametrize("size", [(352, 288), (202, 122)])
+@pytest.mark.parametrize("size", [(352, 288), (202, 122), (64, 33), (200, 72)])
 def test_pyramids_match(launched, size):
+    """Every level of the source's and a reference's pyramid, border included (each border sample the
+    value at its clamped position), at sizes whose coarse levels have odd widths and heights."""
     e, y8 = launched[size]
     want_src = ol.pyramid(y8[2])
     want_ref = ol.pyramid(y8[1])
     for l in range(5):
         assert np.array_equal(e.fetch_plane(-1, l), want_src[l]), ("src", l)
         assert np.array_equal(e.fetch_plane(0, l), want_ref[l]), ("ref", l)
+        assert np.array_equal(e.fetch_plane(-1, l, bordered=True), np.pad(want_src[l], 64, mode="edge")), ("src bordered", l)
+        assert np.array_equal(e.fetch_plane(0, l, bordered=True), np.pad(want_ref[l], 64, mode="edge")), ("ref bordered", l)
+
+
+@pytest.mark.parametrize("size", [(352, 288), (202, 122)])
+def test_phase_borders_match(launched, size):
+    e, y8 = launched[size]
+    want = ol.phases(y8[1])
+    for k in (1, 6, 15):
+        assert np.array_equal(e.fetch_plane(0, 0, k, bordered=True), np.pad(want[k], 64, mode="edge")), k
 
 
 @pytest.mark.parametrize("size", [(352, 288), (202, 122)])
