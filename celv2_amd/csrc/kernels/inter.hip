@@ -50,6 +50,8 @@ __global__ void __launch_bounds__(256, 6) k_inter(Batch<InterArgs> batch) {
   __shared__ uint32_t sc[4][kBlocks];
   __shared__ uint16_t cell_first[kRegion * kRegion + 1];
   __shared__ uint16_t cell_tasks[kTasks];
+  __shared__ uint32_t ukey[kRegion * kRegion];
+  __shared__ uint8_t uord[kRegion * kRegion];
   __shared__ int nuniq;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -103,6 +105,29 @@ __global__ void __launch_bounds__(256, 6) k_inter(Batch<InterArgs> batch) {
   if (wave == 0) {
     if (rep0 != c0) cuid[c0] = cuid[rep0];
     if (c1 < 100 && rep1 != c1) cuid[c1] = cuid[rep1];
+    // The order the distinct MVs are scored in: by quarter-pel phase, then the prediction's row and
+    // column, so the MVs scored together (the 4 waves take consecutive ones) read the same phase
+    // plane at nearby rows and share cache lines. A distinct MV has a distinct key (|MV| < 256
+    // samples at level 0: 10 bits each for the row and column offsets). The per-block minima are
+    // order-free, so the order changes no decision.
+    const int U = nuniq;
+    for (int u = lane; u < U; u += 64) {
+      const uint32_t m = mv[urep[u]];
+      const int mx = mv_x(m), my = mv_y(m);
+      ukey[u] = static_cast<uint32_t>((my & 3) * 4 + (mx & 3)) << 20 | static_cast<uint32_t>((my >> 2) + 512) << 10 |
+                static_cast<uint32_t>((mx >> 2) + 512);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t k0 = lane < U ? ukey[lane] : 0xffffffffu, k1 = lane + 64 < U ? ukey[lane + 64] : 0xffffffffu;
+    int r0 = 0, r1 = 0;
+    for (int v = 0; v < U; v++) {
+      const uint32_t kv = ukey[v];
+      r0 += kv < k0, r1 += kv < k1;
+    }
+    if (lane < U) uord[r0] = static_cast<uint8_t>(lane);
+    if (lane + 64 < U) uord[r1] = static_cast<uint8_t>(lane + 64);
   }
   __syncthreads();
 
@@ -128,9 +153,10 @@ __global__ void __launch_bounds__(256, 6) k_inter(Batch<InterArgs> batch) {
   const BPlane rb = as_buffer(ref);
   const int U = nuniq;
   uint32_t* const mysc = sc[wave];
-  // Each wave takes the distinct MVs u = wave, wave + 4, ... on its own (its own LDS scores; the
-  // block minima are order-free atomics).
-  for (int u = wave; u < U; u += 4) {
+  // Each wave takes the distinct MVs si = wave, wave + 4, ... of the scoring order on its own (its own
+  // LDS scores; the block minima are order-free atomics).
+  for (int si = wave; si < U; si += 4) {
+    const int u = uord[si];
     const uint32_t m = mv[urep[u]];
     const int mx = mv_x(m), my = mv_y(m);
     const int X = clamp_origin(ox + (mx >> 2), ref.w, 64), Y = clamp_origin(oy + (my >> 2), ref.h, 64);
