@@ -1124,7 +1124,7 @@ void launch_batch(const LaunchIn* in, int n) {
         fa.nrefs = in[i].nrefs, fa.cols2 = e->fcols[2], fa.rows2 = e->frows[2], fa.sb_cols = e->sb_cols, fa.sb_rows = e->sb_rows;
         fa.lambda_q8 = in[i].lam;
         fa.feats = e->infer.x;
-        launch_features(fa, S);
+        launch_features(fa, S, /*context=*/!e->pruner);  // with a pruner the network's launch computes it
         e->features_valid = true;
         if (e->pruner) {
           const bool groups = intra_after[i] != 0;
@@ -1132,7 +1132,8 @@ void launch_batch(const LaunchIn* in, int n) {
           const bool modes = e->prune_sthreshold > 0.f;
           cnn_infer(e->infer.x, 1, e->sb_rows, e->sb_cols, e->pruner->params, e->pruner->pb, nullptr, in[i].nrefs,
                     e->prune_threshold, e->prune_band, e->kmap, e->counters + 3 * kCounterWords, S,
-                    groups ? e->nmask : nullptr, e->prune_gthreshold, modes ? e->smap : nullptr, e->prune_sthreshold);
+                    groups ? e->nmask : nullptr, e->prune_gthreshold, modes ? e->smap : nullptr, e->prune_sthreshold,
+                    /*context=*/true);
           use_kmap[i] = true;
           if (modes) use_smap[i] = true;
           if (groups) {

@@ -59,10 +59,12 @@ void cnn_limits(const float* logits, int m, int nrefs, float threshold, float ba
 // imask is non-null, each superblock's intra mode groups: group g (1..3) when sigma(logit 5 + g) >=
 // gthreshold, every group when a reference output was within band of its threshold (a fallback);
 // if smap is non-null, each superblock's motion-search modes: the full search (0) when sigma(logit 9)
-// >= sthreshold or on a fallback, else narrow at levels 1 and 0 (3).
+// >= sthreshold or on a fallback, else narrow at levels 1 and 0 (3). context: x lacks the frame
+// context f[11..13] (launch_features without it); the launch computes it and writes it into x.
 void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
                float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s,
-               uint8_t* imask = nullptr, float gthreshold = 0.f, uint8_t* smap = nullptr, float sthreshold = 0.f);
+               uint8_t* imask = nullptr, float gthreshold = 0.f, uint8_t* smap = nullptr, float sthreshold = 0.f,
+               bool context = false);
 
 // The static scratch of the no-allocation forward pass: features [kStaticPositions][kFeatures]
 // followed by logits [kStaticPositions][kOut] (1.3 MB), one copy per device. cnn_scratch_init takes

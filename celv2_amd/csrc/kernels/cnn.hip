@@ -107,6 +107,9 @@ struct Infer {
   float gthreshold;
   uint8_t* smap;   // [b][h][w] motion-search modes (may be null)
   float sthreshold;
+  // The features' frame context (f[11..13], the frame's means) is not in x yet: each workgroup
+  // computes it from every position of its frame and writes it for its own tile's positions.
+  bool context;
 };
 
 // A superblock's intra mode groups from the group outputs (group 0 always; every group on a fallback).
@@ -134,10 +137,41 @@ __global__ void __launch_bounds__(kInferThreads) k_infer(Infer a) {
   for (int e = tid; e < kHidden * kOut; e += kInferThreads) w3s[e] = a.pb[kW3 + e];
   const int tx0 = static_cast<int>(blockIdx.x) * kTileW, ty0 = static_cast<int>(blockIdx.y) * kTileH;
   const size_t frame = static_cast<size_t>(blockIdx.z) * a.h * a.w;
+  __shared__ float part[3][256];
+  float ctx[3] = {0.f, 0.f, 0.f};
+  if (a.context) {
+    // The frame's means of f[0], f[8] and f[1] + ... + f[6]: a strided pass by 256 threads, then a
+    // tree, in a fixed order (the same sums k_frame_context forms).
+    const int n = a.h * a.w;
+    if (tid < 256) {
+      float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+      for (int sb = tid; sb < n; sb += 256) {
+        const float* f = a.x + (frame + static_cast<size_t>(sb)) * kFeatures;
+        s0 += f[0], s1 += f[8], s2 += f[1] + f[2] + f[3] + f[4] + f[5] + f[6];
+      }
+      part[0][tid] = s0, part[1][tid] = s1, part[2][tid] = s2;
+    }
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+      if (tid < w)
+        for (int c = 0; c < 3; c++) part[c][tid] += part[c][tid + w];
+      __syncthreads();
+    }
+    const float inv = 1.f / static_cast<float>(n);
+    ctx[0] = part[0][0] * inv, ctx[1] = part[1][0] * inv, ctx[2] = part[2][0] * inv;
+    if (tid < kTileH * kTileW) {
+      const int yy = ty0 + tid / kTileW, xx = tx0 + tid % kTileW;
+      if (yy < a.h && xx < a.w) {
+        float* f = const_cast<float*>(a.x) + (frame + static_cast<size_t>(yy) * a.w + xx) * kFeatures;
+        f[11] = ctx[0], f[12] = ctx[1], f[13] = ctx[2];
+      }
+    }
+  }
   for (int e = tid; e < kXH * kXW * kFeatures; e += kInferThreads) {
     const int c = e % kFeatures, pos = e / kFeatures, yy = ty0 - 2 + pos / kXW, xx = tx0 - 2 + pos % kXW;
     const bool in = yy >= 0 && yy < a.h && xx >= 0 && xx < a.w;
-    xs[e] = bf16_bits(in ? a.x[(frame + static_cast<size_t>(yy) * a.w + xx) * kFeatures + c] : 0.f);
+    const float v = !in ? 0.f : a.context && c >= 11 ? ctx[c - 11] : a.x[(frame + static_cast<size_t>(yy) * a.w + xx) * kFeatures + c];
+    xs[e] = bf16_bits(v);
   }
   __syncthreads();
   // Layer 1 on the 6 x 18 region (7 row tiles x 2 column tiles); 0 outside the grid (layer 2's padding).
@@ -427,9 +461,9 @@ void cnn_adamw(float* params, const float* grads, float* m1, float* m2, uint16_t
 
 void cnn_infer(const float* x, int b, int h, int wd, const float* params, const uint16_t* pb, float* logits, int nrefs,
                float threshold, float band, uint8_t* kmap, unsigned long long* fallbacks, hipStream_t s, uint8_t* imask,
-               float gthreshold, uint8_t* smap, float sthreshold) {
+               float gthreshold, uint8_t* smap, float sthreshold, bool context) {
   if (b <= 0 || h <= 0 || wd <= 0) return;
-  const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band, imask, gthreshold, smap, sthreshold};
+  const Infer a{x, h, wd, pb, params, logits, kmap, fallbacks, nrefs, threshold, band, imask, gthreshold, smap, sthreshold, context};
   k_infer<<<dim3((wd + kTileW - 1) / kTileW, (h + kTileH - 1) / kTileH, b), kInferThreads, 0, s>>>(a);
 }
 

@@ -111,7 +111,8 @@ struct FeatureArgs {
   long long lambda_q8;  // the frame's rate multiplier
   float* feats;
 };
-void launch_features(const FeatureArgs& a, hipStream_t s);
+// context: also the frame context f[11..13] (else the network's launch computes it, cnn_infer).
+void launch_features(const FeatureArgs& a, hipStream_t s, bool context = true);
 
 // Constant tables the kernels read: each builds its tables on the host and copies them to the
 // current device's symbols before returning (once per device, from the engine's creation).

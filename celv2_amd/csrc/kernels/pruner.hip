@@ -105,11 +105,11 @@ __global__ void __launch_bounds__(256) k_frame_context(float* feats, int n) {
 
 }  // namespace
 
-void launch_features(const FeatureArgs& a, hipStream_t s) {
+void launch_features(const FeatureArgs& a, hipStream_t s, bool context) {
   const int n = a.sb_cols * a.sb_rows;
   if (n > 0) {
     k_features<<<(n + 3) / 4, 256, 0, s>>>(a);
-    k_frame_context<<<1, 256, 0, s>>>(a.feats, n);
+    if (context) k_frame_context<<<1, 256, 0, s>>>(a.feats, n);
   }
 }
 
