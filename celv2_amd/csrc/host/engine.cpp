@@ -193,13 +193,14 @@ struct celv2hip_engine {
   std::vector<hipEvent_t> group_done;  // one per download group (at most one per row)
   std::vector<int> row_group;          // the group that downloads row r in the last launch
   // Per-launch stage events in a ring (kRing launches): start, references built, motion search,
-  // inter scores, finalize, end; and per prepare(): intra scores start / end.
+  // inter scores, finalize, end, and (6) the motion search's level 0 starting; and per prepare():
+  // intra scores start / end.
   static constexpr int kRing = 512;
   // Stage events are recorded only with timing on (celv2hip_set_timing): each record between two
   // kernels costs the stream a few microseconds. Launches since the last reset without them.
   bool timing = false;
   long long window_untimed = 0;
-  std::vector<std::array<hipEvent_t, 6>> ring;
+  std::vector<std::array<hipEvent_t, 7>> ring;
   std::vector<std::array<hipEvent_t, 2>> pring;
   hipEvent_t* ev = nullptr;  // the current launch's 6 events
   long long prepares = 0, window_launch = 0, window_prepare = 0;
@@ -1089,7 +1090,7 @@ void launch_batch(const LaunchIn* in, int n) {
       a.nrefs = nr, a.cols = e->fcols[l], a.rows = e->frows[l];
       a.pcols = l + 1 < kLevels ? e->fcols[l + 1] : 0, a.prows = l + 1 < kLevels ? e->frows[l + 1] : 0;
       a.level0 = l == 0, a.phase_bytes = used[i][0]->p.phase_bytes;
-      a.searches = e->counters + 4 * kCounterWords;
+      a.searches = e->counters + (l == 0 ? 5 : 4) * kCounterWords;  // level 0's centres counted apart
       a.kmap = use_kmap[i] ? e->kmap : nullptr, a.sb_cols = e->sb_cols, a.sb_rows = e->sb_rows, a.level = l;
       a.smap = use_smap[i] ? e->smap : nullptr;
       const bool feats = l == 2 && (e->pruner || e->want_features);
@@ -1106,6 +1107,7 @@ void launch_batch(const LaunchIn* in, int n) {
     }
     if (l == 0) {
       if (phases_pending) check(hipStreamWaitEvent(S, e0->sub_done, 0), "hipStreamWaitEvent");  // level 0 refines on them
+      for (int i = 0; i < n; i++) if (in[i].e->timing) check(hipEventRecord(in[i].e->ev[6], S), "hipEventRecord");
       if (wavefront) {
         level0 = b;
         continue;
@@ -1498,14 +1500,14 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
     n = e->launches - e->window_launch;
     const long long np = e->prepares - e->window_prepare;
     if (n > celv2hip_engine::kRing || np > celv2hip_engine::kRing) throw Failure(CELV2_ERR_STATE, "stats window longer than the event ring");
-    double acc[7] = {};
+    double acc[8] = {};
     for (long long i = e->window_launch; i < e->launches && !e->window_untimed; i++) {
       auto& a = e->ring[i % celv2hip_engine::kRing];
-      const int pairs[6][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {0, 5}};
-      for (int k = 0; k < 6; k++) {
+      const int pairs[7][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {4, 5}, {0, 5}, {6, 2}};
+      for (int k = 0; k < 7; k++) {
         float t = 0;
         check(hipEventElapsedTime(&t, a[pairs[k][0]], a[pairs[k][1]]), "hipEventElapsedTime");
-        acc[k] += t;
+        acc[k < 6 ? k : 7] += t;
       }
     }
     for (long long i = e->window_prepare; i < e->prepares && !e->window_untimed; i++) {
@@ -1514,15 +1516,15 @@ int celv2hip_stats(celv2hip_engine* e, double* ms, int cap_ms, long long* counts
       check(hipEventElapsedTime(&t, a[0], a[1]), "hipEventElapsedTime");
       acc[6] += t;
     }
-    for (int k = 0; k < 7 && k < cap_ms; k++) ms[k] = acc[k];
+    for (int k = 0; k < 8 && k < cap_ms; k++) ms[k] = acc[k];
     check(hipMemcpy(e->counters_host, e->counters, kCounterBytes, hipMemcpyDeviceToHost), "hipMemcpy counters");
     auto total = [&](int k) {
       long long t = 0;
       for (int i = 0; i < kCounterSlots; i++) t += static_cast<long long>(e->counters_host[k * kCounterWords + kCounterStride * i]);
       return t;
     };
-    const long long v[6] = {n, total(0), total(4), total(1), total(2), total(3)};
-    for (int k = 0; k < 6 && k < cap_counts; k++) counts[k] = v[k];
+    const long long v[7] = {n, total(0), total(4) + total(5), total(1), total(2), total(3), total(5)};
+    for (int k = 0; k < 7 && k < cap_counts; k++) counts[k] = v[k];
   });
   return rc ? -rc : static_cast<int>(n);
 }

@@ -125,7 +125,7 @@ __device__ __forceinline__ int mv_y(uint32_t m) { return static_cast<int16_t>(m 
 // together do not queue on one address; the host sums the slots.
 constexpr int kCounterSlots = 64, kCounterStride = 16;
 constexpr int kCounterWords = kCounterSlots * kCounterStride;  // one counter
-constexpr int kCounterKinds = 5;                                // the engine's counters
+constexpr int kCounterKinds = 6;                                // the engine's counters
 constexpr size_t kCounterBytes = static_cast<size_t>(kCounterKinds) * kCounterWords * sizeof(unsigned long long);
 __device__ __forceinline__ void count(unsigned long long* base, int slot, unsigned long long v) {
   atomicAdd(base + kCounterStride * (slot & (kCounterSlots - 1)), v);

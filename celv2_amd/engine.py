@@ -463,13 +463,14 @@ class RawEngine:
 
     def stats(self):
         """(stage ms sums over the window, counts) - see celv2hip_stats."""
-        ms = (ctypes.c_double * 7)()
-        cnt = (LL * 6)()
-        n = lib().celv2hip_stats(self.h, ms, 7, cnt, 6)
+        ms = (ctypes.c_double * 8)()
+        cnt = (LL * 7)()
+        n = lib().celv2hip_stats(self.h, ms, 8, cnt, 7)
         if n < 0:
             raise EngineError(last_error())
-        keys = ["build", "me", "inter", "finalize", "copy", "total", "intra_prepare"]
-        ckeys = ["launches", "tile_evals", "me_centres", "compound_blocks", "inter_candidates", "pruner_fallbacks"]
+        keys = ["build", "me", "inter", "finalize", "copy", "total", "intra_prepare", "me_level0"]
+        ckeys = ["launches", "tile_evals", "me_centres", "compound_blocks", "inter_candidates", "pruner_fallbacks",
+                 "me_centres_level0"]
         return dict(zip(keys, list(ms))), dict(zip(ckeys, list(cnt)))
 
 

@@ -152,11 +152,12 @@ int celv2hip_sync(celv2hip_engine* e);
  * with HIP events, without host waits between launches. ms[0..6] sums, in milliseconds:
  * [0] import + pyramids + phase planes queued by launch (a source not prepare()d includes its intra
  * scores), [1] motion search (5 levels), [2] inter scores, [3] compound + finalize, [4] record copy,
- * [5] launch start to end, [6] intra scores queued by prepare(). counts: [0] launches, [1] distinct
+ * [5] launch start to end, [6] intra scores queued by prepare(), [7] the motion search's level-0 launch
+ * (in the wavefront form with the inter scores interleaved). counts: [0] launches, [1] distinct
  * (MV, reference, superblock) tile evaluations, [2] motion-search centres searched, [3] compound
  * blocks scored, [4] non-repeated (block, MV, reference) inter candidates scored (while counting is on,
- * celv2hip_set_counting), [5] superblocks the pruner sent to its fallback. Returns the number of
- * launches (negative status on failure). */
+ * celv2hip_set_counting), [5] superblocks the pruner sent to its fallback, [6] the centres of [2]
+ * searched at level 0. Returns the number of launches (negative status on failure). */
 int celv2hip_stats_reset(celv2hip_engine* e);
 /* on != 0: later launches and prepares record the stage events the ms figures come from (off by
  * default: each record between two kernels costs the stream a few microseconds). ms are all zero
