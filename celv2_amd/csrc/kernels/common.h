@@ -167,6 +167,13 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v) {
 }
 
 // Wave-wide minimum of a 32-bit key (all 64 lanes take part; the result is wave-uniform).
+// min(a, b) of wave-uniform values on the scalar unit (left to itself the compiler takes the vector
+// unit's three-operand minimum, which costs the busier unit two copies and the minimum).
+__device__ __forceinline__ uint32_t scalar_min(uint32_t a, uint32_t b) {
+  uint32_t r;
+  asm("s_min_u32 %0, %1, %2" : "=s"(r) : "s"(a), "s"(b));
+  return r;
+}
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   v = min(v, dpp<kDppXor1>(v));
   v = min(v, dpp<kDppXor2>(v));
@@ -176,7 +183,7 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
   const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
   const uint32_t r2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
   const uint32_t r3 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
-  return min(min(r0, r1), min(r2, r3));
+  return scalar_min(scalar_min(r0, r1), scalar_min(r2, r3));
 }
 
 // The same when v is already equal over each aligned quad of lanes (the xor-1 and xor-2 steps would
@@ -188,7 +195,7 @@ __device__ __forceinline__ uint32_t wave_min_quads_u32(uint32_t v) {
   const uint32_t r1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
   const uint32_t r2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
   const uint32_t r3 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
-  return min(min(r0, r1), min(r2, r3));
+  return scalar_min(scalar_min(r0, r1), scalar_min(r2, r3));
 }
 
 // Sums over aligned lane groups of 4 and of 8 (every lane of the group gets the sum).

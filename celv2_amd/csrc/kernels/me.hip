@@ -2,9 +2,9 @@
 //
 // One wave (64 lanes) searches one 8x8 block against one reference (me_cell + me_refine,
 // src/gpu/inter_common.h:135-193):
-//   1. the six candidate centres' SADs, 8 lanes per centre (one block row each), summed by DPP;
-//      the centre ranking is then the reference's sequential rule, wave-uniform (scalar code);
-//   2. each searched centre's 24 x 24 window is read from the plane once (144 coalesced word pairs)
+//   1. the six candidate centres' SADs, 8 lanes per centre (one block row each), summed by DPP and
+//      read into scalar registers; the centre ranking is then scalar code;
+//   2. each searched centre's 24 x 24 window is read from the plane once (144 word pairs, 6 per row)
 //      into the wave's LDS area as 4 byte-shifted copies; both windows' 2 x 289 positions are then
 //      scored in one pass by all 64 lanes: lane (window, half of the dy range, dx < 16) takes 16 rows
 //      as aligned words of the copy its offset selects and scores 9 positions with v_sad_u8 against
@@ -45,22 +45,24 @@ constexpr int kWinB = 624;                     // the second window's start: 624
 constexpr int kWaveLds = kWinB + kWinWords;    // 1224
 constexpr int kLdsSpare = 32;
 
-// Stages a window: item k = lane + 64 i (k < 24 x 6) is word d of row r of every copy, from the
-// aligned row's words d and d + 1 (rows from the window's first byte rounded down).
-__device__ __forceinline__ void stage_window(const BPlane& rb, int X, int Y, const int (&koff)[3], const int (&kdst)[3],
-                                             uint32_t* win) {
-  const int base = corner_offset(rb, Y, X) & ~3;
+// Stages a window: lane (row group q = lane >> 3, word d = lane & 7) copies word d of rows q, q + 8
+// and q + 16 into every copy, from the aligned row's words d and d + 1 (rows from the window's first
+// byte rounded down); lanes with d > 5 idle. off = the lane's row and word offset in the plane, dst =
+// its LDS word in the window (both for row q), base = the window's aligned first byte in the plane
+// (wave-uniform: the buffer's scalar offset).
+__device__ __forceinline__ void stage_window(const BPlane& rb, int base, int off, bool active, uint32_t* dst) {
+  if (active) {
 #pragma unroll
-  for (int i = 0; i < 3; i++)
-    if (kdst[i] >= 0) {
+    for (int i = 0; i < 3; i++) {
       uint32_t lo, hi;
-      bload8(rb, base + koff[i], 0, lo, hi);
-      uint32_t* o = win + kdst[i];
+      bload8(rb, off, base + 8 * i * rb.stride, lo, hi);
+      uint32_t* o = dst + 8 * i * kRowWords;
       o[0] = lo;
       o[1] = __builtin_amdgcn_alignbyte(hi, lo, 1);
       o[2] = __builtin_amdgcn_alignbyte(hi, lo, 2);
       o[3] = __builtin_amdgcn_alignbyte(hi, lo, 3);
     }
+  }
 }
 
 // Lane constants of the search, computed per block (a table of them cost more in loads than the
@@ -174,6 +176,13 @@ struct Opening {
   uint32_t km = 0, sm = 0, mine = 0;
 };
 
+// v in lanes 8k .. 8k+7 (row k >> 1 of the wave, banks 0-1 for even k, 2-3 for odd), old elsewhere:
+// one DPP move with an identity lane pattern and a row and bank mask.
+template <int k>
+__device__ __forceinline__ uint32_t put_group(uint32_t old, uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(old), static_cast<int>(v), 0xE4, 1 << (k >> 1), (k & 1) ? 0xC : 0x3, false));
+}
+
 struct BlockPos {
   int r, bx, by;
 };
@@ -236,48 +245,59 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   bool two = false;
   if (a.coarser[r]) {
     // Centres k = 0..4: the coarser MV under the block and its left, right, above and below
-    // neighbours (clamped into the coarser field), doubled; k = 5: zero. Uniform addresses, and the
-    // coarser field is the previous launch's output: scalar loads.
-    int cx[6], cy[6];
+    // neighbours (clamped into the coarser field), doubled; k = 5: zero (pack_mv(0, 0) = 0). Uniform
+    // addresses, and the coarser field is the previous launch's output: scalar loads.
+    uint32_t m[6];
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       const int ox = k == 1 ? -1 : k == 2 ? 1 : 0, oy = k == 3 ? -1 : k == 4 ? 1 : 0;
       const int qx = clamp_uniform(bx / 2 + ox, 0, a.pcols - 1), qy = clamp_uniform(by / 2 + oy, 0, a.prows - 1);
-      const uint32_t m = a.coarser[r][qy * a.pcols + qx];
-      cx[k] = 2 * mv_x(m), cy[k] = 2 * mv_y(m);
+      m[k] = a.coarser[r][qy * a.pcols + qx];
     }
-    cx[5] = 0, cy[5] = 0;
-    // Each centre's SAD (lanes 8k .. 8k+7, one block row each; lanes 48.. repeat centre 5).
-    const int kk = min(lane >> 3, 5), row = lane & 7;
-    int mx = cx[0], my = cy[0];
-#pragma unroll
-    for (int k = 1; k < 6; k++)
-      if (kk == k) mx = cx[k], my = cy[k];
-    const int off = corner_offset_lane(rb, clamp_origin(8 * by + my, ref.h, 8) + row, clamp_origin(8 * bx + mx, ref.w, 8));
+    m[5] = 0u;
+    // Each centre's SAD: lanes 8k .. 8k+7, one block row each (lanes 48.. repeat centre 0, unread).
+    // Group k's lanes are row k >> 1 of the wave, banks 0-1 (k even) or 2-3 (k odd): each takes its
+    // centre's MV by one masked DPP move.
+    uint32_t mv = m[0];
+    mv = put_group<1>(mv, m[1]), mv = put_group<2>(mv, m[2]), mv = put_group<3>(mv, m[3]);
+    mv = put_group<4>(mv, m[4]), mv = put_group<5>(mv, m[5]);
+    const int row = lane & 7;
+    const int X = clamp_origin(8 * bx + 2 * mv_x(mv), ref.w, 8), Y = clamp_origin(8 * by + 2 * mv_y(mv), ref.h, 8);
+    const int off = corner_offset_lane(rb, Y + row, X);
     uint32_t d0, d1, d2;
     bload12(rb, off & ~3, 0, d0, d1, d2);
     const uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, off & 3), w1 = __builtin_amdgcn_alignbyte(d2, d1, off & 3);
-    const uint32_t c0 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row, 64));
-    const uint32_t c1 = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * row + 1, 64));
+    const int baddr = row << 3;  // lane 2 row's byte address
+    const uint32_t c0 = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(baddr, static_cast<int>(o.mine)));
+    const uint32_t c1 = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(baddr + 4, static_cast<int>(o.mine)));
     const uint32_t s = oct_sum(sad4(c1, w1, sad4(c0, w0, 0)));
     // me_cell's ranking (src/gpu/inter_common.h:164-186) walks the centres in order, skipping one equal
     // to the current best or runner-up, and keeps the best and the runner-up, each the first strict
-    // minimum. A centre equal to an earlier one that is neither has that one's SAD, which lost to
-    // both, so it changes nothing either: the two are the two least (SAD, k) over the centres that
-    // differ from every earlier one. Here in the lanes: a group's key SAD << 3 | k, or none when an
-    // earlier centre is the same, and two wave minima.
-    bool dup = false;
+    // minimum. With keys SAD << 3 | k that is: the best is the least key (a centre equal to an
+    // earlier one has its SAD and a larger k), the runner-up the least key among the centres that
+    // differ from the best (one equal to an earlier centre other than the best has a larger key than
+    // that one, which is eligible too). Scalar code on the six group sums.
+    uint32_t key[6];
 #pragma unroll
-    for (int j = 0; j < 5; j++) dup = dup || (j < kk && cx[j] == mx && cy[j] == my);
-    const uint32_t key = dup ? 0xffffffffu : (s << 3 | static_cast<uint32_t>(kk));
-    const uint32_t k1 = wave_min_u32(key);
-    const uint32_t k2 = wave_min_u32(key == k1 ? 0xffffffffu : key);
-    const int l1 = 8 * static_cast<int>(k1 & 7);
-    p1x = __builtin_amdgcn_readlane(mx, l1), p1y = __builtin_amdgcn_readlane(my, l1);
+    for (int k = 0; k < 6; k++) key[k] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(s), 8 * k)) << 3 | static_cast<uint32_t>(k);
+    uint32_t k1 = key[0];
+#pragma unroll
+    for (int k = 1; k < 6; k++) k1 = scalar_min(k1, key[k]);
+    const int b1 = static_cast<int>(k1 & 7);
+    uint32_t m1 = m[0];
+#pragma unroll
+    for (int k = 1; k < 6; k++) m1 = b1 == k ? m[k] : m1;
+    uint32_t k2 = 0xffffffffu;
+#pragma unroll
+    for (int k = 0; k < 6; k++) k2 = m[k] != m1 ? min(k2, key[k]) : k2;
+    p1x = 2 * mv_x(m1), p1y = 2 * mv_y(m1);
     two = k2 != 0xffffffffu;
     if (two) {
-      const int l2 = 8 * static_cast<int>(k2 & 7);
-      p2x = __builtin_amdgcn_readlane(mx, l2), p2y = __builtin_amdgcn_readlane(my, l2);
+      const int b2 = static_cast<int>(k2 & 7);
+      uint32_t m2 = m[0];
+#pragma unroll
+      for (int k = 1; k < 6; k++) m2 = b2 == k ? m[k] : m2;
+      p2x = 2 * mv_x(m2), p2y = 2 * mv_y(m2);
     }
   }
 
@@ -289,19 +309,13 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     key = wave_min_u32(search_narrow(rb, X1, Y1, X2, Y2, two, c, lane));
   } else {
   // Full searches (me_search, src/gpu/inter_common.h:106-119) from windows staged in LDS.
-  // Stage item k = lane + 64 i is word d of row r of a window (r = k / 6 = (171 k) >> 10 for k < 192,
-  // d = k - 6 r); item 2 exists for lanes < 16 only.
-  int koff[3], kdst[3];
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    const int k = lane + 64 * i, r = (171 * k) >> 10, d = k - kCopyWords * r;
-    koff[i] = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(r))) + 4 * d;
-    kdst[i] = i < 2 || lane < kWin * kCopyWords - 128 ? r * kRowWords + 4 * d : -1;
-  }
+  const int q = lane >> 3, d = lane & 7;
+  const int soff = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(q))) + 4 * d;
+  const int sdst = q * kRowWords + 4 * d;
   const int X1 = clamp_origin_uniform(8 * bx + p1x - kRange, ref.w, kWin), Y1 = clamp_origin_uniform(8 * by + p1y - kRange, ref.h, kWin);
   const int X2 = clamp_origin_uniform(8 * bx + p2x - kRange, ref.w, kWin), Y2 = clamp_origin_uniform(8 * by + p2y - kRange, ref.h, kWin);
-  stage_window(rb, X1, Y1, koff, kdst, win1);
-  if (two) stage_window(rb, X2, Y2, koff, kdst, win2);
+  stage_window(rb, corner_offset(rb, Y1, X1) & ~3, soff, d < kCopyWords, win1 + sdst);
+  if (two) stage_window(rb, corner_offset(rb, Y2, X2) & ~3, soff, d < kCopyWords, win2 + sdst);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
