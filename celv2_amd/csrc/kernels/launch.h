@@ -27,6 +27,16 @@ void launch_subpel(const Batch<SubpelItem>& b, hipStream_t s);
 // me.hip: one pyramid level's motion fields for n references at once. cur: the source's level;
 // ref[r]: reference r's level (at level 0 with its phases, phase_bytes apart); coarser[r]: the level
 // above's field of reference r (nullptr at the top), pcols x prows; out[r]: cols x rows packed MVs.
+// Exact unsigned division n / d for n < 2^24 and 1 <= d < 2^16 on the scalar unit:
+// q = mulhi(n << 8, m) >> l with l = ceil(log2 d), m = ceil(2^(24 + l) / d). Since d <= 2^l,
+// m d - 2^(24 + l) < d <= 2^l, the condition under which floor(n m / 2^(24 + l)) = floor(n / d) for
+// every 24-bit n (Granlund and Montgomery, "Division by invariant integers using multiplication",
+// 1994, theorem 4.2); m < 2^25 + 1 fits 32 bits.
+struct Divisor {
+  uint32_t m;
+  int l;
+};
+Divisor make_divisor(int d);
 struct MeLevelArgs {
   DPlane cur;
   DPlane ref[kMaxRefs];
@@ -48,7 +58,7 @@ struct MeLevelArgs {
   // The block rows this launch searches: [row0, row0 + nrun) (nrun 0: every row). rows stays the
   // level's row count (neighbour clamps, the coarser level's rows).
   int row0, nrun;
-  float inv_nrefs, inv_cols;     // 1 / nrefs, 1 / cols (set by add_me_level)
+  Divisor by_nrefs, by_cols;     // exact division by nrefs and by cols (set by add_me_level)
 };
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a);
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s);

@@ -186,12 +186,9 @@ __device__ __forceinline__ uint32_t put_group(uint32_t old, uint32_t v) {
 struct BlockPos {
   int r, bx, by;
 };
-// n / d for 0 <= n < 2^24 from a float estimate with 1 / d, corrected once (the estimate is within 1).
-__device__ __forceinline__ int div_small(int n, int d, float inv) {
-  int q = static_cast<int>(static_cast<float>(n) * inv);
-  if (q * d > n) q--;
-  else if ((q + 1) * d <= n) q++;
-  return q;
+// n / d for 0 <= n < 2^24 (wave-uniform: scalar multiply-high and shift).
+__device__ __forceinline__ int div_exact(int n, const Divisor& d) {
+  return static_cast<int>(__umulhi(static_cast<uint32_t>(n) << 8, d.m) >> d.l);
 }
 __device__ __forceinline__ bool limited(const MeLevelArgs& a, int r) { return a.kmap && a.level <= 1 && r > 0; }
 __device__ __forceinline__ bool moded(const MeLevelArgs& a) { return a.smap && a.level <= 1; }
@@ -341,7 +338,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     hadamard_operands(lane, hh, nhh);
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     // Row g of the block's tile (tx, ty): word 2 (4 ty + g) + tx of the block.
-    const uint32_t crow = static_cast<uint32_t>(__shfl(static_cast<int>(o.mine), 2 * (4 * ty + g) + tx, 64));
+    const uint32_t crow = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(4 * (2 * (4 * ty + g) + tx), static_cast<int>(o.mine)));
     const f32x4 ccur = mfma16(hh, row_halves(crow), zero);
     int odx[3], ody[3];
 #pragma unroll
@@ -434,14 +431,22 @@ __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
   // reference limits thin out the far ones, and a workgroup's 4 waves search neighbouring blocks.
   const int bi = kOne ? 0 : batch_item(batch, g);
   const MeLevelArgs& a = batch.item[bi];
-  const int u = g - batch.first[bi], grp = div_small(u, a.nrefs, a.inv_nrefs), r = u - grp * a.nrefs;
+  const int u = g - batch.first[bi], grp = div_exact(u, a.by_nrefs), r = u - grp * a.nrefs;
   const int block = a.row0 * a.cols + 4 * grp + wave;
   if (block >= (a.row0 + a.nrun) * a.cols) return;  // whole wave
-  const int by = div_small(block, a.cols, a.inv_cols);
+  const int by = div_exact(block, a.by_cols);
   search_block(a, BlockPos{r, block - by * a.cols, by}, lane, wins + wave * kWaveLds, wins + wave * kWaveLds + kWinB);
 }
 
 }  // namespace
+
+Divisor make_divisor(int d) {
+  if (d < 1 || d >= (1 << 16)) throw std::logic_error("divisor outside 1 .. 2^16 - 1");
+  int l = 0;
+  while ((1 << l) < d) l++;
+  const unsigned long long p = 1ull << (24 + l);
+  return Divisor{static_cast<uint32_t>((p + static_cast<unsigned long long>(d) - 1) / static_cast<unsigned long long>(d)), l};
+}
 
 void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a) {
   MeLevelArgs x = a;
@@ -449,7 +454,10 @@ void add_me_level(Batch<MeLevelArgs>& b, const MeLevelArgs& a) {
   if (x.row0 < 0 || x.row0 + x.nrun > a.rows) throw std::logic_error("motion search rows outside the level");
   const int units = a.nrefs * ((x.nrun * a.cols + 3) / 4);
   if (units <= 0) return;
-  x.inv_nrefs = 1.f / static_cast<float>(a.nrefs), x.inv_cols = 1.f / static_cast<float>(a.cols);
+  // The kernel divides unit numbers (< units) and block numbers (< rows cols + 4) exactly below 2^24.
+  if (units >= (1 << 24) || static_cast<long long>(a.rows) * a.cols + 4 >= (1 << 24) || a.cols >= (1 << 16))
+    throw std::logic_error("motion search: level too large for the launch's unit numbering");
+  x.by_nrefs = make_divisor(a.nrefs), x.by_cols = make_divisor(a.cols);
   b.add(x, units);
 }
 void launch_me_level(const Batch<MeLevelArgs>& b, hipStream_t s) {
