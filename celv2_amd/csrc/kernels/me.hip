@@ -12,10 +12,11 @@
 //      position each. A position's key is (SAD << 10) + (penalty << 10 | window << 9 | dy * 17 + dx),
 //      its lane constant from a compile-time table, so the wave minimum (DPP + 4 readlanes) is the
 //      first least cost in dy-major order, window 1 before window 2 at equal cost;
-//   3. level 0: the half-pel then quarter-pel refinement: per step the 9 positions x 4 tiles are 36
-//      columns of three 16x16x16 Hadamard products on the matrix cores, accumulated onto the block's
-//      own transform (hadamard.h); the key (cost << 4 | order) makes the centre win ties as in
-//      me_refine.
+//   3. level 0: the half-pel then quarter-pel refinement: the half-pel step's 9 positions x 4 tiles
+//      are 36 columns of three 16x16x16 Hadamard products on the matrix cores, accumulated onto the
+//      block's own transform (hadamard.h), the quarter-pel step's 8 new positions two products (its
+//      centre is the half-pel winner, already scored); the key (cost << 4 | order) makes the centre
+//      win ties as in me_refine.
 #include <stdexcept>
 
 #include "common.h"
@@ -217,6 +218,90 @@ __device__ __forceinline__ Opening open_block(const MeLevelArgs& a, const BlockP
   return o;
 }
 
+// The refinement's per-lane constants: lane (j, g) supplies row g of tile t = j & 3 (t = 2 ty + tx)
+// of product column j; off = that row's offset within the block; nhh = -HH as the A operand; ccur =
+// the block's own tiles' transforms, accumulated onto.
+struct RefineLane {
+  int j, g, tx, ty, off;
+  half4 nhh;
+  f32x4 ccur;
+};
+
+// One me_refine step of kStep quarter pels around (qcx, qcy), which it moves to the winner; returns
+// the winner's key (cost << 4 | order). Column j' = 16 c + j of product c is position
+// o = first + (j' >> 2), tile j' & 3. kStep 2 scores positions 0..8 in three products (9..11 padding,
+// reading position 8's rows); kStep 1 scores positions 1..8 in two, the centre's key given
+// (centre_key: its cost << 4).
+template <int kStep>
+__device__ __forceinline__ uint32_t refine_step(const MeLevelArgs& a, const BPlane& rb, const DPlane& ref, int bx, int by, const RefineLane& L,
+                                                int& qcx, int& qcy, uint32_t centre_key) {
+  constexpr int kProducts = kStep == 2 ? 3 : 2, kFirst = kStep == 2 ? 0 : 1;
+  int odx[kProducts], ody[kProducts];
+#pragma unroll
+  for (int cc = 0; cc < kProducts; cc++) {
+    const int o = min(kFirst + 4 * cc + (L.j >> 2), 8);
+    odx[cc] = static_cast<int>((kRefDx >> (2 * o)) & 3u) - 1, ody[cc] = static_cast<int>((kRefDy >> (2 * o)) & 3u) - 1;
+  }
+  // The products' rows first, then the products back to back, then their sums (every partial sum an
+  // integer below 2^24: any order is exact). Row offsets: when no position of the step moves a window
+  // origin into its clamp (checked once for the wave), the offset is the centre's, plus the lane's
+  // row and tile, plus the position's phase and whole-sample carry; otherwise each origin is clamped
+  // as at_q() reads.
+  int off[kProducts];
+  const bool inner = 8 * bx + ((qcx - kStep) >> 2) >= -kBorder && 8 * bx + ((qcx + kStep) >> 2) <= ref.w + kBorder - 8 &&
+                     8 * by + ((qcy - kStep) >> 2) >= -kBorder && 8 * by + 4 + ((qcy + kStep) >> 2) <= ref.h + kBorder - 4;
+  if (inner) {
+    const int centre = corner_offset(rb, 8 * by + (qcy >> 2), 8 * bx + (qcx >> 2)) + L.off;
+#pragma unroll
+    for (int cc = 0; cc < kProducts; cc++) {
+      const int ex = (qcx & 3) + odx[cc] * kStep, ey = (qcy & 3) + ody[cc] * kStep;  // -2 .. 5
+      off[cc] = centre + phase_offset((ey & 3) * 4 + (ex & 3), a.phase_bytes) + __mul24(ey >> 2, rb.stride) + (ex >> 2);
+    }
+  } else {
+#pragma unroll
+    for (int cc = 0; cc < kProducts; cc++) {
+      const int qx = qcx + odx[cc] * kStep, qy = qcy + ody[cc] * kStep;
+      const int X = clamp_origin(8 * bx + (qx >> 2), ref.w, 8) + 4 * L.tx;
+      const int Y = clamp_origin(8 * by + 4 * L.ty + (qy >> 2), ref.h, 4) + L.g;
+      off[cc] = phase_offset((qy & 3) * 4 + (qx & 3), a.phase_bytes) + corner_offset_lane(rb, Y, X);
+    }
+  }
+  uint32_t prow[kProducts];
+#pragma unroll
+  for (int cc = 0; cc < kProducts; cc++) {
+    uint32_t d0, d1;
+    bload8(rb, off[cc] & ~3, 0, d0, d1);
+    prow[cc] = __builtin_amdgcn_alignbyte(d1, d0, off[cc] & 3);
+  }
+  f32x4 m[kProducts];
+#pragma unroll
+  for (int cc = 0; cc < kProducts; cc++) m[cc] = mfma16(L.nhh, row_halves(prow[cc]), L.ccur);
+  float sv[kProducts];
+#pragma unroll
+  for (int cc = 0; cc < kProducts; cc++) {
+    float v = abs_sum4(m[cc]);
+    v += as_f32(dpp<kDppXor1>(as_u32f(v)));  // the position's 4 tiles (lanes 4o' .. 4o'+3)
+    sv[cc] = v + as_f32(dpp<kDppXor2>(as_u32f(v)));
+  }
+  // A position's 4 rows of coefficients: products 0 and 1 in one pass (rows 0 and 2 of the result
+  // hold product 0's totals, rows 1 and 3 product 1's), product 2 in another.
+  const float s01 = sum_halves32(sum_rows16(sv[0], sv[1]), sum_rows16(sv[0], sv[1]));
+  const int o01 = kFirst + 4 * (L.g & 1) + (L.j >> 2);
+  const uint32_t c01 = static_cast<uint32_t>(s01) + (o01 ? kQpelPenalty : 0);
+  uint32_t best = (c01 << 4) | static_cast<uint32_t>(o01);
+  if constexpr (kProducts == 3) {
+    const float s2 = sum_halves32(sum_rows16(sv[2], sv[2]), sum_rows16(sv[2], sv[2]));
+    const int o2 = 8 + (L.j >> 2);
+    const uint32_t c2 = static_cast<uint32_t>(s2) + kQpelPenalty;
+    if (o2 < 9) best = min(best, (c2 << 4) | static_cast<uint32_t>(o2));
+  }
+  const uint32_t key = scalar_min(wave_min_quads_u32(best), centre_key);  // best is equal over each quad
+  const int ob = static_cast<int>(key & 15);
+  qcx += (static_cast<int>((kRefDx >> (2 * ob)) & 3u) - 1) * kStep;
+  qcy += (static_cast<int>((kRefDy >> (2 * ob)) & 3u) - 1) * kStep;
+  return key;
+}
+
 // Block (q.bx, q.by) against reference q.r of one launch item, by one wave; win1 / win2 are the
 // wave's two LDS windows.
 __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPos& q, int lane, uint32_t* win1, uint32_t* win2) {
@@ -330,83 +415,22 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   int mvx = (second ? p2x : p1x) + ip % 17 - kRange, mvy = (second ? p2y : p1y) + ip / 17 - kRange;
 
   if (a.level0) {
-    // me_refine (src/gpu/inter_common.h:135-154) from the integer MV, in quarter pels. Column
-    // j' = 16 c + j of product c is position o = j' >> 2, tile t = j' & 3 (t = 2 ty + tx); lane
-    // (j, g) supplies row g of that tile. Positions 9..11 are padding (never chosen).
+    // me_refine (src/gpu/inter_common.h:135-154) from the integer MV, in quarter pels: the half-pel
+    // step scores its 9 positions, the quarter-pel step the 8 around the half-pel winner, whose
+    // cost without the step penalty is its SATD from the half-pel step (the same position read
+    // the same way).
     const int j = lane & 15, g = lane >> 4, t = j & 3, tx = t & 1, ty = t >> 1;
     half4 hh, nhh;
     hadamard_operands(lane, hh, nhh);
     const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
     // Row g of the block's tile (tx, ty): word 2 (4 ty + g) + tx of the block.
     const uint32_t crow = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(4 * (2 * (4 * ty + g) + tx), static_cast<int>(o.mine)));
-    const f32x4 ccur = mfma16(hh, row_halves(crow), zero);
-    int odx[3], ody[3];
-#pragma unroll
-    for (int cc = 0; cc < 3; cc++) {
-      // Padding columns (positions 9..11) read position 8's rows: the same cache lines, results unused.
-      const int o = min(4 * cc + (j >> 2), 8);
-      odx[cc] = static_cast<int>((kRefDx >> (2 * o)) & 3u) - 1, ody[cc] = static_cast<int>((kRefDy >> (2 * o)) & 3u) - 1;
-    }
-    // The lane's row and tile within the block, as an offset (rows within the bordered range).
-    const int lane_off = static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(4 * ty + g))) + 4 * tx;
+    const RefineLane L{j, g, tx, ty, static_cast<int>(umul24_sv(static_cast<uint32_t>(rb.stride), static_cast<uint32_t>(4 * ty + g))) + 4 * tx,
+                       nhh, mfma16(hh, row_halves(crow), zero)};
     int qcx = 4 * mvx, qcy = 4 * mvy;
-#pragma unroll
-    for (int step = 2; step >= 1; step--) {
-      // The three products' rows first, then the three products back to back, then their sums
-      // (every partial sum an integer below 2^24: any order is exact).
-      // Row offsets. When no position of the step moves a window origin into its clamp (checked
-      // once for the wave), the offset is the centre's, plus the lane's row and tile, plus the
-      // position's phase and whole-sample carry; otherwise each origin is clamped as at_q() reads.
-      int off[3];
-      const bool inner = 8 * bx + ((qcx - step) >> 2) >= -kBorder && 8 * bx + ((qcx + step) >> 2) <= ref.w + kBorder - 8 &&
-                         8 * by + ((qcy - step) >> 2) >= -kBorder && 8 * by + 4 + ((qcy + step) >> 2) <= ref.h + kBorder - 4;
-      if (inner) {
-        const int centre = corner_offset(rb, 8 * by + (qcy >> 2), 8 * bx + (qcx >> 2)) + lane_off;
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) {
-          const int ex = (qcx & 3) + odx[cc] * step, ey = (qcy & 3) + ody[cc] * step;  // -2 .. 5
-          off[cc] = centre + phase_offset((ey & 3) * 4 + (ex & 3), a.phase_bytes) + __mul24(ey >> 2, rb.stride) + (ex >> 2);
-        }
-      } else {
-#pragma unroll
-        for (int cc = 0; cc < 3; cc++) {
-          const int qx = qcx + odx[cc] * step, qy = qcy + ody[cc] * step;
-          const int X = clamp_origin(8 * bx + (qx >> 2), ref.w, 8) + 4 * tx;
-          const int Y = clamp_origin(8 * by + 4 * ty + (qy >> 2), ref.h, 4) + g;
-          off[cc] = phase_offset((qy & 3) * 4 + (qx & 3), a.phase_bytes) + corner_offset_lane(rb, Y, X);
-        }
-      }
-      uint32_t prow[3];
-#pragma unroll
-      for (int cc = 0; cc < 3; cc++) {
-        uint32_t d0, d1;
-        bload8(rb, off[cc] & ~3, 0, d0, d1);
-        prow[cc] = __builtin_amdgcn_alignbyte(d1, d0, off[cc] & 3);
-      }
-      f32x4 m[3];
-#pragma unroll
-      for (int cc = 0; cc < 3; cc++) m[cc] = mfma16(nhh, row_halves(prow[cc]), ccur);
-      float sv[3];
-#pragma unroll
-      for (int cc = 0; cc < 3; cc++) {
-        float v = abs_sum4(m[cc]);
-        v += as_f32(dpp<kDppXor1>(as_u32f(v)));  // the position's 4 tiles (lanes 4o' .. 4o'+3)
-        sv[cc] = v + as_f32(dpp<kDppXor2>(as_u32f(v)));
-      }
-      // Its 4 rows of coefficients: products 0 and 1 in one pass (rows 0 and 2 of the result hold
-      // product 0's totals, rows 1 and 3 product 1's), product 2 in another.
-      const float s01 = sum_halves32(sum_rows16(sv[0], sv[1]), sum_rows16(sv[0], sv[1]));
-      const float s2 = sum_halves32(sum_rows16(sv[2], sv[2]), sum_rows16(sv[2], sv[2]));
-      const int o01 = 4 * (g & 1) + (j >> 2), o2 = 8 + (j >> 2);
-      const uint32_t c01 = static_cast<uint32_t>(s01) + (o01 ? kQpelPenalty : 0);
-      const uint32_t c2 = static_cast<uint32_t>(s2) + kQpelPenalty;
-      uint32_t best = (c01 << 4) | static_cast<uint32_t>(o01);
-      if (o2 < 9) best = min(best, (c2 << 4) | static_cast<uint32_t>(o2));
-      const uint32_t key = wave_min_quads_u32(best);  // best is equal over each quad (lanes 4o' .. 4o' + 3)
-      const int ob = static_cast<int>(key & 15);
-      qcx += (static_cast<int>((kRefDx >> (2 * ob)) & 3u) - 1) * step;
-      qcy += (static_cast<int>((kRefDy >> (2 * ob)) & 3u) - 1) * step;
-    }
+    const uint32_t k2 = refine_step<2>(a, rb, ref, bx, by, L, qcx, qcy, 0xffffffffu);
+    const uint32_t satd = (k2 >> 4) - ((k2 & 15) ? static_cast<uint32_t>(kQpelPenalty) : 0u);
+    refine_step<1>(a, rb, ref, bx, by, L, qcx, qcy, satd << 4);
     mvx = qcx, mvy = qcy;
   }
   if (lane == 0) {
