@@ -40,11 +40,8 @@ constexpr int kCopyWords = 6;               // words per copied row: words 0..5 
 constexpr int kRowWords = 4 * kCopyWords + 1;  // 25
 constexpr int kWinWords = kWin * kRowWords;    // 600
 constexpr int kWinB = 624;                     // the second window's start: 624 = 16 (mod 32)
-// A wave's area: both windows; a position past the range (key forced to the maximum) reads row 24,
-// past its window's rows: inside the gap before the second window, or up to 24 words into the next
-// wave's area (the workgroup's array has that much to spare after the last wave's).
+// A wave's area: both windows (24 rows each; the second 24 words after the first's end).
 constexpr int kWaveLds = kWinB + kWinWords;    // 1224
-constexpr int kLdsSpare = 32;
 
 // Stages a window: lane (row group q = lane >> 3, word d = lane & 7) copies word d of rows q, q + 8
 // and q + 16 into every copy, from the aligned row's words d and d + 1 (rows from the window's first
@@ -67,54 +64,54 @@ __device__ __forceinline__ void stage_window(const BPlane& rb, int base, int off
 }
 
 // Lane constants of the search, computed per block (a table of them cost more in loads than the
-// arithmetic: the vector-memory path is the busier one):
-//   t[k], k < 9: the keys of the lane's main positions (dy group g = lane >> 5, window w =
-//     (lane >> 4) & 1, dx = lane & 15, dy = 9 g + k): (kPelPenalty (|dx - 8| + |dy - 8|)) << 10 |
-//     w << 9 | dy * 17 + dx, or kNoPosition past the range (dy 17). Along k the penalty falls by
-//     kPelPenalty while dy < 8 (group 0) and rises after (group 1): each key is the last plus a step;
-//   t[9]: the leftover position of lanes 0..33 (dx 16, window lane >= 17, dy lane % 17), else none;
-//   t[10..14]: dx, w, the first row's word offset of the main positions; w and the first row's word
-//     offset of the leftover one.
-constexpr uint32_t kNoPosition = 0x7fffffffu;  // above any key (SAD << 10 < 2^24)
-__device__ __forceinline__ void search_lanes(int lane, uint32_t (&t)[16]) {
+// arithmetic). Lane (dy group g = lane >> 5, window w = (lane >> 4) & 1, column dx = lane & 15)
+// scores the 8 main positions (dx, 9 g + k), k < 8, from the 15 rows of its strip, and one extra
+// position (ex, ey): (dx, 8) for g = 0, except (16, 16) in place of the centre (8, 8) for dx = 8, and
+// (16, dx) for g = 1. Over the 64 lanes that is every position of both windows but the two centres,
+// whose SADs the centre ranking has already: 2 x 288 = 9 x 64.
+//   t[k], k < 8: the main keys (kPelPenalty (|dx - 8| + |dy - 8|)) << 10 | w << 9 | dy * 17 + dx;
+//     along k the penalty falls by kPelPenalty while dy < 8 (group 0) and rises after (group 1):
+//     each key is the last plus a step;
+//   t[8]: the extra position's key;
+//   t[9]: w; t[10]: the strip's first word, dx + 9 g kRowWords; t[11]: the extra position's first
+//     word, ex + ey kRowWords (both before the window's byte shift: a sample at byte offset o of row y
+//     is word o + y kRowWords of the copies, words o and o + 4 holding its 8 bytes).
+__device__ __forceinline__ void search_lanes(int lane, uint32_t (&t)[12]) {
   const int g = lane >> 5, w = (lane >> 4) & 1, dx = lane & 15;
   const uint32_t pen0 = static_cast<uint32_t>(kPelPenalty * (abs(dx - kRange) + (g ? 1 : kRange)));
   t[0] = pen0 << 10 | static_cast<uint32_t>(w) << 9 | static_cast<uint32_t>(153 * g + dx);
   const uint32_t step = g ? (static_cast<uint32_t>(kPelPenalty) << 10) + 17u : 17u - (static_cast<uint32_t>(kPelPenalty) << 10);
 #pragma unroll
-  for (int k = 1; k < 9; k++) t[k] = t[k - 1] + step;
-  if (g) t[8] = kNoPosition;
-  const int lw = lane >= 17 ? 1 : 0, ldy = lane - 17 * lw;
-  const uint32_t lpen = static_cast<uint32_t>(kPelPenalty * (kRange + abs(ldy - kRange)));
-  t[9] = lane < 34 ? (lpen << 10 | static_cast<uint32_t>(lw) << 9 | static_cast<uint32_t>(17 * ldy + 16)) : kNoPosition;
-  t[10] = static_cast<uint32_t>(dx), t[11] = static_cast<uint32_t>(w), t[12] = static_cast<uint32_t>(9 * kRowWords * g);
-  t[13] = static_cast<uint32_t>(lane < 34 ? lw : 0), t[14] = static_cast<uint32_t>(lane < 34 ? kRowWords * ldy : 0), t[15] = 0;
+  for (int k = 1; k < 8; k++) t[k] = t[k - 1] + step;
+  const int ex = g || dx == kRange ? 2 * kRange : dx, ey = g ? dx : dx == kRange ? 2 * kRange : kRange;
+  const uint32_t xpen = static_cast<uint32_t>(kPelPenalty * (abs(ex - kRange) + abs(ey - kRange)));
+  t[8] = xpen << 10 | static_cast<uint32_t>(w) << 9 | static_cast<uint32_t>(17 * ey + ex);
+  t[9] = static_cast<uint32_t>(w), t[10] = static_cast<uint32_t>(dx + 9 * kRowWords * g), t[11] = static_cast<uint32_t>(ex + kRowWords * ey);
 }
 
 // The lane's least key over its positions of the two staged windows A and B (LDS word addresses,
 // byte shifts shA / shB of their first samples; B may be A).
-__device__ __forceinline__ uint32_t score_pair(const uint32_t (&t)[16], const uint32_t* winA, const uint32_t* winB, int shA, int shB,
-                                               const uint32_t c[16], int lane) {
-  const uint32_t* row = (t[11] ? winB : winA) + ((t[11] ? shB : shA) + static_cast<int>(t[10])) + t[12];
-  uint32_t w[16][2];
+__device__ __forceinline__ uint32_t score_pair(const uint32_t (&t)[12], const uint32_t* winA, const uint32_t* winB, int shA, int shB,
+                                               const uint32_t c[16]) {
+  const uint32_t* win = t[9] ? winB : winA;
+  const int sh = t[9] ? shB : shA;
+  const uint32_t* row = win + sh + t[10];
+  uint32_t w[15][2];
 #pragma unroll
-  for (int rr = 0; rr < 16; rr++) w[rr][0] = row[rr * kRowWords], w[rr][1] = row[rr * kRowWords + 4];
-  uint32_t best = kNoPosition;
+  for (int rr = 0; rr < 15; rr++) w[rr][0] = row[rr * kRowWords], w[rr][1] = row[rr * kRowWords + 4];
+  uint32_t best = 0xffffffffu;
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
+  for (int k = 0; k < 8; k++) {
     uint32_t acc = 0;
 #pragma unroll
     for (int i = 0; i < 8; i++) acc = sad4(c[2 * i + 1], w[k + i][1], sad4(c[2 * i], w[k + i][0], acc));
     best = min(best, (acc << 10) + t[k]);
   }
-  if (lane < 34) {  // column 16 of both windows
-    const uint32_t* lrow = (t[13] ? winB : winA) + ((t[13] ? shB : shA) + 16) + t[14];
-    uint32_t acc = 0;
+  const uint32_t* xrow = win + sh + t[11];
+  uint32_t acc = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) acc = sad4(c[2 * i + 1], lrow[i * kRowWords + 4], sad4(c[2 * i], lrow[i * kRowWords], acc));
-    best = min(best, (acc << 10) + t[9]);
-  }
-  return best;
+  for (int i = 0; i < 8; i++) acc = sad4(c[2 * i + 1], xrow[i * kRowWords + 4], sad4(c[2 * i], xrow[i * kRowWords], acc));
+  return min(best, (acc << 10) + t[8]);
 }
 
 // The narrow form of the two full searches (the pruned form's search mode): me_search with range
@@ -125,6 +122,7 @@ __device__ __forceinline__ uint32_t score_pair(const uint32_t (&t)[16], const ui
 // form: (SAD << 10) + (penalty << 10 | w << 9 | (dy + kRange - R) * 17 + dx + kRange - R), so the wave
 // minimum is the first least cost in dy-major order, window 1 before window 2 at equal cost, and
 // decodes as the full form's keys do.
+constexpr uint32_t kNoPosition = 0x7fffffffu;  // above any key (SAD << 10 < 2^24)
 constexpr int kNarrowSide = 2 * kNarrowRange + 1;                         // positions per window side
 constexpr int kNarrowRows = kNarrowRange <= 2 ? 1 : kNarrowRange == 3 ? 2 : 3;  // positions per lane
 constexpr int kNarrowGroups = (kNarrowSide + kNarrowRows - 1) / kNarrowRows;
@@ -324,17 +322,19 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   }
 
   int p1x = 0, p1y = 0, p2x = 0, p2y = 0;
+  uint32_t s1, s2 = 0;  // the searched centres' SADs
   bool two = false;
-  if (a.coarser[r]) {
+  {
     // Centres k = 0..4: the coarser MV under the block and its left, right, above and below
     // neighbours (clamped into the coarser field), doubled; k = 5: zero (pack_mv(0, 0) = 0). Uniform
-    // addresses, and the coarser field is the previous launch's output: scalar loads.
+    // addresses, and the coarser field is the previous launch's output: scalar loads. The top level
+    // has no coarser field: every centre is zero, one window.
     uint32_t m[6];
 #pragma unroll
     for (int k = 0; k < 5; k++) {
       const int ox = k == 1 ? -1 : k == 2 ? 1 : 0, oy = k == 3 ? -1 : k == 4 ? 1 : 0;
       const int qx = clamp_uniform(bx / 2 + ox, 0, a.pcols - 1), qy = clamp_uniform(by / 2 + oy, 0, a.prows - 1);
-      m[k] = a.coarser[r][qy * a.pcols + qx];
+      m[k] = a.coarser[r] ? a.coarser[r][qy * a.pcols + qx] : 0u;
     }
     m[5] = 0u;
     // Each centre's SAD: lanes 8k .. 8k+7, one block row each (lanes 48.. repeat centre 0, unread).
@@ -372,10 +372,11 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     uint32_t k2 = 0xffffffffu;
 #pragma unroll
     for (int k = 0; k < 6; k++) k2 = m[k] != m1 ? min(k2, key[k]) : k2;
-    p1x = 2 * mv_x(m1), p1y = 2 * mv_y(m1);
+    p1x = 2 * mv_x(m1), p1y = 2 * mv_y(m1), s1 = k1 >> 3;
     two = k2 != 0xffffffffu;
     if (two) {
       const int b2 = static_cast<int>(k2 & 7);
+      s2 = k2 >> 3;
       uint32_t m2 = m[0];
 #pragma unroll
       for (int k = 1; k < 6; k++) m2 = b2 == k ? m[k] : m2;
@@ -404,10 +405,14 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
 
   // A window's first sample sits at byte X & 3 of its aligned row (rows are a multiple of 64 bytes
   // apart, the border 64 samples wide). With one centre, the second window's lanes score the first
-  // window again (its keys lose every tie to the first window's).
-  uint32_t tk[16];
+  // window again (its keys lose every tie to the first window's). The centres (8, 8) come in with
+  // the SADs the ranking found (the same samples: both origins clamp to reads the border makes equal
+  // to the reference's per-sample clamping) and no penalty.
+  uint32_t tk[12];
   search_lanes(lane, tk);
-  key = wave_min_u32(score_pair(tk, win1, two ? win2 : win1, X1 & 3, two ? X2 & 3 : X1 & 3, c, lane));
+  key = wave_min_u32(score_pair(tk, win1, two ? win2 : win1, X1 & 3, two ? X2 & 3 : X1 & 3, c));
+  key = scalar_min(key, s1 << 10 | (kRange * 17 + kRange));
+  if (two) key = scalar_min(key, s2 << 10 | 1u << 9 | (kRange * 17 + kRange));
   }
   const uint32_t best_cost = key >> 10;
   const int ip = static_cast<int>(key & 511);
@@ -444,7 +449,7 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
 // fields then sit at constant kernel-argument offsets: no per-field address arithmetic).
 template <bool kOne>
 __global__ void __launch_bounds__(256) k_me_level(Batch<MeLevelArgs> batch) {
-  __shared__ uint32_t wins[4 * kWaveLds + kLdsSpare];
+  __shared__ uint32_t wins[4 * kWaveLds];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform: SGPR addressing
   const int total = batch.total();
