@@ -315,10 +315,13 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
   // The block, every lane holding every word: scalar loads of its 8 rows (its address is uniform and
   // the kernel writes no source plane), no VALU.
   uint32_t c[16];
+  {
+    const uint8_t* base = a.cur.p + static_cast<ptrdiff_t>(8 * by) * a.cur.stride + 8 * bx;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    const uint32_t* row = reinterpret_cast<const uint32_t*>(a.cur.p + static_cast<ptrdiff_t>(8 * by + i) * a.cur.stride + 8 * bx);
-    c[2 * i] = row[0], c[2 * i + 1] = row[1];
+    for (int i = 0; i < 8; i++) {
+      const uint32_t* row = reinterpret_cast<const uint32_t*>(base + static_cast<uint32_t>(i * a.cur.stride));
+      c[2 * i] = row[0], c[2 * i + 1] = row[1];
+    }
   }
 
   int p1x = 0, p1y = 0, p2x = 0, p2y = 0;
@@ -329,14 +332,16 @@ __device__ __forceinline__ void search_block(const MeLevelArgs& a, const BlockPo
     // neighbours (clamped into the coarser field), doubled; k = 5: zero (pack_mv(0, 0) = 0). Uniform
     // addresses, and the coarser field is the previous launch's output: scalar loads. The top level
     // has no coarser field: every centre is zero, one window.
-    uint32_t m[6];
-#pragma unroll
-    for (int k = 0; k < 5; k++) {
-      const int ox = k == 1 ? -1 : k == 2 ? 1 : 0, oy = k == 3 ? -1 : k == 4 ? 1 : 0;
-      const int qx = clamp_uniform(bx / 2 + ox, 0, a.pcols - 1), qy = clamp_uniform(by / 2 + oy, 0, a.prows - 1);
-      m[k] = a.coarser[r] ? a.coarser[r][qy * a.pcols + qx] : 0u;
+    uint32_t m[6] = {0u, 0u, 0u, 0u, 0u, 0u};
+    if (const uint32_t* cf = a.coarser[r]) {
+      // Each centre's cell clamped into the coarser field; one 64-bit base, 32-bit byte offsets
+      // (scalar loads with an offset register).
+      const auto at = [&](int ox, int oy) {
+        const int qx = clamp_uniform(bx / 2 + ox, 0, a.pcols - 1), qy = clamp_uniform(by / 2 + oy, 0, a.prows - 1);
+        return *reinterpret_cast<const uint32_t*>(reinterpret_cast<const uint8_t*>(cf) + static_cast<uint32_t>(4 * (qy * a.pcols + qx)));
+      };
+      m[0] = at(0, 0), m[1] = at(-1, 0), m[2] = at(1, 0), m[3] = at(0, -1), m[4] = at(0, 1);
     }
-    m[5] = 0u;
     // Each centre's SAD: lanes 8k .. 8k+7, one block row each (lanes 48.. repeat centre 0, unread).
     // Group k's lanes are row k >> 1 of the wave, banks 0-1 (k even) or 2-3 (k odd): each takes its
     // centre's MV by one masked DPP move.
