@@ -40,7 +40,7 @@ SAD_PEAK_TPX, SAD_MEASURED_TPX = 256 * 64 * 4 * 2.4e9 / 1e12, 125.4
 # HBM bytes the motion search's level-0 launch (the dominant kernel launch) moves per 1080p frame with
 # 7 references (rocprofv3 FETCH_SIZE and WRITE_SIZE per dispatch, scaled by a 64 MiB device copy:
 # profiles/me_traffic_1080p10_7refs.md).
-ME_TRAFFIC_BYTES = 341.2e6
+ME_TRAFFIC_BYTES = 342.2e6
 # HBM bytes of one forward-transform call over a 1080p frame's candidate blocks (FETCH_SIZE + WRITE_SIZE,
 # scaled the same way: profiles/me_traffic_1080p10_7refs.md).
 TX_TRAFFIC_BYTES = 142.0e6
