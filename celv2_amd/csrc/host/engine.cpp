@@ -412,6 +412,15 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
   return guarded([&] {
     if (width <= 0 || height <= 0) throw Failure(CELV2_ERR_SIZE, "HipEngine: frame size must be positive");
     if (sb_size != 64) throw Failure(CELV2_ERR_SIZE, "HipEngine: only 64x64 superblocks are supported");
+    // The kernels address a reference's 16 quarter-pel planes with 32-bit signed byte offsets from
+    // its level-0 corner (buffer resources of 2^31 - 1 bytes): all 16 must span less than that.
+    // Checked before any device call.
+    {
+      int stride;
+      const unsigned long long span = static_cast<unsigned long long>(plane_bytes((width + 7) & ~7, (height + 7) & ~7, &stride)) * kPhases;
+      if (span > 0x7fffffffull)
+        throw Failure(CELV2_ERR_SIZE, "HipEngine: frame too large (a reference's 16 quarter-pel planes must span under 2 GiB; about 11,500 x 11,500 at most)");
+    }
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw Failure(CELV2_ERR_DEVICE, "HipEngine: no HIP device");
     if (device < 0 || device >= count) throw Failure(CELV2_ERR_DEVICE, "HipEngine: no such HIP device");
@@ -420,14 +429,6 @@ int celv2hip_create(int width, int height, int sb_size, int device, celv2hip_eng
     e->width = width, e->height = height, e->sb_size = sb_size, e->device = device;
     e->pw = (width + 7) & ~7, e->ph = (height + 7) & ~7;
     e->sb_cols = (width + 63) / 64, e->sb_rows = (height + 63) / 64;
-    // The kernels address a reference's 16 quarter-pel planes with 32-bit signed byte offsets from
-    // its level-0 corner (buffer resources of 2^31 - 1 bytes): all 16 must span less than that.
-    {
-      int stride;
-      const unsigned long long span = static_cast<unsigned long long>(plane_bytes(e->pw, e->ph, &stride)) * kPhases;
-      if (span > 0x7fffffffull)
-        throw Failure(CELV2_ERR_SIZE, "HipEngine: frame too large (a reference's 16 quarter-pel planes must span under 2 GiB; about 11,500 x 11,500 at most)");
-    }
     hipDeviceProp_t prop;
     check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
     e->name = prop.name[0] ? prop.name : prop.gcnArchName;

@@ -53,6 +53,23 @@ def test_engine_fails_cleanly_without_a_gpu():
     assert "HIP" in engine.last_error()
 
 
+def test_oversize_frames_are_rejected_before_any_device_call():
+    """A reference's 16 quarter-pel planes must span under 2^31 bytes (the kernels' 32-bit buffer
+    offsets): 12000x12000 and 16384x8192 are refused with the size error, with or without a GPU, and
+    11264x11264 (under the bound) gets past the check (to the device error where there is none)."""
+    from celv2_amd import engine
+
+    lib = engine.lib()
+    h = ctypes.c_void_p()
+    for w, hh in ((12000, 12000), (16384, 8192)):
+        assert lib.celv2hip_create(w, hh, 64, 0, ctypes.byref(h)) == 2, (w, hh)  # CELV2_ERR_SIZE
+        assert not h.value
+        assert "too large" in engine.last_error()
+    if engine.device_count() == 0:
+        assert lib.celv2hip_create(11264, 11264, 64, 0, ctypes.byref(h)) == 3  # CELV2_ERR_DEVICE
+        assert not h.value
+
+
 def test_record_layout_matches():
     from celv2_amd import engine
     from celv2_amd.records import SB_DTYPE
