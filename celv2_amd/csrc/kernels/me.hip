@@ -5,13 +5,14 @@
 //   1. the six candidate centres' SADs, 8 lanes per centre (one block row each), summed by DPP and
 //      read into scalar registers; the centre ranking is then scalar code;
 //   2. each searched centre's 24 x 24 window is read from the plane once (144 word pairs, 6 per row)
-//      into the wave's LDS area as 4 byte-shifted copies; both windows' 2 x 289 positions are then
-//      scored in one pass by all 64 lanes: lane (window, half of the dy range, dx < 16) takes 16 rows
-//      as aligned words of the copy its offset selects and scores 9 positions with v_sad_u8 against
-//      the block held in SGPRs (scalar loads); lanes 0..33 also score the two windows' column 16, one
-//      position each. A position's key is (SAD << 10) + (penalty << 10 | window << 9 | dy * 17 + dx),
-//      its lane constant from a compile-time table, so the wave minimum (DPP + 4 readlanes) is the
-//      first least cost in dy-major order, window 1 before window 2 at equal cost;
+//      into the wave's LDS area as 4 byte-shifted copies; both windows' positions but their centres
+//      (2 x 288, whose centre SADs step 1 has) are then scored in one pass by all 64 lanes, 9 each:
+//      lane (window, half of the dy range, dx < 16) takes 15 rows as aligned words of the copy its
+//      offset selects for 8 positions, and 8 more rows for one position of column 16 or row 8, with
+//      v_sad_u8 against the block held in SGPRs (scalar loads). A position's key is
+//      (SAD << 10) + (penalty << 10 | window << 9 | dy * 17 + dx), its lane constant computed per
+//      block, so the wave minimum (DPP + 4 readlanes, then the centres' keys on the scalar unit) is
+//      the first least cost in dy-major order, window 1 before window 2 at equal cost;
 //   3. level 0: the half-pel then quarter-pel refinement: the half-pel step's 9 positions x 4 tiles
 //      are 36 columns of three 16x16x16 Hadamard products on the matrix cores, accumulated onto the
 //      block's own transform (hadamard.h), the quarter-pel step's 8 new positions two products (its
